@@ -1,0 +1,357 @@
+"""The checker: a CPU restatement of EMMA's radiation + chemistry hot path
+(``emma_oracle.c``) and, where it was built, the reference's own CPU functions
+(``_ref/libemma_ref_g{1,3}.so``, see ``Makefile``).
+
+TEST INFRASTRUCTURE ONLY.  Imported by ``tests/``, ``__graft_entry__.smoke()``
+and ``bench.py``'s cpu_baseline leg, where it is the thing results are
+compared against (or timed as the CPU baseline) -- never the product path.
+
+Both expose the same level-stepping interface so that tests can drive them
+side by side:
+
+* ``clean_new(l)``                      -- ``clean_new_rad`` (rad_utils.c:2050)
+* ``radsolver(l, dt, aexp, chemonly)``  -- ``RadSolver`` (rad_utils.c:1861)
+* ``coarse_step(aexp)``                 -- the RADSTEP level recursion of
+  ``Advance_level`` (advanceamr.c:243-1300): per level, clean, recurse into
+  the finer level (two sub-steps), then RadSolver.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from emma_amd.synth import Fields
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "_build", "libemma_oracle.so")
+REF_SO = {1: os.path.join(HERE, "_ref", "libemma_ref_g1.so"), 3: os.path.join(HERE, "_ref", "libemma_ref_g3.so")}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+class _OrcFields(ctypes.Structure):
+    _fields_ = [("ncell", ctypes.c_int)] + [(k, _dp) for k in Fields.NAMES]
+
+
+def _cfields(F: Fields) -> _OrcFields:
+    s = _OrcFields()
+    s.ncell = F.ncell
+    for k in Fields.NAMES:
+        a = getattr(F, k)
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+        setattr(s, k, _d(a))
+    return s
+
+
+def build(ref: bool = False) -> None:
+    """Compile the oracle (and the reference driver when asked)."""
+    import subprocess
+
+    targets = ["oracle"] + (["ref"] if ref else [])
+    subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
+
+
+class Oracle:
+    """ctypes front end of emma_oracle.c plus a host-side level state."""
+
+    def __init__(self, path: str = ORACLE_SO):
+        if not os.path.exists(path):
+            build()
+        self.lib = ctypes.CDLL(path)
+        L = self.lib
+        L.orc_eddington.restype = ctypes.c_double
+        L.orc_eddington.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int, ctypes.c_int]
+        L.orc_chem_cells.argtypes = [ctypes.c_void_p, ctypes.POINTER(_OrcFields), ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int, _ip, _ip]
+        L.orc_level_advance.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _ip, ctypes.POINTER(_OrcFields),
+                                        ctypes.POINTER(_OrcFields), ctypes.POINTER(_OrcFields),
+                                        ctypes.POINTER(_OrcFields), ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_int, _ip, _ip]
+        L.orc_level_final.argtypes = [ctypes.c_int, ctypes.c_int, _ip, ctypes.POINTER(_OrcFields),
+                                      ctypes.POINTER(_OrcFields), ctypes.POINTER(_OrcFields), ctypes.c_double,
+                                      _dp, _dp, _dp, _dp]
+        L.orc_level_fluxes.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, ctypes.POINTER(_OrcFields),
+                                       ctypes.POINTER(_OrcFields), ctypes.c_double, _dp]
+        L.orc_eddington_n.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double, _ip, _ip, _dp]
+        L.orc_rates_n.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_e2t_n.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, _dp]
+        L.orc_dtrad.restype = ctypes.c_double
+        L.orc_dtrad.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
+        L.orc_getcellnei.argtypes = [ctypes.c_int, _ip, _ip]
+        self.params = None
+        self.tree = None
+
+    # ---- scalar / vector helpers
+    def eddington(self, fx, fy, fz, e, c, i, j):
+        fx, fy, fz, e = [np.ascontiguousarray(a, dtype=np.float64) for a in (fx, fy, fz, e)]
+        n = fx.size
+        i = np.ascontiguousarray(np.broadcast_to(i, (n,)), dtype=np.int32)
+        j = np.ascontiguousarray(np.broadcast_to(j, (n,)), dtype=np.int32)
+        out = np.empty(n)
+        self.lib.orc_eddington_n(n, _d(fx), _d(fy), _d(fz), _d(e), c, _i(i), _i(j), _d(out))
+        return out
+
+    def rates(self, temp, x, nH):
+        temp, x, nH = [np.ascontiguousarray(np.broadcast_to(a, np.shape(temp)), dtype=np.float64) for a in (temp, x, nH)]
+        n = temp.size
+        outs = [np.empty(n) for _ in range(5)]
+        self.lib.orc_rates_n(n, _d(temp), _d(x), _d(nH), *[_d(o) for o in outs])
+        return dict(zip(("alpha_a", "alpha_b", "beta", "lambda", "tcool"), outs))
+
+    def e2t(self, params, nhplus, eint, nh, aexp=1.0):
+        cp = params.to_c()
+        nhplus, eint, nh = [np.ascontiguousarray(a, dtype=np.float64) for a in (nhplus, eint, nh)]
+        out = np.empty(nhplus.size)
+        self.lib.orc_e2t_n(ctypes.byref(cp), nhplus.size, _d(nhplus), _d(eint), _d(nh), aexp, _d(out))
+        return out
+
+    def getcellnei(self, icell):
+        vn = np.zeros(6, np.int32)
+        vc = np.zeros(6, np.int32)
+        self.lib.orc_getcellnei(icell, _i(vn), _i(vc))
+        return vn.tolist(), vc.tolist()
+
+    def dt_rad(self, params, level, aexp=1.0):
+        cp = params.to_c()
+        return self.lib.orc_dtrad(level, aexp, ctypes.byref(cp))
+
+    def chem_cells(self, params, F: Fields, dtnew, aexp=1.0, chemonly=0):
+        """chemrad on every cell of F (all treated as leaves); returns
+        (new Fields, nsub per cell, status, bad cell)."""
+        cp = params.to_c()
+        out = F.copy()
+        cf = _cfields(out)
+        nsub = np.zeros(out.ncell, np.int32)
+        bad = np.full(1, -1, np.int32)
+        st = self.lib.orc_chem_cells(ctypes.byref(cp), ctypes.byref(cf), dtnew, aexp, chemonly, _i(nsub), _i(bad))
+        return out, nsub, st, int(bad[0])
+
+    def level_fluxes(self, params, level, F: Fields, coarse: Fields | None, aexp=1.0):
+        L = self.tree.level(level) if self.tree is not None else None
+        return self._fluxes(params.ngrp, L, F, coarse, params.cloc(aexp))
+
+    def _fluxes(self, G, L, F, coarse, c):
+        out = np.zeros(L.ncell * 30 * G)
+        cf = _cfields(F)
+        cc = _cfields(coarse) if coarse is not None else None
+        nei = np.ascontiguousarray(L.nei_oct.reshape(-1), dtype=np.int32)
+        ch = np.ascontiguousarray(L.child, dtype=np.int32)
+        st = self.lib.orc_level_fluxes(G, L.noct, _i(nei), _i(ch), ctypes.byref(cf),
+                                       ctypes.byref(cc) if cc is not None else None, c, _d(out))
+        if st != 0:
+            raise RuntimeError(f"orc_level_fluxes status {st}")
+        return out.reshape(L.ncell, 6, G, 5)
+
+    # ---- level state interface (mirrors the reference driver)
+    def setup(self, params, tree, fields: dict, tphy: float = 0.0):
+        self.params = params
+        self.cparams = params.to_c()
+        self.tree = tree
+        self.old = {l: f.copy() for l, f in fields.items()}
+        self.new = {l: f.copy() for l, f in fields.items()}
+        self.t_first = {l: np.full(f.ncell, -1.0) for l, f in fields.items()}
+        self.t_last = {l: np.full(f.ncell, -1.0) for l, f in fields.items()}
+        self.tphy = tphy
+        self.nsub = {}
+
+    def get(self, level, which=0) -> Fields:
+        return (self.new if which else self.old)[level].copy()
+
+    def set(self, level, which, F: Fields):
+        (self.new if which else self.old)[level] = F.copy()
+
+    def clean_new(self, level):
+        self.new[level] = self.old[level].copy()
+
+    def advancerad(self, level, dtnew, aexp=1.0, chemonly=0):
+        t = self.tree
+        L = t.level(level)
+        G = self.params.ngrp
+        co = self.old.get(level - 1) if level > t.lmin else None
+        cn = self.new.get(level - 1) if level > t.lmin else None
+        a, b = _cfields(self.old[level]), _cfields(self.new[level])
+        ca = _cfields(co) if co is not None else None
+        cb = _cfields(cn) if cn is not None else None
+        nsub = np.zeros(L.ncell, np.int32)
+        bad = np.full(1, -1, np.int32)
+        nei = np.ascontiguousarray(L.nei_oct.reshape(-1), dtype=np.int32)
+        ch = np.ascontiguousarray(L.child, dtype=np.int32)
+        dx = 0.5**level
+        st = self.lib.orc_level_advance(ctypes.byref(self.cparams), L.noct, _i(nei), _i(ch), ctypes.byref(a),
+                                        ctypes.byref(b), ctypes.byref(ca) if ca else None,
+                                        ctypes.byref(cb) if cb else None, dx, dtnew, aexp, chemonly, _i(nsub),
+                                        _i(bad))
+        self.nsub[level] = nsub
+        if st != 0:
+            raise OracleError(st, int(bad[0]))
+        return L.noct
+
+    def final(self, level):
+        t = self.tree
+        L = t.level(level)
+        fo = self.old.get(level + 1)
+        ch = np.ascontiguousarray(L.child, dtype=np.int32)
+        a, b = _cfields(self.old[level]), _cfields(self.new[level])
+        c = _cfields(fo) if fo is not None else None
+        tf, tl = self.t_first[level], self.t_last[level]
+        ftf = self.t_first.get(level + 1)
+        ftl = self.t_last.get(level + 1)
+        self.lib.orc_level_final(self.params.ngrp, L.noct, _i(ch), ctypes.byref(a), ctypes.byref(b),
+                                 ctypes.byref(c) if c else None, self.tphy, _d(tf), _d(tl),
+                                 _d(ftf) if ftf is not None else None, _d(ftl) if ftl is not None else None)
+
+    def radsolver(self, level, dtnew, aexp=1.0, chemonly=0):
+        n = self.advancerad(level, dtnew, aexp, chemonly)
+        if n > 0:
+            self.final(level)
+
+    def coarse_step(self, aexp=1.0):
+        coarse_step(self, self.tree, self.params, aexp)
+
+
+class OracleError(RuntimeError):
+    def __init__(self, status, cell):
+        super().__init__(f"oracle status {status} at cell {cell}")
+        self.status = status
+        self.cell = cell
+
+
+def coarse_step(solver, tree, params, aexp=1.0):
+    """One coarse step of the RADSTEP level recursion (advanceamr.c:243-1300):
+    each level cleans rfieldnew, advances the finer level by two half steps,
+    then runs RadSolver with its own dt_rad."""
+
+    def adv(l):
+        dt = params.dt_rad(l, aexp)
+        for _ in range(1 if l == tree.lmin else 2):
+            solver.clean_new(l)
+            if l < tree.lmax and tree.level(l + 1).noct > 0:
+                adv(l + 1)
+            solver.radsolver(l, dt, aexp, 0)
+
+    adv(tree.lmin)
+
+
+def ref_available(ngrp: int) -> bool:
+    return os.path.exists(REF_SO.get(ngrp, ""))
+
+
+class Reference:
+    """The reference's own CPU functions, driven through ref_driver.c."""
+
+    def __init__(self, ngrp: int):
+        path = REF_SO[ngrp]
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = ctypes.CDLL(path)
+        L = self.lib
+        assert L.refdrv_ngrp() == ngrp
+        self.ngrp = ngrp
+        L.refdrv_set_params.argtypes = [_dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.refdrv_build.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _ip]
+        L.refdrv_set_level.argtypes = [ctypes.c_int, ctypes.c_int] + [_dp] * 9
+        L.refdrv_get_level.argtypes = [ctypes.c_int, ctypes.c_int] + [_dp] * 9
+        L.refdrv_get_txion.argtypes = [ctypes.c_int, _dp, _dp]
+        L.refdrv_clean_new_rad.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.refdrv_radsolver.restype = ctypes.c_double
+        L.refdrv_radsolver.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.refdrv_advancerad.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.refdrv_eddington.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double, _ip, _ip, _dp]
+        L.refdrv_getcellnei.argtypes = [ctypes.c_int, _ip, _ip]
+        L.refdrv_e2t.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, _dp]
+        L.refdrv_layout.argtypes = [ctypes.POINTER(ctypes.c_long)]
+        L.refdrv_stencil_sweeps.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double]
+        L.refdrv_stencil_update.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double]
+        L.refdrv_stencil_chem.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_int]
+        L.refdrv_set_tphy.argtypes = [ctypes.c_double]
+        self.stride = 4096
+
+    def layout(self):
+        out = np.zeros(22, np.int64)
+        self.lib.refdrv_layout(out.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
+        return out
+
+    def set_params(self, p, tphy=0.0):
+        G = self.ngrp
+        assert p.ngrp == G
+        u = p.units7()
+        arr = [np.ascontiguousarray(np.asarray(getattr(p, k), dtype=np.float64)) for k in ("hnu", "alphae", "alphai", "factgrp")]
+        self._keep = (u, arr)
+        self.lib.refdrv_set_params(_d(u), p.clight, p.fudgecool, int(p.ncvgcool), *[_d(a) for a in arr], p.H0, p.om,
+                                   p.ov, tphy)
+
+    def setup(self, params, tree, fields: dict, tphy: float = 0.0):
+        self.params = params
+        self.tree = tree
+        noct, parent, neicell, child = tree.flat()
+        self._tree_arrays = (noct, parent, neicell, child)
+        st = self.lib.refdrv_build(tree.lmin, tree.lmax, _i(noct), _i(parent), _i(neicell), _i(child))
+        assert st == 0
+        self.set_params(params, tphy)
+        for l, F in fields.items():
+            self.set(l, 0, F)
+            self.set(l, 1, F)
+
+    def set(self, level, which, F: Fields):
+        F = F.copy()
+        self.lib.refdrv_set_level(level, which, *[_d(getattr(F, k)) for k in Fields.NAMES])
+
+    def get(self, level, which=0) -> Fields:
+        n = self.tree.level(level).ncell
+        F = Fields.zeros(self.ngrp, n)
+        self.lib.refdrv_get_level(level, which, *[_d(getattr(F, k)) for k in Fields.NAMES])
+        return F
+
+    def txion(self, level):
+        n = self.tree.level(level).ncell
+        a, b = np.empty(n), np.empty(n)
+        self.lib.refdrv_get_txion(level, _d(a), _d(b))
+        return a, b
+
+    def clean_new(self, level, aexp=1.0):
+        self.lib.refdrv_clean_new_rad(level, aexp)
+
+    def radsolver(self, level, dtnew, aexp=1.0, chemonly=0):
+        self.lib.refdrv_radsolver(level, dtnew, aexp, chemonly, self.stride)
+
+    def advancerad(self, level, dtnew, aexp=1.0, chemonly=0):
+        return self.lib.refdrv_advancerad(level, dtnew, aexp, chemonly, self.stride)
+
+    def coarse_step(self, aexp=1.0):
+        coarse_step(self, self.tree, self.params, aexp)
+
+    def eddington(self, fx, fy, fz, e, c, i, j):
+        fx, fy, fz, e = [np.ascontiguousarray(a, dtype=np.float64) for a in (fx, fy, fz, e)]
+        n = fx.size
+        i = np.ascontiguousarray(np.broadcast_to(i, (n,)), dtype=np.int32)
+        j = np.ascontiguousarray(np.broadcast_to(j, (n,)), dtype=np.int32)
+        out = np.empty(n)
+        self.lib.refdrv_eddington(n, _d(fx), _d(fy), _d(fz), _d(e), c, _i(i), _i(j), _d(out))
+        return out
+
+    def getcellnei(self, icell):
+        vn = np.zeros(6, np.int32)
+        vc = np.zeros(6, np.int32)
+        self.lib.refdrv_getcellnei(icell, _i(vn), _i(vc))
+        return vn.tolist(), vc.tolist()
+
+    def e2t(self, nhplus, eint, nh, aexp=1.0):
+        nhplus, eint, nh = [np.ascontiguousarray(a, dtype=np.float64) for a in (nhplus, eint, nh)]
+        out = np.empty(nhplus.size)
+        self.lib.refdrv_e2t(nhplus.size, _d(nhplus), _d(eint), _d(nh), aexp, _d(out))
+        return out

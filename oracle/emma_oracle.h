@@ -1,0 +1,102 @@
+/*
+ * emma_oracle.h -- CPU restatement of EMMA's M1 radiative-transfer + hydrogen
+ * chemistry hot path, used ONLY as test infrastructure (the checker).
+ *
+ * Nothing in the product (the HIP library under emma_amd/) links or calls this
+ * code.  It is imported by tests/, by __graft_entry__.smoke() and by the
+ * cpu_baseline leg of bench.py, and only there as the thing results are
+ * compared against (or, for cpu_baseline, the CPU timing reference).
+ *
+ * Parity flag set restated here (reference compile flags):
+ *   WRAD WCHEM COOLING SEMI_IMPLICIT OTSA TESTCOSMO WRADTEST, FP64, RADSTEP
+ *   (no COARSERAD), no POLYTROP/HESIMPLE/HELIUM/S_X/UVBKG/WCLUMP/NORECSOURCE/
+ *   SCHAYE/SUPERNOVAE/HOMOSOURCE/TRANS*.
+ *
+ * Data layout (shared with the HIP library and the reference driver):
+ *   a level holds noct octs, cell index = oct*8 + icell, icell = ix + 2 iy + 4 iz;
+ *   per-group fields are group-major: e[g*ncell + cell].
+ *   nei_oct[oct*6 + face] >= 0 : the same-level oct on that face;
+ *   nei_oct[oct*6 + face] <  0 : the level-1 neighbour cell is a leaf, its
+ *                                coarse cell index is -1 - nei_oct.
+ *   child[cell] >= 0 : the cell is split (index of its child oct at level+1).
+ */
+#ifndef EMMA_ORACLE_H
+#define EMMA_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAXGRP 8
+
+/* status codes, same numbering as the library's (include/emma_rt.h) */
+#define ORC_OK 0
+#define ORC_ERR_ARG (-1)
+#define ORC_ERR_NEG_PHOTON (-3) /* rad_utils.c:1611-1650 (abort in the reference) */
+#define ORC_ERR_EMIN (-4)       /* chem_utils.c:382-385 */
+#define ORC_ERR_NOCONV (-5)     /* chem_utils.c:453-466 */
+#define ORC_ERR_NEIGHBOR (-6)   /* rad_utils.c:1441-1444 */
+
+typedef struct orc_params {
+  int ngrp;
+  int ncvgcool;      /* RUNPARAMS.ncvgcool (0 = no cap) */
+  double fudgecool;  /* RUNPARAMS.fudgecool */
+  double clight;     /* RUNPARAMS.clight, in units of c */
+  /* struct UNITS, prototypes.h:171-179 */
+  double unit_l, unit_v, unit_t, unit_n, unit_mass, unit_d, unit_N;
+  /* struct ATOMIC after readAtomic (parameters.c:482-497): hnu in J,
+     alphae/alphai already multiplied by clightorg*c (m^3/s) */
+  double hnu[ORC_MAXGRP], alphae[ORC_MAXGRP], alphai[ORC_MAXGRP], factgrp[ORC_MAXGRP];
+  /* struct COSMOPARAM (TESTCOSMO) */
+  double H0, om, ov;
+} orc_params;
+
+/* one level's radiation state (struct Rtype, SoA) */
+typedef struct orc_fields {
+  int ncell;                      /* 8 * noct */
+  double *e, *fx, *fy, *fz, *src; /* [ngrp*ncell] */
+  double *nhplus, *eint, *nh, *temp; /* [ncell] */
+} orc_fields;
+
+/* one cell's state, struct Rtype (prototypes.h:580-603) */
+typedef struct orc_cell {
+  double e[ORC_MAXGRP], fx[ORC_MAXGRP], fy[ORC_MAXGRP], fz[ORC_MAXGRP], src[ORC_MAXGRP];
+  double nhplus, eint, nh, temp;
+} orc_cell;
+
+void orc_getcellnei(int icell, int *vnei, int *vcell);
+double orc_eddington(double fx, double fy, double fz, double ee, double c, int i, int j);
+double orc_dtrad(int level, double aexp, const orc_params *P);
+double orc_cloc(double aexp, const orc_params *P);
+void orc_e2t(orc_cell *R, double aexp, const orc_params *P);
+void orc_cooling(double temp, double x, double nH, double *lambda, double *tcool);
+double orc_alpha_a(double temp);
+double orc_alpha_b(double temp);
+double orc_beta(double temp);
+int orc_chem_cell(const orc_params *P, orc_cell *R, double dtnew, double aexp, int chemonly, int *nsub);
+
+/* batch chemistry on SoA cells (all cells treated as leaves) */
+int orc_chem_cells(const orc_params *P, orc_fields *f, double dtnew, double aexp, int chemonly,
+                   int *nsub, int *bad_cell);
+
+/* rfieldnew := rfield on one level (clean_new_rad without MPI) */
+void orc_clean_new(int ngrp, const orc_fields *old, orc_fields *newf);
+
+/* advancerad on one level (gather, sweeps, update, chem, scatter with refluxing) */
+int orc_level_advance(const orc_params *P, int noct, const int *nei_oct, const int *child,
+                      const orc_fields *old, orc_fields *newf, const orc_fields *cold, orc_fields *cnew,
+                      double dx, double dt, double aexp, int chemonly, int *nsub, int *bad_cell);
+
+/* RadSolver's final update on one level (rfield := rfieldnew, restriction of split cells) */
+void orc_level_final(int ngrp, int noct, const int *child, orc_fields *old, const orc_fields *newf,
+                     const orc_fields *fine_old, double tphy, double *t_first, double *t_last,
+                     const double *fine_t_first, const double *fine_t_last);
+
+/* fluxes of one level: rflux[cell*6*ngrp*5 + face*ngrp*5 + g*5 + var] (rad_sweepX/Y/Z layout) */
+int orc_level_fluxes(int ngrp, int noct, const int *nei_oct, const int *child, const orc_fields *old,
+                     const orc_fields *cold, double c, double *rflux);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
