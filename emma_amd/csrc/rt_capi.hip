@@ -1,0 +1,776 @@
+// rt_capi.hip -- the C-ABI of libemma_rt.so (declared in include/emma_rt.h):
+// device management, parameters, the device-resident level tree, RadSolver
+// for one level, and the coarse-step recursion captured as a HIP graph.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <vector>
+
+#include "emma_rt.h"
+#include "rt_kernels.h"
+#include "rt_stencil.h"
+
+using namespace emma;
+
+#define HIPCHK(x)                         \
+  do {                                    \
+    hipError_t e_ = (x);                  \
+    if (e_ != hipSuccess) {               \
+      last_hip_error() = e_;              \
+      return EMMA_RT_ERR_HIP;             \
+    }                                     \
+  } while (0)
+
+static hipError_t& last_hip_error() {
+  static thread_local hipError_t e = hipSuccess;
+  return e;
+}
+
+namespace {
+
+struct DevLevel {
+  int level = 0, noct = 0, ncell = 0, nslot = 0, cur = 0;
+  void* e[2] = {nullptr, nullptr};
+  void* fx[2] = {nullptr, nullptr};
+  void* fy[2] = {nullptr, nullptr};
+  void* fz[2] = {nullptr, nullptr};
+  void* src = nullptr;
+  double* nhplus[2] = {nullptr, nullptr};
+  double* eint[2] = {nullptr, nullptr};
+  double* temp[2] = {nullptr, nullptr};
+  double* nh = nullptr;
+  int* child = nullptr;
+  int* nei = nullptr;
+  void* racc = nullptr;
+  double* t_first = nullptr;
+  double* t_last = nullptr;
+  std::vector<int> hchild;  // host copy with reflux slots encoded
+};
+
+struct GraphKey {
+  int ncoarse;
+  double aexp;
+  std::vector<int> parity;
+  int xion;
+  bool operator<(const GraphKey& o) const {
+    if (ncoarse != o.ncoarse) return ncoarse < o.ncoarse;
+    if (aexp != o.aexp) return aexp < o.aexp;
+    if (xion != o.xion) return xion < o.xion;
+    return parity < o.parity;
+  }
+};
+
+}  // namespace
+
+struct emma_rt_ctx {
+  emma_rt_params p;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int precision = EMMA_RT_FP64;
+  int rbytes = 8;
+  int lmin = 0, lmax = -1;
+  bool has_tree = false;
+  std::vector<DevLevel> lev;  // index level - lmin
+  int* d_status = nullptr;
+  unsigned long long* d_counters = nullptr;
+  double* d_photons = nullptr;
+  emma_rt_stats stats;
+  int timing = 0;
+  int xion = 0;
+  double tphy = 0.;
+  std::vector<hipEvent_t> ev;  // pairs
+  size_t ev_used = 0;
+  std::map<GraphKey, std::pair<hipGraphExec_t, std::vector<int>>> graphs;
+  // stencil path
+  void* d_stencil = nullptr;
+  int stride = 0;
+  int stencil_rbytes = 0;
+};
+
+static void free_tree(emma_rt_ctx* c) {
+  for (auto& L : c->lev) {
+    for (int b = 0; b < 2; b++) {
+      (void)hipFree(L.e[b]);
+      (void)hipFree(L.fx[b]);
+      (void)hipFree(L.fy[b]);
+      (void)hipFree(L.fz[b]);
+      (void)hipFree(L.nhplus[b]);
+      (void)hipFree(L.eint[b]);
+      (void)hipFree(L.temp[b]);
+    }
+    (void)hipFree(L.src);
+    (void)hipFree(L.nh);
+    (void)hipFree(L.child);
+    (void)hipFree(L.nei);
+    (void)hipFree(L.racc);
+    (void)hipFree(L.t_first);
+    (void)hipFree(L.t_last);
+  }
+  c->lev.clear();
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.first);
+  c->graphs.clear();
+  c->has_tree = false;
+}
+
+static void drop_graphs(emma_rt_ctx* c) {
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.first);
+  c->graphs.clear();
+}
+
+// chemrad's per-call constants (chem_utils.c:266-331), evaluated on the host
+static ChemConst make_chem(const emma_rt_params& p, double dtnew, double aexp) {
+  ChemConst K;
+  memset(&K, 0, sizeof(K));
+  K.ngrp = p.ngrp;
+  K.ncvgcool = p.ncvgcool;
+  K.static_cosmo = (p.H0 == 0.) ? 1 : 0;
+  K.fudgecool = p.fudgecool;
+  K.unit_l = p.unit_l;
+  K.unit_t = p.unit_t;
+  K.unit_N = p.unit_N;
+  K.unit_n = p.unit_n;
+  K.unit_d = p.unit_d;
+  K.unit_v = p.unit_v;
+  K.aexp = aexp;
+  K.a2 = aexp * aexp;
+  K.a3 = aexp * aexp * aexp;
+  K.pa2 = pow(aexp, 2);
+  K.pa3 = pow(aexp, 3);
+  K.pa4 = pow(aexp, 4);
+  K.pa5 = pow(aexp, 5);
+  K.pv2 = pow(p.unit_v, 2);
+  K.pstar = p.unit_n * p.unit_d * pow(p.unit_v, 2);
+  K.dt = dtnew * p.unit_t * pow(aexp, 2);
+  K.emin = kPmin / (kGamma - 1.) / pow(aexp, 5) * p.unit_n * p.unit_d * pow(p.unit_v, 2);
+  K.H0 = p.H0;
+  K.om = p.om;
+  K.ov = p.ov;
+  for (int g = 0; g < p.ngrp; g++) {
+    K.heat[g] = p.alphae[g] * p.hnu[g] - p.alphai[g] * kHnu0;
+    K.alphai[g] = p.alphai[g];
+    K.factgrp[g] = p.factgrp[g];
+    K.emin_grp[g] = kEmin * p.factgrp[g] / p.unit_N;
+  }
+  return K;
+}
+
+static double cloc_of(const emma_rt_params& p, double aexp) { return aexp * p.clight * 299792458. / p.unit_v; }
+
+static double dtrad_of(const emma_rt_params& p, int level, double aexp) {
+  return 0.85 / (aexp * p.clight * 299792458. / p.unit_v) / 3.0 / (double)(1 << level);
+}
+
+static int check_params(const emma_rt_params* p) {
+  if (p == nullptr || p->ngrp < 1 || p->ngrp > EMMA_RT_MAX_GRP) return EMMA_RT_ERR_ARG;
+  if (!(p->unit_t > 0) || !(p->unit_v > 0) || !(p->unit_N > 0) || !(p->clight > 0)) return EMMA_RT_ERR_ARG;
+  return EMMA_RT_OK;
+}
+
+// ---------------------------------------------------------------- device mgmt
+extern "C" int emma_rt_countdevices(int rank, int* count) {
+  (void)rank;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (count) *count = (e == hipSuccess) ? n : 0;
+  if (e != hipSuccess || n == 0) return EMMA_RT_ERR_NODEVICE;
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_initlocaldevice(int rank, int count) {
+  if (count <= 0) return EMMA_RT_ERR_NODEVICE;
+  HIPCHK(hipSetDevice(rank % count));
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_checkdevice(int rank, int* device) {
+  (void)rank;
+  int d = -1;
+  HIPCHK(hipGetDevice(&d));
+  if (device) *device = d;
+  return EMMA_RT_OK;
+}
+
+extern "C" const char* emma_rt_strerror(int s) {
+  switch (s) {
+    case EMMA_RT_OK: return "ok";
+    case EMMA_RT_ERR_ARG: return "bad argument";
+    case EMMA_RT_ERR_HIP: return hipGetErrorString(last_hip_error());
+    case EMMA_RT_ERR_NEG_PHOTON: return "negative or NaN photon density after transport";
+    case EMMA_RT_ERR_EMIN: return "photon density below EMIN entering chemistry";
+    case EMMA_RT_ERR_NOCONV: return "chemistry sub-stepping did not converge (fudgecool < 1e-20)";
+    case EMMA_RT_ERR_NEIGHBOR: return "missing neighbour oct";
+    case EMMA_RT_ERR_LAYOUT: return "struct layout mismatch";
+    case EMMA_RT_ERR_NODEVICE: return "no GPU device";
+    case EMMA_RT_ERR_STATE: return "call out of order";
+    default: return "unknown status";
+  }
+}
+
+// ---------------------------------------------------------------- context
+extern "C" int emma_rt_create(const emma_rt_params* p, emma_rt_ctx** out) {
+  if (out == nullptr) return EMMA_RT_ERR_ARG;
+  *out = nullptr;
+  int st = check_params(p);
+  if (st) return st;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return EMMA_RT_ERR_NODEVICE;
+  emma_rt_ctx* c = new emma_rt_ctx();
+  c->p = *p;
+  memset(&c->stats, 0, sizeof(c->stats));
+  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+      hipMalloc((void**)&c->d_status, 4 * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&c->d_counters, 2 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc((void**)&c->d_photons, sizeof(double)) != hipSuccess) {
+    delete c;
+    return EMMA_RT_ERR_HIP;
+  }
+  if (launch_fill((unsigned int*)c->d_status, 4, 0u, c->stream) != hipSuccess ||
+      launch_fill((unsigned int*)c->d_counters, 4, 0u, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    delete c;
+    return EMMA_RT_ERR_HIP;
+  }
+  *out = c;
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_set_params(emma_rt_ctx* c, const emma_rt_params* p) {
+  if (c == nullptr) return EMMA_RT_ERR_ARG;
+  int st = check_params(p);
+  if (st) return st;
+  if (c->has_tree && p->ngrp != c->p.ngrp) return EMMA_RT_ERR_ARG;
+  c->p = *p;
+  drop_graphs(c);
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_destroy(emma_rt_ctx* c) {
+  if (c == nullptr) return EMMA_RT_OK;
+  (void)hipStreamSynchronize(c->stream);
+  free_tree(c);
+  stencil_free(c->d_stencil);
+  for (auto e : c->ev) (void)hipEventDestroy(e);
+  (void)hipFree(c->d_status);
+  (void)hipFree(c->d_counters);
+  (void)hipFree(c->d_photons);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return EMMA_RT_OK;
+}
+
+// ---------------------------------------------------------------- tree
+template <typename T>
+static hipError_t dalloc(T** p, size_t n) {
+  return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int* noct, const int* nei_oct,
+                                   const int* child, int precision) {
+  if (c == nullptr || noct == nullptr || nei_oct == nullptr || child == nullptr) return EMMA_RT_ERR_ARG;
+  if (lmin < 1 || lmax < lmin || lmax > 30) return EMMA_RT_ERR_ARG;
+  if (precision != EMMA_RT_FP64 && precision != EMMA_RT_MIXED) return EMMA_RT_ERR_ARG;
+  free_tree(c);
+  const int nl = lmax - lmin + 1;
+  const int G = c->p.ngrp;
+  c->precision = precision;
+  c->rbytes = precision == EMMA_RT_FP64 ? 8 : 4;
+  c->lmin = lmin;
+  c->lmax = lmax;
+  c->lev.resize(nl);
+  // host-side validation and reflux-slot assignment
+  size_t on = 0, oc = 0;
+  std::vector<size_t> off_n(nl), off_c(nl);
+  for (int k = 0; k < nl; k++) {
+    if (noct[k] < 0 || (k == 0 && noct[k] == 0)) return EMMA_RT_ERR_ARG;
+    off_n[k] = on;
+    off_c[k] = oc;
+    on += (size_t)noct[k] * 6;
+    oc += (size_t)noct[k] * 8;
+    DevLevel& L = c->lev[k];
+    L.level = lmin + k;
+    L.noct = noct[k];
+    L.ncell = noct[k] * 8;
+    L.hchild.assign(child + off_c[k], child + off_c[k] + L.ncell);
+  }
+  for (int k = 0; k < nl; k++) {
+    DevLevel& L = c->lev[k];
+    const int nnext = (k + 1 < nl) ? c->lev[k + 1].noct : 0;
+    for (int i = 0; i < L.ncell; i++) {
+      int ch = L.hchild[i];
+      if (ch < -1 || ch >= nnext) return EMMA_RT_ERR_ARG;
+    }
+    const int* nei = nei_oct + off_n[k];
+    const int ncoarse = k > 0 ? c->lev[k - 1].ncell : 0;
+    for (int i = 0; i < L.noct * 6; i++) {
+      int no = nei[i];
+      if (no >= L.noct) return EMMA_RT_ERR_NEIGHBOR;
+      if (no < 0) {
+        if (k == 0) return EMMA_RT_ERR_NEIGHBOR;  // the coarsest level is complete
+        int j = -1 - no;
+        if (j >= ncoarse) return EMMA_RT_ERR_NEIGHBOR;
+        DevLevel& C = c->lev[k - 1];
+        if (C.hchild[j] >= 0) return EMMA_RT_ERR_NEIGHBOR;  // injected cells are leaves
+        if (C.hchild[j] == -1) C.hchild[j] = -2 - (C.nslot++);
+      }
+    }
+  }
+  // device allocation and upload
+  for (int k = 0; k < nl; k++) {
+    DevLevel& L = c->lev[k];
+    const size_t gn = (size_t)G * L.ncell;
+    for (int b = 0; b < 2; b++) {
+      if (c->rbytes == 8) {
+        HIPCHK(dalloc((double**)&L.e[b], gn));
+        HIPCHK(dalloc((double**)&L.fx[b], gn));
+        HIPCHK(dalloc((double**)&L.fy[b], gn));
+        HIPCHK(dalloc((double**)&L.fz[b], gn));
+      } else {
+        HIPCHK(dalloc((float**)&L.e[b], gn));
+        HIPCHK(dalloc((float**)&L.fx[b], gn));
+        HIPCHK(dalloc((float**)&L.fy[b], gn));
+        HIPCHK(dalloc((float**)&L.fz[b], gn));
+      }
+      HIPCHK(dalloc(&L.nhplus[b], L.ncell));
+      HIPCHK(dalloc(&L.eint[b], L.ncell));
+      HIPCHK(dalloc(&L.temp[b], L.ncell));
+    }
+    if (c->rbytes == 8)
+      HIPCHK(dalloc((double**)&L.src, gn));
+    else
+      HIPCHK(dalloc((float**)&L.src, gn));
+    HIPCHK(dalloc(&L.nh, L.ncell));
+    HIPCHK(dalloc(&L.child, L.ncell));
+    HIPCHK(dalloc(&L.nei, (size_t)L.noct * 6));
+    const size_t accn = (size_t)L.nslot * 4 * (G == 1 ? 1 : (G == 3 ? 3 : kMaxGrp));
+    HIPCHK(hipMalloc(&L.racc, std::max<size_t>(accn, 1) * c->rbytes));
+    HIPCHK(hipMemset(L.racc, 0, std::max<size_t>(accn, 1) * c->rbytes));
+    HIPCHK(hipMemcpy(L.child, L.hchild.data(), L.ncell * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L.nei, nei_oct + off_n[k], (size_t)L.noct * 6 * sizeof(int), hipMemcpyHostToDevice));
+  }
+  c->has_tree = true;
+  if (c->xion) {
+    c->xion = 0;
+    return emma_rt_enable_xion_times(c, 1);
+  }
+  return EMMA_RT_OK;
+}
+
+static DevLevel* level_of(emma_rt_ctx* c, int level) {
+  if (c == nullptr || !c->has_tree || level < c->lmin || level > c->lmax) return nullptr;
+  return &c->lev[level - c->lmin];
+}
+
+template <typename R>
+static int upload_r(void* dst, const double* src, size_t n, hipStream_t s) {
+  if (sizeof(R) == sizeof(double)) {
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return EMMA_RT_OK;
+  }
+  std::vector<R> tmp(n);
+  for (size_t i = 0; i < n; i++) tmp[i] = (R)src[i];
+  HIPCHK(hipMemcpyAsync(dst, tmp.data(), n * sizeof(R), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return EMMA_RT_OK;
+}
+
+template <typename R>
+static int download_r(double* dst, const void* src, size_t n, hipStream_t s) {
+  if (sizeof(R) == sizeof(double)) {
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return EMMA_RT_OK;
+  }
+  std::vector<R> tmp(n);
+  HIPCHK(hipMemcpyAsync(tmp.data(), src, n * sizeof(R), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (size_t i = 0; i < n; i++) dst[i] = (double)tmp[i];
+  return EMMA_RT_OK;
+}
+
+static int upload_rf(emma_rt_ctx* c, void* dst, const double* src, size_t n) {
+  return c->rbytes == 8 ? upload_r<double>(dst, src, n, c->stream) : upload_r<float>(dst, src, n, c->stream);
+}
+
+static int download_rf(emma_rt_ctx* c, double* dst, const void* src, size_t n) {
+  return c->rbytes == 8 ? download_r<double>(dst, src, n, c->stream) : download_r<float>(dst, src, n, c->stream);
+}
+
+extern "C" int emma_rt_set_level(emma_rt_ctx* c, int level, const double* e, const double* fx, const double* fy,
+                                 const double* fz, const double* src, const double* nhplus, const double* eint,
+                                 const double* nh, const double* temp) {
+  DevLevel* L = level_of(c, level);
+  if (L == nullptr) return EMMA_RT_ERR_STATE;
+  if (!e || !fx || !fy || !fz || !src || !nhplus || !eint || !nh || !temp) return EMMA_RT_ERR_ARG;
+  const size_t gn = (size_t)c->p.ngrp * L->ncell;
+  const int b = L->cur;
+  int st;
+  if ((st = upload_rf(c, L->e[b], e, gn)) || (st = upload_rf(c, L->fx[b], fx, gn)) ||
+      (st = upload_rf(c, L->fy[b], fy, gn)) || (st = upload_rf(c, L->fz[b], fz, gn)) ||
+      (st = upload_rf(c, L->src, src, gn)))
+    return st;
+  HIPCHK(hipMemcpy(L->nhplus[b], nhplus, L->ncell * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(L->eint[b], eint, L->ncell * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(L->temp[b], temp, L->ncell * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(L->nh, nh, L->ncell * sizeof(double), hipMemcpyHostToDevice));
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_get_level(emma_rt_ctx* c, int level, double* e, double* fx, double* fy, double* fz,
+                                 double* src, double* nhplus, double* eint, double* nh, double* temp) {
+  DevLevel* L = level_of(c, level);
+  if (L == nullptr) return EMMA_RT_ERR_STATE;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const size_t gn = (size_t)c->p.ngrp * L->ncell;
+  const int b = L->cur;
+  int st;
+  if (e && (st = download_rf(c, e, L->e[b], gn))) return st;
+  if (fx && (st = download_rf(c, fx, L->fx[b], gn))) return st;
+  if (fy && (st = download_rf(c, fy, L->fy[b], gn))) return st;
+  if (fz && (st = download_rf(c, fz, L->fz[b], gn))) return st;
+  if (src && (st = download_rf(c, src, L->src, gn))) return st;
+  if (nhplus) HIPCHK(hipMemcpy(nhplus, L->nhplus[b], L->ncell * sizeof(double), hipMemcpyDeviceToHost));
+  if (eint) HIPCHK(hipMemcpy(eint, L->eint[b], L->ncell * sizeof(double), hipMemcpyDeviceToHost));
+  if (nh) HIPCHK(hipMemcpy(nh, L->nh, L->ncell * sizeof(double), hipMemcpyDeviceToHost));
+  if (temp) HIPCHK(hipMemcpy(temp, L->temp[b], L->ncell * sizeof(double), hipMemcpyDeviceToHost));
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_enable_xion_times(emma_rt_ctx* c, int enable) {
+  if (c == nullptr) return EMMA_RT_ERR_ARG;
+  if (!c->has_tree) {
+    c->xion = enable ? 1 : 0;
+    return EMMA_RT_OK;
+  }
+  if (enable && !c->xion) {
+    for (auto& L : c->lev) {
+      HIPCHK(dalloc(&L.t_first, L.ncell));
+      HIPCHK(dalloc(&L.t_last, L.ncell));
+      std::vector<double> m1(L.ncell, -1.);
+      HIPCHK(hipMemcpy(L.t_first, m1.data(), L.ncell * sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(L.t_last, m1.data(), L.ncell * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
+  if (!enable && c->xion) {
+    for (auto& L : c->lev) {
+      (void)hipFree(L.t_first);
+      (void)hipFree(L.t_last);
+      L.t_first = L.t_last = nullptr;
+    }
+  }
+  c->xion = enable ? 1 : 0;
+  drop_graphs(c);
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_get_xion_times(emma_rt_ctx* c, int level, double* t_first, double* t_last) {
+  DevLevel* L = level_of(c, level);
+  if (L == nullptr || !c->xion) return EMMA_RT_ERR_STATE;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(t_first, L->t_first, L->ncell * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(t_last, L->t_last, L->ncell * sizeof(double), hipMemcpyDeviceToHost));
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_set_tphy(emma_rt_ctx* c, double tphy) {
+  if (c == nullptr) return EMMA_RT_ERR_ARG;
+  c->tphy = tphy;
+  drop_graphs(c);
+  return EMMA_RT_OK;
+}
+
+// ---------------------------------------------------------------- level step
+template <typename R>
+static LevelArgs<R> level_args(emma_rt_ctx* c, int k, double dtnew, double aexp, int chemonly, bool counters) {
+  DevLevel& L = c->lev[k];
+  const int a = L.cur, b = 1 - L.cur;
+  LevelArgs<R> A;
+  memset(&A, 0, sizeof(A));
+  A.ncell = L.ncell;
+  A.ng = c->p.ngrp;
+  A.level = L.level;
+  A.chemonly = chemonly;
+  A.e = (const R*)L.e[a];
+  A.fx = (const R*)L.fx[a];
+  A.fy = (const R*)L.fy[a];
+  A.fz = (const R*)L.fz[a];
+  A.nhplus = L.nhplus[a];
+  A.eint = L.eint[a];
+  A.src = (R*)L.src;
+  A.nh = L.nh;
+  A.eo = (R*)L.e[b];
+  A.fxo = (R*)L.fx[b];
+  A.fyo = (R*)L.fy[b];
+  A.fzo = (R*)L.fz[b];
+  A.nhpluso = L.nhplus[b];
+  A.einto = L.eint[b];
+  A.tempo = L.temp[b];
+  A.child = L.child;
+  A.nei = L.nei;
+  A.racc = (R*)L.racc;
+  if (k > 0) {
+    DevLevel& C = c->lev[k - 1];
+    A.ncoarse = C.ncell;
+    A.ce = (const R*)C.e[C.cur];
+    A.cfx = (const R*)C.fx[C.cur];
+    A.cfy = (const R*)C.fy[C.cur];
+    A.cfz = (const R*)C.fz[C.cur];
+    A.cchild = C.child;
+    A.cracc = (R*)C.racc;
+  }
+  if (k + 1 < (int)c->lev.size()) {
+    DevLevel& F = c->lev[k + 1];
+    A.nfine = F.ncell;
+    A.fe = (const R*)F.e[F.cur];
+    A.ffx = (const R*)F.fx[F.cur];
+    A.ffy = (const R*)F.fy[F.cur];
+    A.ffz = (const R*)F.fz[F.cur];
+    A.fsrc = (const R*)F.src;
+    A.fnhplus = F.nhplus[F.cur];
+    A.feint = F.eint[F.cur];
+    A.ftemp = F.temp[F.cur];
+    A.fnh = F.nh;
+    A.ft_first = F.t_first;
+    A.ft_last = F.t_last;
+  }
+  if (c->xion) {
+    A.t_first = L.t_first;
+    A.t_last = L.t_last;
+  }
+  A.tphy = c->tphy;
+  const double cl = cloc_of(c->p, aexp);
+  const double dx = pow(0.5, L.level);
+  A.c = (R)cl;
+  A.hc = (R)(0.5 * cl);
+  A.dtsurdx = (R)(dtnew / dx);
+  A.status = c->d_status;
+  A.counters = counters ? c->d_counters : nullptr;
+  return A;
+}
+
+static hipError_t launch_level(emma_rt_ctx* c, int k, double dtnew, double aexp, int chemonly, bool counters) {
+  DevLevel& L = c->lev[k];
+  const ChemConst K = make_chem(c->p, dtnew, aexp);
+  hipError_t e;
+  if (c->rbytes == 8)
+    e = launch_level_step<double>(level_args<double>(c, k, dtnew, aexp, chemonly, counters), K, c->stream);
+  else
+    e = launch_level_step<float>(level_args<float>(c, k, dtnew, aexp, chemonly, counters), K, c->stream);
+  L.cur = 1 - L.cur;
+  return e;
+}
+
+static hipEvent_t next_event(emma_rt_ctx* c) {
+  if (c->ev_used == c->ev.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->ev.push_back(e);
+  }
+  return c->ev[c->ev_used++];
+}
+
+static int timed_launch(emma_rt_ctx* c, int k, double dtnew, double aexp, int chemonly) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->timing) {
+    e0 = next_event(c);
+    e1 = next_event(c);
+    if (!e0 || !e1) return EMMA_RT_ERR_HIP;
+    HIPCHK(hipEventRecord(e0, c->stream));
+  }
+  HIPCHK(launch_level(c, k, dtnew, aexp, chemonly, true));
+  if (c->timing) HIPCHK(hipEventRecord(e1, c->stream));
+  c->stats.level_steps++;
+  return EMMA_RT_OK;
+}
+
+static int begin_call(emma_rt_ctx* c) {
+  HIPCHK(launch_fill((unsigned int*)c->d_status, 4, 0u, c->stream));
+  HIPCHK(launch_fill((unsigned int*)c->d_counters, 4, 0u, c->stream));
+  memset(&c->stats, 0, sizeof(c->stats));
+  c->ev_used = 0;
+  return EMMA_RT_OK;
+}
+
+static int end_call(emma_rt_ctx* c) {
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int status[4];
+  unsigned long long cnt[2];
+  HIPCHK(hipMemcpy(status, c->d_status, sizeof(status), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(cnt, c->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  c->stats.status = status[0];
+  c->stats.bad_level = status[1];
+  c->stats.bad_cell = status[2];
+  c->stats.chem_substeps = (long long)cnt[0];
+  c->stats.cell_updates = (long long)cnt[1];
+  double ms = 0.;
+  for (size_t i = 0; i + 1 < c->ev_used; i += 2) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]));
+    ms += t;
+  }
+  c->stats.kernel_ms = ms;
+  return status[0];
+}
+
+extern "C" int emma_rt_radsolver(emma_rt_ctx* c, int level, double dtnew, double aexp, int chemonly) {
+  DevLevel* L = level_of(c, level);
+  if (L == nullptr) return EMMA_RT_ERR_STATE;
+  int st = begin_call(c);
+  if (st) return st;
+  st = timed_launch(c, level - c->lmin, dtnew, aexp, chemonly);
+  if (st) return st;
+  return end_call(c);
+}
+
+// RADSTEP recursion of Advance_level (advanceamr.c:243-1300): per step of
+// level l, the finer level takes two steps first, then level l steps.
+static int recurse(emma_rt_ctx* c, int k, double aexp) {
+  const int nl = (int)c->lev.size();
+  const DevLevel& L = c->lev[k];
+  const double dt = dtrad_of(c->p, L.level, aexp);
+  const int nsub = (k == 0) ? 1 : 2;
+  for (int s = 0; s < nsub; s++) {
+    if (k + 1 < nl && c->lev[k + 1].noct > 0) {
+      int st = recurse(c, k + 1, aexp);
+      if (st) return st;
+    }
+    int st = timed_launch(c, k, dt, aexp, 0);
+    if (st) return st;
+  }
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
+  if (c == nullptr || !c->has_tree) return EMMA_RT_ERR_STATE;
+  if (ncoarse < 1) return EMMA_RT_ERR_ARG;
+  int st = begin_call(c);
+  if (st) return st;
+  if (c->timing) {  // eager launches with per-kernel events
+    for (int i = 0; i < ncoarse; i++) {
+      st = recurse(c, 0, aexp);
+      if (st) return st;
+    }
+    return end_call(c);
+  }
+  GraphKey key;
+  key.ncoarse = ncoarse;
+  key.aexp = aexp;
+  key.xion = c->xion;
+  for (auto& L : c->lev) key.parity.push_back(L.cur);
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rs = EMMA_RT_OK;
+    for (int i = 0; i < ncoarse && rs == EMMA_RT_OK; i++) rs = recurse(c, 0, aexp);
+    hipError_t ce = hipStreamEndCapture(c->stream, &g);
+    if (rs) return rs;
+    HIPCHK(ce);
+    hipGraphExec_t ex;
+    hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(ie);
+    std::vector<int> endpar;
+    for (auto& L : c->lev) endpar.push_back(L.cur);
+    // restore the parity the replay starts from; the launch below advances it
+    for (size_t k = 0; k < c->lev.size(); k++) c->lev[k].cur = key.parity[k];
+    it = c->graphs.emplace(key, std::make_pair(ex, endpar)).first;
+    c->stats.level_steps = 0;
+  }
+  HIPCHK(hipGraphLaunch(it->second.first, c->stream));
+  for (size_t k = 0; k < c->lev.size(); k++) c->lev[k].cur = it->second.second[k];
+  long long steps = 0;
+  for (int k = 0; k < (int)c->lev.size(); k++)
+    if (c->lev[k].noct > 0) steps += (long long)ncoarse << k;
+  c->stats.level_steps = steps;
+  return end_call(c);
+}
+
+extern "C" int emma_rt_enable_timing(emma_rt_ctx* c, int enable) {
+  if (c == nullptr) return EMMA_RT_ERR_ARG;
+  c->timing = enable ? 1 : 0;
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_get_stats(emma_rt_ctx* c, emma_rt_stats* out) {
+  if (c == nullptr || out == nullptr) return EMMA_RT_ERR_ARG;
+  *out = c->stats;
+  if (c->has_tree) {
+    HIPCHK(launch_fill((unsigned int*)c->d_photons, 2, 0u, c->stream));
+    for (auto& L : c->lev) {
+      const double vol = pow(0.5, 3 * L.level);
+      if (c->rbytes == 8)
+        HIPCHK(launch_photon_sum<double>((const double*)L.e[L.cur], L.child, L.ncell, c->p.ngrp, vol, c->d_photons,
+                                         c->stream));
+      else
+        HIPCHK(launch_photon_sum<float>((const float*)L.e[L.cur], L.child, L.ncell, c->p.ngrp, vol, c->d_photons,
+                                        c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(&out->photons, c->d_photons, sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return EMMA_RT_OK;
+}
+
+// ---------------------------------------------------------------- stencil path
+extern "C" int emma_rt_rgrid_layout(int ngrp, int real_bytes, long* out) {
+  if (out == nullptr) return EMMA_RT_ERR_ARG;
+  RgridLayout L;
+  if (!rgrid_layout(ngrp, real_bytes, &L)) return EMMA_RT_ERR_ARG;
+  memcpy(out, L.v, sizeof(L.v));
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_check_layout(int ngrp, int real_bytes, const long* host) {
+  if (host == nullptr) return EMMA_RT_ERR_ARG;
+  RgridLayout L;
+  if (!rgrid_layout(ngrp, real_bytes, &L)) return EMMA_RT_ERR_ARG;
+  for (int i = 0; i < 22; i++)
+    if (L.v[i] != host[i]) return EMMA_RT_ERR_LAYOUT;
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_create_radstencil(emma_rt_ctx* c, int stride, int real_bytes) {
+  if (c == nullptr || stride <= 0 || (real_bytes != 4 && real_bytes != 8)) return EMMA_RT_ERR_ARG;
+  RgridLayout L;
+  if (!rgrid_layout(c->p.ngrp, real_bytes, &L)) return EMMA_RT_ERR_ARG;
+  stencil_free(c->d_stencil);
+  c->d_stencil = nullptr;
+  HIPCHK(hipMalloc(&c->d_stencil, (size_t)stride * L.v[19]));
+  c->stride = stride;
+  c->stencil_rbytes = real_bytes;
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_destroy_radstencil(emma_rt_ctx* c) {
+  if (c == nullptr) return EMMA_RT_ERR_ARG;
+  stencil_free(c->d_stencil);
+  c->d_stencil = nullptr;
+  c->stride = 0;
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_advance_stencil(emma_rt_ctx* c, void* stencil, int nread, double dxcur, double dtnew,
+                                       double aexp, int chemonly) {
+  if (c == nullptr || stencil == nullptr || nread < 0) return EMMA_RT_ERR_ARG;
+  if (c->d_stencil == nullptr) return EMMA_RT_ERR_STATE;
+  if (nread > c->stride) return EMMA_RT_ERR_ARG;
+  if (nread == 0) return EMMA_RT_OK;
+  RgridLayout L;
+  rgrid_layout(c->p.ngrp, c->stencil_rbytes, &L);
+  const size_t bytes = (size_t)nread * L.v[19];
+  int st = begin_call(c);
+  if (st) return st;
+  HIPCHK(hipMemcpyAsync(c->d_stencil, stencil, bytes, hipMemcpyHostToDevice, c->stream));
+  const ChemConst K = make_chem(c->p, dtnew, aexp);
+  const double cl = cloc_of(c->p, aexp);
+  HIPCHK(launch_stencil(c->d_stencil, L, c->p.ngrp, c->stencil_rbytes, nread, cl, dtnew / dxcur, chemonly, K,
+                        c->d_status, c->d_counters, c->stream));
+  HIPCHK(hipMemcpyAsync(stencil, c->d_stencil, bytes, hipMemcpyDeviceToHost, c->stream));
+  return end_call(c);
+}

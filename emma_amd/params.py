@@ -29,7 +29,7 @@ PMIN = 1e-10
 EMIN = 1e-8
 GAMMA = 5.0 / 3.0
 
-MAX_GRP = 8
+MAX_GRP = 9
 
 # Group tables of the reference's atomic data files (src/atomic_data/*.dat):
 # columns hnu [eV], alphae [m^2], alphai [m^2], factgrp.

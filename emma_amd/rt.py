@@ -1,0 +1,255 @@
+"""Host mirror of EMMA's radiation-step interface over libemma_rt.so.
+
+The reference's host calls ``RadSolver(level, param, firstoct, cpu, stencil,
+stride, dtnew, aexp, chemonly)`` (``rad_utils.c:1861``) once per level and
+sub-step, after ``clean_new_rad`` (``rad_utils.c:2050``), in the level
+recursion of ``Advance_level`` (``advanceamr.c:243``).  :class:`RadGPU` keeps
+those names and argument meanings over a device-resident tree:
+
+* ``radsolver(level, dtnew, aexp, chemonly)`` -- one level step on the GPU;
+* ``clean_new(level)`` -- nothing to do (the device state is double
+  buffered), kept so host loops read like the reference's;
+* ``coarse_step(aexp, ncoarse)`` -- the whole RADSTEP recursion of
+  ``ncoarse`` coarse steps as one HIP graph.
+
+Errors the reference turns into ``abort()`` come back as
+:class:`EmmaRTError` with the library's status code.  There is no CPU
+fallback: without the built library (``python -m emma_amd.build``) every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .params import RunParams
+from .synth import Fields
+from . import tree as T
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libemma_rt.so")
+
+OK = 0
+ERR_ARG = -1
+ERR_HIP = -2
+ERR_NEG_PHOTON = -3
+ERR_EMIN = -4
+ERR_NOCONV = -5
+ERR_NEIGHBOR = -6
+ERR_LAYOUT = -7
+ERR_NODEVICE = -8
+ERR_STATE = -9
+
+FP64 = 0
+MIXED = 1
+
+EXPORTS = [
+    "emma_rt_countdevices", "emma_rt_initlocaldevice", "emma_rt_checkdevice", "emma_rt_strerror",
+    "emma_rt_create", "emma_rt_set_params", "emma_rt_destroy",
+    "emma_rt_rgrid_layout", "emma_rt_check_layout", "emma_rt_create_radstencil", "emma_rt_destroy_radstencil",
+    "emma_rt_advance_stencil",
+    "emma_rt_tree_create", "emma_rt_set_level", "emma_rt_get_level", "emma_rt_enable_xion_times",
+    "emma_rt_get_xion_times", "emma_rt_set_tphy", "emma_rt_radsolver", "emma_rt_coarse_step",
+    "emma_rt_get_stats", "emma_rt_enable_timing",
+]
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("cell_updates", ctypes.c_longlong),
+        ("chem_substeps", ctypes.c_longlong),
+        ("level_steps", ctypes.c_longlong),
+        ("kernel_ms", ctypes.c_double),
+        ("photons", ctypes.c_double),
+        ("status", ctypes.c_int),
+        ("bad_level", ctypes.c_int),
+        ("bad_cell", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class EmmaRTError(RuntimeError):
+    def __init__(self, status: int, msg: str, stats=None):
+        super().__init__(f"emma_rt status {status}: {msg}")
+        self.status = status
+        self.stats = stats
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libemma_rt.so; raise if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} missing: build it with `python -m emma_amd.build`")
+    L = ctypes.CDLL(path)
+    L.emma_rt_strerror.restype = ctypes.c_char_p
+    L.emma_rt_countdevices.argtypes = [ctypes.c_int, _ip]
+    L.emma_rt_initlocaldevice.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.emma_rt_checkdevice.argtypes = [ctypes.c_int, _ip]
+    L.emma_rt_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+    L.emma_rt_set_params.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.emma_rt_destroy.argtypes = [ctypes.c_void_p]
+    L.emma_rt_rgrid_layout.argtypes = [ctypes.c_int, ctypes.c_int, _lp]
+    L.emma_rt_check_layout.argtypes = [ctypes.c_int, ctypes.c_int, _lp]
+    L.emma_rt_create_radstencil.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.emma_rt_destroy_radstencil.argtypes = [ctypes.c_void_p]
+    L.emma_rt_advance_stencil.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    L.emma_rt_tree_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, ctypes.c_int]
+    L.emma_rt_set_level.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_dp] * 9
+    L.emma_rt_get_level.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_dp] * 9
+    L.emma_rt_enable_xion_times.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.emma_rt_get_xion_times.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+    L.emma_rt_set_tphy.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    L.emma_rt_radsolver.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    L.emma_rt_coarse_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+    L.emma_rt_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
+    L.emma_rt_enable_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    _lib = L
+    return L
+
+
+def strerror(status: int) -> str:
+    return load().emma_rt_strerror(status).decode()
+
+
+def rgrid_layout(ngrp: int, real_bytes: int = 8) -> np.ndarray:
+    out = np.zeros(22, dtype=np.int64)
+    st = load().emma_rt_rgrid_layout(ngrp, real_bytes, out.ctypes.data_as(_lp))
+    if st != OK:
+        raise EmmaRTError(st, strerror(st))
+    return out
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    load().emma_rt_countdevices(0, ctypes.byref(n))
+    return n.value
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+class RadGPU:
+    """A device context holding one tree of levels (the resident path)."""
+
+    def __init__(self, params: RunParams, precision: int = FP64):
+        self.lib = load()
+        self.params = params
+        self.precision = precision
+        self._cp = params.to_c()
+        h = ctypes.c_void_p()
+        self._check(self.lib.emma_rt_create(ctypes.byref(self._cp), ctypes.byref(h)))
+        self.h = h
+        self.tree = None
+
+    def _check(self, st, stats=None):
+        if st != OK:
+            raise EmmaRTError(st, strerror(st), stats)
+        return st
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.emma_rt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- set-up
+    def set_params(self, params: RunParams):
+        self.params = params
+        self._cp = params.to_c()
+        self._check(self.lib.emma_rt_set_params(self.h, ctypes.byref(self._cp)))
+
+    def load_tree(self, tree: T.Tree):
+        noct = np.array(tree.noct(), dtype=np.int32)
+        nei = np.ascontiguousarray(
+            np.concatenate([tree.level(l).nei_oct.reshape(-1) for l in range(tree.lmin, tree.lmax + 1)]), dtype=np.int32)
+        child = np.ascontiguousarray(
+            np.concatenate([tree.level(l).child for l in range(tree.lmin, tree.lmax + 1)]), dtype=np.int32)
+        self._check(self.lib.emma_rt_tree_create(self.h, tree.lmin, tree.lmax, noct.ctypes.data_as(_ip),
+                                                 nei.ctypes.data_as(_ip), child.ctypes.data_as(_ip),
+                                                 self.precision))
+        self.tree = tree
+
+    def setup(self, params: RunParams, tree: T.Tree, fields: dict, tphy: float = 0.0):
+        if params is not self.params:
+            self.set_params(params)
+        self.load_tree(tree)
+        for l, F in fields.items():
+            self.set(l, 0, F)
+        self.set_tphy(tphy)
+
+    def set(self, level: int, which: int, F: Fields):
+        if which != 0:
+            raise ValueError("the device holds one state per level (rfield); rfieldnew is internal")
+        F = F.copy()
+        self._check(self.lib.emma_rt_set_level(self.h, level, *[_d(getattr(F, k)) for k in Fields.NAMES]))
+
+    def get(self, level: int, which: int = 0) -> Fields:
+        n = self.tree.level(level).ncell
+        F = Fields.zeros(self.params.ngrp, n)
+        self._check(self.lib.emma_rt_get_level(self.h, level, *[_d(getattr(F, k)) for k in Fields.NAMES]))
+        return F
+
+    def enable_xion_times(self, on: bool = True):
+        self._check(self.lib.emma_rt_enable_xion_times(self.h, int(on)))
+
+    def xion_times(self, level: int):
+        n = self.tree.level(level).ncell
+        a, b = np.empty(n), np.empty(n)
+        self._check(self.lib.emma_rt_get_xion_times(self.h, level, _d(a), _d(b)))
+        return a, b
+
+    def set_tphy(self, tphy: float):
+        self._check(self.lib.emma_rt_set_tphy(self.h, float(tphy)))
+
+    def enable_timing(self, on: bool = True):
+        self._check(self.lib.emma_rt_enable_timing(self.h, int(on)))
+
+    # ---- the reference's calls
+    def clean_new(self, level: int):
+        """clean_new_rad: no device work (double-buffered state)."""
+        return None
+
+    def radsolver(self, level: int, dtnew: float, aexp: float = 1.0, chemonly: int = 0):
+        st = self.lib.emma_rt_radsolver(self.h, level, dtnew, aexp, chemonly)
+        self._check(st, self.stats() if st < 0 and st != ERR_HIP else None)
+
+    def coarse_step(self, aexp: float = 1.0, ncoarse: int = 1):
+        st = self.lib.emma_rt_coarse_step(self.h, aexp, ncoarse)
+        self._check(st, self.stats() if st < 0 and st != ERR_HIP else None)
+
+    def stats(self) -> dict:
+        s = Stats()
+        self.lib.emma_rt_get_stats(self.h, ctypes.byref(s))
+        return s.as_dict()
+
+    # ---- stencil path
+    def create_radstencil(self, stride: int, real_bytes: int = 8):
+        self._check(self.lib.emma_rt_create_radstencil(self.h, stride, real_bytes))
+
+    def advance_stencil(self, stencil: np.ndarray, nread: int, dxcur: float, dtnew: float, aexp: float = 1.0,
+                        chemonly: int = 0):
+        """Run the sweeps, update and chemistry on a host buffer holding
+        ``nread`` struct RGRID records (modified in place)."""
+        assert stencil.flags.c_contiguous
+        st = self.lib.emma_rt_advance_stencil(self.h, stencil.ctypes.data_as(ctypes.c_void_p), nread, dxcur, dtnew,
+                                              aexp, chemonly)
+        self._check(st, self.stats() if st < 0 and st != ERR_HIP else None)

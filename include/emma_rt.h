@@ -1,0 +1,160 @@
+/*
+ * emma_rt.h -- C-ABI of the MI355X (gfx950) M1 radiative-transfer + hydrogen
+ * chemistry library (libemma_rt.so).
+ *
+ * This is the boundary through which EMMA's C host reaches the GPU for the
+ * radiation step.  It replaces the reference's CUDA module
+ * (src/rad_utils_gpu.cu, src/chem_utils_gpu.cu, src/interface.cu, declared in
+ * src/rad_utils_gpu.h:1-6 and src/interface.h:1-13).  Signatures take plain
+ * pointers and sizes only.  Every entry point returns a status code
+ * (EMMA_RT_OK or a negative EMMA_RT_ERR_*); the library never aborts or exits
+ * where the reference calls abort()/exit().
+ *
+ * Two ways in:
+ *  1. Stencil path (drop-in for advanceradGPU's kernel sequence,
+ *     rad_utils_gpu.cu:1054-1086): the host keeps gathering struct RGRID
+ *     batches with gatherstencilrad and scattering them back with
+ *     scatterstencilrad; emma_rt_advance_stencil runs the three sweeps, the
+ *     update and the chemistry on the batch in the reference's own byte layout.
+ *  2. Resident-level path (fast path): the levels' radiation state lives on the
+ *     device as structure-of-arrays with index tables for the oct neighbours;
+ *     emma_rt_radsolver is RadSolver (rad_utils.c:1861-2022) for one level,
+ *     gather and scatter included, and emma_rt_coarse_step runs the whole
+ *     RADSTEP level recursion of one or more coarse steps as one HIP graph.
+ *
+ * Layout of the resident path (shared with the checker and the host mirror):
+ *   cell index = oct*8 + icell, icell = ix + 2 iy + 4 iz (oct.c:116-185);
+ *   group fields are group-major, e[g*ncell + cell];
+ *   nei_oct[oct*6 + face] >= 0 : same-level neighbour oct on that face,
+ *                          < 0 : the level-1 neighbour cell is a leaf whose
+ *                                cell index is -1 - nei_oct (it is injected
+ *                                into the stencil, rad_utils.c:1462-1476);
+ *   child[cell] >= 0 : the cell is split, index of its child oct at level+1.
+ */
+#ifndef EMMA_RT_H
+#define EMMA_RT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMMA_RT_MAX_GRP 9
+
+/* status codes */
+#define EMMA_RT_OK 0
+#define EMMA_RT_ERR_ARG (-1)        /* bad argument or size */
+#define EMMA_RT_ERR_HIP (-2)        /* HIP runtime error (WCUDA_ERR exit, gpu_type.h:4-15) */
+#define EMMA_RT_ERR_NEG_PHOTON (-3) /* N<0 or NaN after transport (abort, rad_utils.c:1611-1650) */
+#define EMMA_RT_ERR_EMIN (-4)       /* photon density below EMIN (abort, chem_utils.c:382-385) */
+#define EMMA_RT_ERR_NOCONV (-5)     /* chemistry sub-step underflow (abort, chem_utils.c:453-466) */
+#define EMMA_RT_ERR_NEIGHBOR (-6)   /* missing neighbour (abort, rad_utils.c:1441-1444) */
+#define EMMA_RT_ERR_LAYOUT (-7)     /* host struct layout differs from the library's */
+#define EMMA_RT_ERR_NODEVICE (-8)   /* no GPU visible */
+#define EMMA_RT_ERR_STATE (-9)      /* call out of order (no tree / no stencil buffer) */
+
+/* precision of the resident path */
+#define EMMA_RT_FP64 0  /* everything FP64 */
+#define EMMA_RT_MIXED 1 /* photon density, flux and source FP32; x, eint, T, n_H FP64 */
+
+/* Run parameters the path reads from struct RUNPARAMS (prototypes.h:299-413):
+   units (struct UNITS, prototypes.h:171-179), clight/fudgecool/ncvgcool
+   (prototypes.h:366-371), the atomic group table as readAtomic leaves it
+   (parameters.c:490-497: hnu in J, alphae/alphai in m^3/s at the reduced
+   light speed) and the COSMOPARAM entries chemrad uses (chem_utils.c:738). */
+typedef struct emma_rt_params {
+  int ngrp;
+  int ncvgcool;
+  double fudgecool;
+  double clight;
+  double unit_l, unit_v, unit_t, unit_n, unit_mass, unit_d, unit_N;
+  double hnu[EMMA_RT_MAX_GRP], alphae[EMMA_RT_MAX_GRP], alphai[EMMA_RT_MAX_GRP], factgrp[EMMA_RT_MAX_GRP];
+  double H0, om, ov;
+} emma_rt_params;
+
+typedef struct emma_rt_ctx emma_rt_ctx;
+
+/* Counters of the last radsolver / coarse-step call. */
+typedef struct emma_rt_stats {
+  long long cell_updates;   /* leaf cells advanced (one per leaf per level step) */
+  long long chem_substeps;  /* accepted chemistry sub-steps, summed over cells */
+  long long level_steps;    /* level-step kernel launches */
+  double kernel_ms;         /* summed device time of the level-step kernels (HIP events) */
+  double photons;           /* FP64 sum of N*dx^3 over leaf cells after the call (photon budget) */
+  int status;               /* first error raised on the device, EMMA_RT_OK if none */
+  int bad_level;            /* level and cell of that error */
+  int bad_cell;
+} emma_rt_stats;
+
+/* ---- device management: interface.cu:61-84 (countdevices, initlocaldevice,
+   checkdevice).  initlocaldevice binds rank % count, which the reference
+   meant to do but does not (interface.cu:73 always selects device 0). ---- */
+int emma_rt_countdevices(int rank, int *count);
+int emma_rt_initlocaldevice(int rank, int count);
+int emma_rt_checkdevice(int rank, int *device);
+const char *emma_rt_strerror(int status);
+
+/* ---- parameters: create_param_GPU (rad_utils_gpu.cu:57-106) makes a
+   one-time device copy of RUNPARAMS; here the context owns the copy and
+   emma_rt_set_params refreshes it (clight may change, advanceamr.c:198-215). */
+int emma_rt_create(const emma_rt_params *p, emma_rt_ctx **out);
+int emma_rt_set_params(emma_rt_ctx *ctx, const emma_rt_params *p);
+int emma_rt_destroy(emma_rt_ctx *ctx);
+
+/* ---- stencil path (drop-in for the kernels of advanceradGPU) ----
+   The caller's struct RGRID layout (prototypes.h:1165-1168) is described by
+   22 numbers: sizeof/offsetof of Rtype (e,fx,fy,fz,src,nhplus,eint,nh,temp),
+   CELLLIGHT_R (size, split), CELLFLUX_R (size, deltaR, rfield, rfieldnew,
+   rflux), OCTLIGHT_R, OCTFLUX_R, RGRID (size, oct, New).  The library checks
+   it against the layout it computes for (ngrp, real_bytes) and refuses a
+   mismatch with EMMA_RT_ERR_LAYOUT. */
+int emma_rt_rgrid_layout(int ngrp, int real_bytes, long *layout22);
+int emma_rt_check_layout(int ngrp, int real_bytes, const long *layout22);
+/* create_radstencil_GPU (rad_utils_gpu.cu:30-36): device buffer of `stride`
+   RGRIDs of REAL = double (real_bytes 8) or float (4). */
+int emma_rt_create_radstencil(emma_rt_ctx *ctx, int stride, int real_bytes);
+int emma_rt_destroy_radstencil(emma_rt_ctx *ctx);
+/* rad_utils_gpu.cu:1054-1086: copy `nread` RGRIDs host->device, run
+   rad_sweepX/Y/Z + updatefieldrad + chemrad semantics (CPU semantics of
+   rad_utils.c/chem_utils.c), copy back.  `stencil` is host memory in the
+   reference's RGRID layout; only New.cell[*].rfieldnew and rflux change. */
+int emma_rt_advance_stencil(emma_rt_ctx *ctx, void *stencil, int nread, double dxcur, double dtnew, double aexp,
+                            int chemonly);
+
+/* ---- resident-level path ---- */
+/* Device tree of levels lmin..lmax: noct[lmax-lmin+1]; nei_oct and child are
+   the levels' arrays concatenated in level order (6 and 8 ints per oct). */
+int emma_rt_tree_create(emma_rt_ctx *ctx, int lmin, int lmax, const int *noct, const int *nei_oct,
+                        const int *child, int precision);
+/* host SoA (double) <-> device state of one level (rfield, prototypes.h:849) */
+int emma_rt_set_level(emma_rt_ctx *ctx, int level, const double *e, const double *fx, const double *fy,
+                      const double *fz, const double *src, const double *nhplus, const double *eint,
+                      const double *nh, const double *temp);
+int emma_rt_get_level(emma_rt_ctx *ctx, int level, double *e, double *fx, double *fy, double *fz, double *src,
+                      double *nhplus, double *eint, double *nh, double *temp);
+/* reionisation times (rad_utils.c:1901-1913): enable per-cell t_first/t_last
+   bookkeeping (initialised to -1) and read them back */
+int emma_rt_enable_xion_times(emma_rt_ctx *ctx, int enable);
+int emma_rt_get_xion_times(emma_rt_ctx *ctx, int level, double *t_first, double *t_last);
+int emma_rt_set_tphy(emma_rt_ctx *ctx, double tphy);
+
+/* RadSolver (rad_utils.c:1861-2022) on one level, RADSTEP mode: transport
+   from the level's current state (coarse leaves injected), update, chemistry,
+   1/8 refluxing of coarse-fine faces into the coarser level, final update
+   (rfield := rfieldnew; split cells := mean of their 8 children).
+   Synchronous; returns the first device error as a status. */
+int emma_rt_radsolver(emma_rt_ctx *ctx, int level, double dtnew, double aexp, int chemonly);
+
+/* `ncoarse` coarse steps of the RADSTEP recursion (advanceamr.c:243-1300):
+   level l takes 2^(l-lmin) steps of dt_rad(l) (advanceamr.c:226) per coarse
+   step, finer levels first.  Captured once as a HIP graph and replayed. */
+int emma_rt_coarse_step(emma_rt_ctx *ctx, double aexp, int ncoarse);
+
+int emma_rt_get_stats(emma_rt_ctx *ctx, emma_rt_stats *out);
+/* record HIP events around every level-step kernel (adds the per-launch
+   device times to emma_rt_stats.kernel_ms) */
+int emma_rt_enable_timing(emma_rt_ctx *ctx, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define ORC_MAXGRP 8
+#define ORC_MAXGRP 9
 
 /* status codes, same numbering as the library's (include/emma_rt.h) */
 #define ORC_OK 0
