@@ -279,6 +279,7 @@ class Reference:
         L.refdrv_stencil_chem.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_double, ctypes.c_int]
         L.refdrv_set_tphy.argtypes = [ctypes.c_double]
+        L.refdrv_rates_n.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         self.stride = 4096
 
     def layout(self):
@@ -355,3 +356,27 @@ class Reference:
         out = np.empty(nhplus.size)
         self.lib.refdrv_e2t(nhplus.size, _d(nhplus), _d(eint), _d(nh), aexp, _d(out))
         return out
+
+    def rates(self, temp, x, nH):
+        temp, x, nH = [np.ascontiguousarray(np.broadcast_to(a, np.shape(temp)), dtype=np.float64) for a in (temp, x, nH)]
+        n = temp.size
+        outs = [np.empty(n) for _ in range(5)]
+        self.lib.refdrv_rates_n(n, _d(temp), _d(x), _d(nH), *[_d(o) for o in outs])
+        return dict(zip(("alpha_a", "alpha_b", "beta", "lambda", "tcool"), outs))
+
+    # struct RGRID batches (numpy arrays of emma_amd.rgrid.rgrid_dtype), in place
+    def stencil_sweeps(self, st, dx, dt, c):
+        self.lib.refdrv_stencil_sweeps(st.ctypes.data_as(ctypes.c_void_p), st.shape[0], dx, dt, c)
+
+    def stencil_update(self, st, dx, dt, c):
+        self.lib.refdrv_stencil_update(st.ctypes.data_as(ctypes.c_void_p), st.shape[0], dx, dt, c)
+
+    def stencil_chem(self, st, dx, dt, aexp=1.0, chemonly=0):
+        self.lib.refdrv_stencil_chem(st.ctypes.data_as(ctypes.c_void_p), st.shape[0], dx, dt, aexp, chemonly)
+
+    def stencil_advance(self, st, dx, dt, aexp=1.0, chemonly=0):
+        """advancerad's per-batch work (rad_utils.c:1814-1830) on a gathered batch"""
+        c = self.params.cloc(aexp)
+        self.stencil_sweeps(st, dx, dt, c)
+        self.stencil_update(st, dx, dt, c)
+        self.stencil_chem(st, dx, dt, aexp, chemonly)

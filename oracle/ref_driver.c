@@ -33,6 +33,11 @@ REAL Eddington(REAL fx, REAL fy, REAL fz, REAL ee, REAL c, int i, int j);
 int advancerad(struct OCT **firstoct, int level, struct CPUINFO *cpu, struct RGRID *stencil, int stride,
                REAL dxcur, REAL dtnew, REAL aexp, struct RUNPARAMS *param, int chemonly);
 
+REAL cucompute_alpha_b(REAL temp, REAL unit_number, REAL aexp);
+REAL cucompute_alpha_a(REAL temp, REAL unit_number, REAL aexp);
+REAL cucompute_beta(REAL temp, REAL unit_number, REAL aexp);
+void cuCompCooling(REAL temp, REAL x, REAL nH, REAL *lambda, REAL *tcool, REAL aexp, REAL CLUMPF, int src);
+
 #define MAXLEV 24
 
 static struct RUNPARAMS g_param;
@@ -311,5 +316,19 @@ void refdrv_e2t(int n, const double *nhplus, const double *eint, const double *n
     R.nh = nh[k];
     E2T(&R, aexp, &g_param);
     temp[k] = R.temp;
+  }
+}
+
+/* rate fits and cooling function of chem_utils.c:37-187, as chemrad calls them */
+void refdrv_rates_n(int n, const double *temp, const double *x, const double *nH, double *alpha_a, double *alpha_b,
+                    double *beta, double *lambda, double *tcool) {
+  for (int k = 0; k < n; k++) {
+    alpha_a[k] = cucompute_alpha_a(temp[k], 1., 1.);
+    alpha_b[k] = cucompute_alpha_b(temp[k], 1., 1.);
+    beta[k] = cucompute_beta(temp[k], 1., 1.);
+    REAL l, t;
+    cuCompCooling(temp[k], x[k], nH[k], &l, &t, 1., 1., 0);
+    lambda[k] = l;
+    tcool[k] = t;
   }
 }

@@ -1,0 +1,119 @@
+"""The oracle against the reference's own CPU functions, run live
+(oracle/_ref, built from /root/reference by `make -C oracle ref`).  Skipped
+where that build is absent.  Exact (bit-identical) agreement is required."""
+import numpy as np
+import pytest
+
+from emma_amd import synth
+from emma_amd.params import make_params
+from emma_amd.synth import Fields
+from oracle import Oracle, Reference, ref_available
+
+TABLES = {1: "SB99_equiv_mono_TH", 3: "Starburst99_1grpT_3grpS"}
+
+
+def need(G):
+    if not ref_available(G):
+        pytest.skip("reference build (oracle/_ref) not present")
+
+
+def same_state(o, r, tree):
+    for l in range(tree.lmin, tree.lmax + 1):
+        for which in (0, 1):
+            A, B = o.get(l, which), r.get(l, which)
+            for k in Fields.NAMES:
+                a, b = getattr(A, k), getattr(B, k)
+                assert np.array_equal(a, b), (l, which, k, np.count_nonzero(a != b))
+
+
+@pytest.mark.parametrize("G", [1, 3])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_amr_coarse_steps(G, seed):
+    need(G)
+    W = synth.amr_workload(3, [40 + 20 * seed, 90, 60], TABLES[G], seed=seed, sigma=1.5, nsrc=3, grid=16)
+    o, r = Oracle(), Reference(G)
+    o.setup(W.params, W.tree, W.fields)
+    r.setup(W.params, W.tree, W.fields)
+    for _ in range(3):
+        o.coarse_step()
+        r.coarse_step()
+    same_state(o, r, W.tree)
+
+
+@pytest.mark.parametrize("G", [1, 3])
+def test_expanding_cosmology(G):
+    """H0 != 0 and a != 1 exercise the a(t) update inside the sub-step loop
+    (chem_utils.c:737-756) and every unit conversion carrying powers of a."""
+    need(G)
+    W = synth.uniform_workload(3, TABLES[G], seed=4, nsrc=2)
+    W.params.H0, W.params.om, W.params.ov = 70.0, 0.3, 0.7
+    aexp = 0.125
+    rng = np.random.default_rng(44)
+    F = synth.random_fields(W.params, W.fields[3].nh, rng, aexp)  # |F| <= c(a) N at this a
+    F.src[:] = W.fields[3].src
+    W.fields[3] = F
+    o, r = Oracle(), Reference(G)
+    o.setup(W.params, W.tree, W.fields)
+    r.setup(W.params, W.tree, W.fields)
+    for _ in range(3):
+        for s in (o, r):
+            s.clean_new(3)
+            s.radsolver(3, W.params.dt_rad(3, aexp), aexp)
+    same_state(o, r, W.tree)
+
+
+@pytest.mark.parametrize("G", [1, 3])
+def test_chemonly(G):
+    """chemonly=1: the chemistry leaves N and F to the transport
+    (chem_utils.c:512-515, 744-748, 775-783)."""
+    need(G)
+    W = synth.uniform_workload(3, TABLES[G], seed=5)
+    o, r = Oracle(), Reference(G)
+    o.setup(W.params, W.tree, W.fields)
+    r.setup(W.params, W.tree, W.fields)
+    for s in (o, r):
+        s.clean_new(3)
+        s.radsolver(3, W.params.dt_rad(3), 1.0, 1)
+    same_state(o, r, W.tree)
+
+
+def test_ncvgcool_and_fudge():
+    """ncvgcool caps the accepted sub-steps; fudgecool < 1 shortens them."""
+    need(1)
+    W = synth.uniform_workload(3, TABLES[1], seed=6)
+    W.params.ncvgcool = 2
+    W.params.fudgecool = 0.3
+    o, r = Oracle(), Reference(1)
+    o.setup(W.params, W.tree, W.fields)
+    r.setup(W.params, W.tree, W.fields)
+    for s in (o, r):
+        s.clean_new(3)
+        s.radsolver(3, W.params.dt_rad(3))
+    same_state(o, r, W.tree)
+    assert o.nsub[3].max() <= 2
+
+
+@pytest.mark.parametrize("G", [1, 3])
+def test_eddington_and_rates_random(G):
+    need(G)
+    rng = np.random.default_rng(G)
+    o, r = Oracle(), Reference(G)
+    n = 5000
+    e = 10.0 ** rng.uniform(-12, 3, n)
+    F = rng.normal(size=(3, n)) * e * 0.1 * rng.uniform(0, 1.5, n)  # includes |F| > cE
+    i, j = rng.integers(0, 3, n), rng.integers(0, 3, n)
+    assert np.array_equal(o.eddington(F[0], F[1], F[2], e, 0.1, i, j), r.eddington(F[0], F[1], F[2], e, 0.1, i, j))
+    T = 10.0 ** rng.uniform(0.5, 8, n)
+    x = rng.uniform(0, 1, n)
+    nh = 10.0 ** rng.uniform(-4, 7, n)
+    a, b = o.rates(T, x, nh), r.rates(T, x, nh)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_reference_layout_matches_numpy_view():
+    from emma_amd import rgrid
+
+    for G in (1, 3):
+        need(G)
+        assert np.array_equal(Reference(G).layout(), rgrid.layout(G))
