@@ -150,6 +150,13 @@ static ChemConst make_chem(const emma_rt_params& p, double dtnew, double aexp) {
   K.H0 = p.H0;
   K.om = p.om;
   K.ov = p.ov;
+  K.k_e = p.unit_N / K.a3;
+  K.k_f = p.unit_l / p.unit_t * p.unit_N / K.pa4;
+  K.k_src = p.unit_N / p.unit_t / K.a2 / K.pa3;
+  K.k_eint = p.unit_n * p.unit_d * K.pv2 / K.pa5;
+  K.ki_e = K.pa3 / p.unit_N;
+  K.ki_f = K.pa4 / p.unit_l * p.unit_t / p.unit_N;
+  K.ki_eint = K.pa5 / (p.unit_n * p.unit_d * K.pv2);
   for (int g = 0; g < p.ngrp; g++) {
     K.heat[g] = p.alphae[g] * p.hnu[g] - p.alphai[g] * kHnu0;
     K.alphai[g] = p.alphai[g];
@@ -224,13 +231,13 @@ extern "C" int emma_rt_create(const emma_rt_params* p, emma_rt_ctx** out) {
   memset(&c->stats, 0, sizeof(c->stats));
   if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
       hipMalloc((void**)&c->d_status, 4 * sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&c->d_counters, 2 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc((void**)&c->d_counters, kCounterSlots * kCounterStride * sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc((void**)&c->d_photons, sizeof(double)) != hipSuccess) {
     delete c;
     return EMMA_RT_ERR_HIP;
   }
   if (launch_fill((unsigned int*)c->d_status, 4, 0u, c->stream) != hipSuccess ||
-      launch_fill((unsigned int*)c->d_counters, 4, 0u, c->stream) != hipSuccess ||
+      launch_fill((unsigned int*)c->d_counters, 2 * kCounterSlots * kCounterStride, 0u, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess) {
     delete c;
     return EMMA_RT_ERR_HIP;
@@ -590,7 +597,7 @@ static int timed_launch(emma_rt_ctx* c, int k, double dtnew, double aexp, int ch
 
 static int begin_call(emma_rt_ctx* c) {
   HIPCHK(launch_fill((unsigned int*)c->d_status, 4, 0u, c->stream));
-  HIPCHK(launch_fill((unsigned int*)c->d_counters, 4, 0u, c->stream));
+  HIPCHK(launch_fill((unsigned int*)c->d_counters, 2 * kCounterSlots * kCounterStride, 0u, c->stream));
   memset(&c->stats, 0, sizeof(c->stats));
   c->ev_used = 0;
   return EMMA_RT_OK;
@@ -599,9 +606,14 @@ static int begin_call(emma_rt_ctx* c) {
 static int end_call(emma_rt_ctx* c) {
   HIPCHK(hipStreamSynchronize(c->stream));
   int status[4];
-  unsigned long long cnt[2];
+  std::vector<unsigned long long> slots(kCounterSlots * kCounterStride);
   HIPCHK(hipMemcpy(status, c->d_status, sizeof(status), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(cnt, c->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(slots.data(), c->d_counters, slots.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  unsigned long long cnt[2] = {0ull, 0ull};
+  for (int i = 0; i < kCounterSlots; i++) {
+    cnt[0] += slots[(size_t)i * kCounterStride + 0];
+    cnt[1] += slots[(size_t)i * kCounterStride + 1];
+  }
   c->stats.status = status[0];
   c->stats.bad_level = status[1];
   c->stats.bad_cell = status[2];

@@ -61,8 +61,14 @@ struct LevelArgs {
   // numerics
   R c, hc, dtsurdx;
   int* status;                   // [code, level, cell]
-  unsigned long long* counters;  // [accepted sub-steps, leaf cell-updates] or null
+  unsigned long long* counters;  // kCounterSlots x [accepted sub-steps, leaf cell-updates] or null
 };
+
+// Counters are spread over kCounterSlots cache lines (block b adds into slot
+// b % kCounterSlots) so that thousands of blocks do not serialise on one
+// address; the host sums the slots.
+constexpr int kCounterSlots = 256;
+constexpr int kCounterStride = 16;  // unsigned long longs per slot (128 B)
 
 template <typename R>
 hipError_t launch_level_step(const LevelArgs<R>& a, const ChemConst& K, hipStream_t s);

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include "rt_kernels.h"
+#include "rt_physics_fast.h"
 
 namespace emma {
 
@@ -46,7 +47,9 @@ __device__ __forceinline__ T sel(int i, T a, T b, T c) {
 }
 
 // ------------------------------------------------------------------ level step
-template <typename R, int G>
+// Fast selects the mixed-precision physics (rt_physics_fast.h); the FP64
+// kernel (Fast = false) keeps the reference's operation order.
+template <typename R, int G, bool Fast>
 __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a, ChemConst K) {
   constexpr int GA = G > 0 ? G : kMaxGrp;
   const int ng = G > 0 ? G : a.ng;
@@ -118,7 +121,10 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a, ChemCon
           Ux[g] = a.fx[g * n + cell];
           Uy[g] = a.fy[g * n + cell];
           Uz[g] = a.fz[g * n + cell];
-          Q[g] = closure<R>(Ux[g], Uy[g], Uz[g], Ue[g], c);
+          if constexpr (Fast)
+            Q[g] = closure_fast(Ux[g], Uy[g], Uz[g], Ue[g], c);
+          else
+            Q[g] = closure<R>(Ux[g], Uy[g], Uz[g], Ue[g], c);
           De[g] = R(0);
           Dx[g] = R(0);
           Dy[g] = R(0);
@@ -163,7 +169,11 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a, ChemCon
           if (g < ng) {
             R ve, vx, vy, vz;
             load4(a, Ne, Nx, Ny, Nz, nn, j, g, ve, vx, vy, vz);
-            const Closure<R> qn = closure<R>(vx, vy, vz, ve, c);
+            Closure<R> qn;
+            if constexpr (Fast)
+              qn = closure_fast(vx, vy, vz, ve, c);
+            else
+              qn = closure<R>(vx, vy, vz, ve, c);
             R F[4];
             lf_flux<R>(d, s, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], ve, vx, vy, vz, qn, hc, F);
 #pragma unroll
@@ -225,7 +235,11 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a, ChemCon
       const double nh = a.nh[cell];
       double temp = 0.;
       int nit = 0;
-      const int st = chem_cell<G>(K, ng, e, fx, fy, fz, src, nhplus, eint, nh, temp, a.chemonly, nit);
+      int st;
+      if constexpr (Fast)
+        st = chem_cell_fast<G>(K, ng, e, fx, fy, fz, src, nhplus, eint, nh, temp, a.chemonly, nit);
+      else
+        st = chem_cell<G>(K, ng, e, fx, fy, fz, src, nhplus, eint, nh, temp, a.chemonly, nit);
       if (st != 0) raise_status(a.status, st, a.level, cell);
       nit_acc = (unsigned)nit;
 #pragma unroll
@@ -257,8 +271,9 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a, ChemCon
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      atomicAdd(a.counters + 0, (unsigned long long)(red[0][0] + red[0][1] + red[0][2] + red[0][3]));
-      atomicAdd(a.counters + 1, (unsigned long long)(red[1][0] + red[1][1] + red[1][2] + red[1][3]));
+      unsigned long long* slot = a.counters + (size_t)(blockIdx.x % kCounterSlots) * kCounterStride;
+      atomicAdd(slot + 0, (unsigned long long)(red[0][0] + red[0][1] + red[0][2] + red[0][3]));
+      atomicAdd(slot + 1, (unsigned long long)(red[1][0] + red[1][1] + red[1][2] + red[1][3]));
     }
   }
 }
@@ -297,10 +312,11 @@ hipError_t launch_level_step(const LevelArgs<R>& a0, const ChemConst& K, hipStre
   nb = (nb + 7) & ~7;
   a.nblocks = nb;
   const dim3 grid(nb), block(256);
+  constexpr bool fast = sizeof(R) == 4;
   switch (a.ng) {
-    case 1: hipLaunchKernelGGL((level_step_kernel<R, 1>), grid, block, 0, s, a, K); break;
-    case 3: hipLaunchKernelGGL((level_step_kernel<R, 3>), grid, block, 0, s, a, K); break;
-    default: hipLaunchKernelGGL((level_step_kernel<R, 0>), grid, block, 0, s, a, K); break;
+    case 1: hipLaunchKernelGGL((level_step_kernel<R, 1, fast>), grid, block, 0, s, a, K); break;
+    case 3: hipLaunchKernelGGL((level_step_kernel<R, 3, fast>), grid, block, 0, s, a, K); break;
+    default: hipLaunchKernelGGL((level_step_kernel<R, 0, fast>), grid, block, 0, s, a, K); break;
   }
   return hipGetLastError();
 }

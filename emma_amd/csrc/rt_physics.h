@@ -46,6 +46,15 @@ struct ChemConst {
   double alphai[kMaxGrp];
   double factgrp[kMaxGrp];
   double emin_grp[kMaxGrp];            // EMIN*factgrp/unit_N
+  // products of the conversion factors above, for the mixed-precision path
+  // (which multiplies where the FP64 path divides)
+  double k_e;      // unit_N / a^3            code N -> photons/m^3
+  double k_f;      // unit_l/unit_t*unit_N/a^4
+  double k_src;    // unit_N/unit_t/a^2/a^3
+  double k_eint;   // unit_n*unit_d*unit_v^2/a^5
+  double ki_e;     // a^3/unit_N
+  double ki_f;     // a^4/unit_l*unit_t/unit_N
+  double ki_eint;  // a^5/(unit_n*unit_d*unit_v^2)
 };
 
 // ---------------------------------------------------------------- Eddington

@@ -1,0 +1,223 @@
+// rt_physics_fast.h -- the mixed-precision variant of the radiation step
+// physics: FP32 transport with hardware reciprocal / reciprocal square root,
+// FP32 rate fits through the hardware exp2/log2, and the chemistry sub-step
+// update (x, eint, photon absorption, accept/reject decisions) in FP64 with
+// reciprocal-and-Newton division.  Same equations and control flow as the
+// FP64 path in rt_physics.h; only the rounding differs.
+#pragma once
+
+#include "rt_physics.h"
+
+namespace emma {
+
+__device__ __forceinline__ float frcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float flog2(float x) { return __builtin_amdgcn_logf(x); }
+
+// FP64 reciprocal: hardware estimate refined by two Newton steps (~1 ulp)
+__device__ __forceinline__ double frcp64(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  r = fma(fma(-b, r, 1.0), r, r);
+  return r;
+}
+
+__device__ __forceinline__ double fdiv64(double a, double b) {
+  const double r = frcp64(b);
+  const double q = a * r;
+  return fma(fma(-b, q, a), r, q);
+}
+
+// Levermore closure (rad_utils.c:448-475) with fast FP32 reciprocals
+__device__ __forceinline__ Closure<float> closure_fast(float fx, float fy, float fz, float ee, float c) {
+  Closure<float> q;
+  const float c2e = ee * c * c;
+  float ff = 0.f;
+  q.n[0] = q.n[1] = q.n[2] = 0.f;
+  if (ee > 0.f) {
+    const float f2 = fx * fx + fy * fy + fz * fz;
+    if (f2 > 1e-36f) {
+      const float ri = __builtin_amdgcn_rsqf(f2);
+      q.n[0] = fx * ri;
+      q.n[1] = fy * ri;
+      q.n[2] = fz * ri;
+      ff = f2 * ri * frcp32(c * ee);
+    }
+  }
+  const float ff2 = ff * ff;
+  const float arg = fmaxf(4.f - 3.f * ff2, 0.f);
+  const float chi = (3.f + 4.f * ff2) * frcp32(5.f + 2.f * __builtin_amdgcn_sqrtf(arg));
+  q.diag = 0.5f * (1.f - chi) * c2e;
+  q.arg = 0.5f * (3.f * chi - 1.f) * c2e;
+  return q;
+}
+
+// the rate fits of chem_utils.c:37-187 in FP32: pow(x, a) = exp2(a log2 x)
+// with log2(lambda) shared, exp(-E/T) = exp2(-E log2(e) / T)
+__device__ __forceinline__ Rates rates_fast(double Td) {
+  const float T = (float)Td;
+  const float rT = frcp32(T);
+  const float sT = __builtin_amdgcn_sqrtf(T);
+  const float sT5 = sT * 3.16227766e-3f;  // sqrt(T/1e5)
+  const float inv1 = frcp32(1.f + sT5);
+  const float lam = 315614.f * rT;
+  const float l2 = flog2(lam);
+  const float LOG2E = 1.44269504f;
+  Rates r;
+  const float u6 = fexp2(0.376f * (l2 - 1.16992500f));   // (lam/2.25)^0.376
+  r.A6 = (double)(3.435e-30f * T * fexp2(1.970f * l2 - 3.720f * flog2(1.f + u6)));
+  const float ub = fexp2(0.407f * (l2 - 1.45417589f));   // (lam/2.74)^0.407
+  r.alphab = (double)(2.753e-20f * fexp2(1.5f * l2 - 2.242f * flog2(1.f + ub)));
+  r.A1 = (double)(fexp2(-157809.1f * LOG2E * rT) * 2.54e-21f * sT * inv1);
+  r.A3 = (double)(fexp2(-118348.f * LOG2E * rT) * 7.5e-19f * inv1);
+  r.A4 = (double)(2.13e-27f * sT);
+  r.beta = (double)(1.17e-16f * sT * inv1 * fexp2(-157809.f * LOG2E * rT));
+  return r;
+}
+
+// chemrad for one unsplit cell, mixed-precision variant of chem_cell<G>
+template <int G>
+__device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fx, double* fy, double* fz,
+                              const double* src, double& nhplus, double& eint_code, double nh, double& temp,
+                              int chemonly, int& nit) {
+  constexpr int GA = G > 0 ? G : kMaxGrp;
+  double egy[GA], fl0[GA], fl1[GA], fl2[GA], srcl[GA], et[GA], p[GA];
+  const double notchem = (double)(!chemonly);
+#pragma unroll
+  for (int g = 0; g < GA; g++) {
+    if (g < ng) {
+      egy[g] = e[g] * K.k_e;
+      fl0[g] = fx[g] * K.k_f;
+      fl1[g] = fy[g] * K.k_f;
+      fl2[g] = fz[g] * K.k_f;
+      srcl[g] = src[g] * K.k_src;
+    }
+  }
+  const double rnh = frcp64(nh);
+  double x0 = nhplus * rnh;
+  const double nH = nh * K.k_e;
+  const double rnH = frcp64(nH);
+  double eint = eint_code * K.k_eint;
+  nit = 0;
+  if (egy[0] < kEmin) return -4;
+  double aexp = K.aexp;
+  double fudgecool = K.fudgecool;
+  double tcur = 0.;
+  const double dt = K.dt;
+  const double kT = 1. / (1.5 * kKboltz);
+  double tcached = -1.;
+  Rates r = rates_fast(1e4);
+  int tries = 0;
+  while (tcur < dt) {
+    if (++tries > kMaxTries) return -5;
+    const double tloc = eint * kT * frcp64(1. + x0) * rnH;
+    if (!(tloc == tcached)) {
+      r = rates_fast(tloc);
+      tcached = tloc;
+    }
+    const double Cool0 = cooling(r, x0, nH);
+    if (fudgecool < 1e-20) return -5;
+    double ai = 0.;
+#pragma unroll
+    for (int g = 0; g < GA; g++)
+      if (g < ng) ai += K.heat[g] * egy[g];
+    ai *= nH * (1.0 - x0);
+    const double tcool = fabs(fdiv64(eint, ai * notchem - Cool0));
+    const double dtcool = fmin(fudgecool * tcool, dt - tcur);
+    int test = 0;
+#pragma unroll
+    for (int g = 0; g < GA; g++) {
+      if (g < ng) {
+        const double aig = K.alphai[g] * (1. - x0) * nH * dtcool;
+        p[g] = 1. + aig;
+        et[g] = chemonly ? egy[g] : (egy[g] + srcl[g] * dtcool * K.factgrp[g]) * frcp64(p[g]);
+        if ((et[g] < 0) || isnan(et[g])) test = 1;
+      }
+    }
+    if (test) {
+      fudgecool *= 0.1;
+      continue;
+    }
+    ai = 0.;
+#pragma unroll
+    for (int g = 0; g < GA; g++)
+      if (g < ng) ai += K.alphai[g] * et[g];
+    ai *= notchem;
+    const double RECT = r.alphab * x0 * x0 * nH * dtcool;
+    const double xt = 1. - fdiv64(RECT + (1.0 - x0), 1. + dtcool * (r.beta * x0 * nH + ai));
+    if (((xt > 1.) || (xt < 0.)) || isnan(xt)) {
+      fudgecool *= 0.1;
+      continue;
+    }
+    const double Cool1 = cooling(r, xt, nH);
+    ai = 0.;
+#pragma unroll
+    for (int g = 0; g < GA; g++)
+      if (g < ng) ai += K.heat[g] * et[g];
+    ai *= nH * (1.0 - xt);
+    double eintt = eint + dtcool * (ai * notchem - Cool1);
+    if (eintt < 0.) {
+      fudgecool *= 0.1;
+      continue;
+    }
+    if (fabs(fdiv64(eintt, eint) - 1.) > kFracVar) {
+      fudgecool *= 0.1;
+      continue;
+    }
+    eintt = fmax(K.emin, eintt);
+    double r3 = 1., r4 = 1., r5 = 1.;
+    if (!K.static_cosmo) {  // chem_utils.c:737-756
+      const double aold = aexp;
+      const double hubblet = K.H0 * sqrt(K.om / aexp + K.ov * (aexp * aexp)) / aexp * (1e3 / (1e6 * kParsec));
+      aexp += hubblet * dtcool * aexp;
+      const double ra = aold / aexp;
+      r3 = ra * ra * ra;
+      r4 = r3 * ra;
+      r5 = r4 * ra;
+    }
+#pragma unroll
+    for (int g = 0; g < GA; g++) {
+      if (g < ng) {
+        egy[g] = et[g] * r3;
+        if (!chemonly) {
+          const double q = frcp64(p[g]) * r4;
+          fl0[g] *= q;
+          fl1[g] *= q;
+          fl2[g] *= q;
+        }
+      }
+    }
+    eint = eintt * r5;
+    x0 = xt;
+    tcur += dtcool;
+    fudgecool = K.fudgecool;
+    nit++;
+    if ((nit == K.ncvgcool) && (K.ncvgcool != 0)) break;
+  }
+  double ki_e = K.ki_e, ki_f = K.ki_f, ki_eint = K.ki_eint, a3f = K.pa3;
+  if (!K.static_cosmo) {
+    const double ra = aexp / K.aexp;
+    a3f = K.pa3 * ra * ra * ra;
+    ki_e *= ra * ra * ra;
+    ki_f *= ra * ra * ra * ra;
+    ki_eint *= ra * ra * ra * ra * ra;
+  }
+  if (!chemonly) {
+#pragma unroll
+    for (int g = 0; g < GA; g++) {
+      if (g < ng) {
+        e[g] = fmax(egy[g] * ki_e, K.emin_grp[g]);
+        fx[g] = fl0[g] * ki_f;
+        fy[g] = fl1[g] * ki_f;
+        fz[g] = fl2[g] * ki_f;
+      }
+    }
+  }
+  nhplus = x0 * nh;
+  eint_code = eint * ki_eint;
+  // E2T: eint_code back in physical units is eint itself
+  temp = eint * kT * frcp64((nh * K.unit_N / a3f) * (1. + x0));
+  return 0;
+}
+
+}  // namespace emma

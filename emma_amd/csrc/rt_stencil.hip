@@ -6,6 +6,7 @@
 // gatherstencilrad / scatterstencilrad keep working unchanged.
 #include <hip/hip_runtime.h>
 
+#include "rt_kernels.h"
 #include "rt_stencil.h"
 
 namespace emma {
@@ -135,8 +136,9 @@ __global__ __launch_bounds__(256) void stencil_kernel(char* base, StencilGeo geo
   rn[5 * G + 1] = (R)eint;
   rn[5 * G + 3] = (R)temp;
   if (counters != nullptr) {
-    atomicAdd(counters + 0, (unsigned long long)nit);
-    atomicAdd(counters + 1, 1ull);
+    unsigned long long* slot = counters + (size_t)(blockIdx.x % kCounterSlots) * kCounterStride;
+    atomicAdd(slot + 0, (unsigned long long)nit);
+    atomicAdd(slot + 1, 1ull);
   }
 }
 
