@@ -1,17 +1,25 @@
 """Benchmark of the M1 + chemistry radiation step on one MI355X.
 
-A step is one radiative sub-cycle of the workload's level (every leaf cell
-advanced once: transport, update and chemistry).  Default workload: 128^3
-periodic level (level 7, 2,097,152 cells), 3 photon groups, 64 point
-sources, seeded lognormal density; mixed precision (FP32 photon density and
-flux, FP64 chemistry).  --steps sub-cycles run in the timed region.
+Workloads (seeded synthetic states; see emma_amd/synth.py):
+  c3 (default) 128^3 periodic level (level 7, 2,097,152 cells), 3 photon groups,
+               64 point sources, lognormal n_H; mixed precision.  A step is one
+               radiative sub-cycle of the level.
+  c2           64^3 periodic level, 1 group; FP64 plain.  A step is one sub-cycle.
+  c4           AMR levels 7-11 with 262144/80000/25000/6000/1200 octs, 3 groups,
+               mixed precision.  A step is one coarse step of the RADSTEP level
+               recursion (level l takes 2^(l-7) sub-steps), replayed as one HIP graph.
 
-Prints ONE JSON line (see the keys below).  value = cell-updates/s of the
-whole job with the state already resident in HBM.  roofline.achieved is the
-algorithmic bytes of one launch of the level-step kernel divided by its mean
-device time (HIP events around every launch on the library's stream).
-cpu_baseline times the oracle (the CPU restatement of the reference path,
-one core) on a bounded sample of the same workload.
+Prints ONE JSON line.  value = leaf cell-updates/s of the timed region (state
+resident in HBM; the library call is synchronous, so the wall clock brackets
+complete device work).  roofline.achieved = algorithmic bytes of the level-step
+launches / their summed device time (HIP events around every launch on the
+library's stream; for c4 from an event-timed eager pass of one coarse step right
+after the graph-timed region).  fp64_plain: the same workload stepped by the FP64
+kernel.  cpu_baseline: the oracle (the CPU restatement of the reference path,
+one core) on a bounded sample drawn like the workload.
+
+--calibrate-copy runs a plain device copy of known size (for scaling the
+FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc runs).
 """
 from __future__ import annotations
 
@@ -39,16 +47,25 @@ def make_workload(name: str, seed: int):
         return synth.config_c3(seed=seed, nsub=1)
     if name == "c2":
         return synth.config_c2(seed=seed)
+    if name == "c4":
+        return synth.config_c4(seed=seed, nsub=1)
     raise SystemExit(f"unknown workload {name}")
+
+
+WORKLOAD_NAMES = {
+    "c3": "128^3 periodic level (2,097,152 cells), 3 groups, 64 sources, lognormal n_H",
+    "c2": "64^3 periodic level (262,144 cells), 1 group, lognormal n_H",
+    "c4": "AMR levels 7-11 (262144/80000/25000/6000/1200 octs), 3 groups, lognormal sigma=1.5",
+}
 
 
 def cpu_baseline(name: str, seconds: float = 10.0):
     """Oracle (scalar C restatement, 1 core) on a level-5 sample (32^3 cells)
-    drawn like the workload; sub-cycles repeat until `seconds` have passed."""
+    drawn like the workload; restarted from the drawn state every 4 sub-cycles."""
     from emma_amd import synth
     from oracle import Oracle
 
-    if name == "c3":
+    if name in ("c3", "c4"):
         W = synth.uniform_workload(5, "Starburst99_1grpT_3grpS", seed=3, nsrc=1)
     else:
         W = synth.uniform_workload(5, "SB99_equiv_mono_TH", seed=2)
@@ -56,8 +73,7 @@ def cpu_baseline(name: str, seconds: float = 10.0):
     dt = W.params.dt_rad(5)
     n = 0
     el = 0.0
-    # restart from the drawn state every 4 sub-cycles: without sources the
-    # sample drains to the EMIN floor, where the reference aborts (chem_utils.c:382)
+    # without sources the sample drains to the EMIN floor, where the reference aborts (chem_utils.c:382)
     while el < seconds:
         o.setup(W.params, W.tree, W.fields)
         t0 = time.perf_counter()
@@ -76,12 +92,76 @@ def cpu_baseline(name: str, seconds: float = 10.0):
     }
 
 
-def load_traffic(workload: str):
-    p = os.path.join(ROOT, "profiles", f"level_step_{workload}_traffic.json")
+def load_traffic(key: str):
+    p = os.path.join(ROOT, "profiles", f"level_step_{key}_traffic.json")
     if os.path.exists(p):
         with open(p) as f:
             return json.load(f).get("bytes_per_launch")
     return None
+
+
+def level_bytes(W, precision):
+    """algorithmic bytes of one step of each level (leaf cells x bytes/cell)"""
+    Bc = alg_bytes_per_cell(W.params.ngrp, precision)
+    return {l: W.tree.leaf_cells(l) * Bc for l in range(W.tree.lmin, W.tree.lmax + 1)}
+
+
+def run_uniform(W, precision, steps, warmup):
+    from emma_amd import rt
+
+    lvl = W.tree.lmin
+    g = rt.RadGPU(W.params, rt.MIXED if precision == "mixed" else rt.FP64)
+    g.setup(W.params, W.tree, W.fields)
+    dt = W.params.dt_rad(lvl)
+    for _ in range(warmup):
+        g.radsolver(lvl, dt)
+    g.enable_timing(True)
+    t0 = time.perf_counter()
+    g.coarse_step(1.0, steps)  # single level: `steps` sub-cycles, one launch each
+    wall = time.perf_counter() - t0
+    st = g.stats()
+    kernel_s = st["kernel_ms"] * 1e-3
+    bytes_total = level_bytes(W, precision)[lvl] * st["level_steps"]
+    return wall, st, kernel_s, bytes_total, st["level_steps"]
+
+
+def run_amr(W, precision, steps, warmup):
+    from emma_amd import rt
+
+    g = rt.RadGPU(W.params, rt.MIXED if precision == "mixed" else rt.FP64)
+    g.setup(W.params, W.tree, W.fields)
+    for _ in range(warmup):
+        g.coarse_step(1.0, 1)
+    # capture + instantiate the graph(s) for `steps` coarse steps (two calls: an
+    # odd count flips the coarsest level's buffer parity, keyed separately)
+    g.coarse_step(1.0, steps)
+    g.coarse_step(1.0, steps)
+    t0 = time.perf_counter()
+    g.coarse_step(1.0, steps)  # timed: one graph replay
+    wall = time.perf_counter() - t0
+    st = g.stats()
+    # event-timed eager pass of one coarse step for the per-launch device times
+    g.enable_timing(True)
+    g.coarse_step(1.0, 1)
+    ev = g.stats()
+    lb = level_bytes(W, precision)
+    per_coarse = sum(b << (l - W.tree.lmin) for l, b in lb.items())
+    return wall, st, ev["kernel_ms"] * 1e-3, per_coarse, ev["level_steps"]
+
+
+def copy_calibration(nbytes: int = 1 << 30, reps: int = 10):
+    import torch
+
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps,
+                      "GBps": 2 * nbytes * reps / el / 1e9}), flush=True)
 
 
 def main():
@@ -89,44 +169,36 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--workload", default="c3", choices=["c3", "c2"])
+    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c4"])
     ap.add_argument("--precision", default=None, choices=["mixed", "fp64"])
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-plain", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: the radiation step is measured on one GPU's shard "
-              "(multi-GPU halo exchange is not part of this build)", file=sys.stderr)
+        print("bench.py measures one GPU: this build runs the radiation step on one GPU's shard "
+              "(the multi-GPU halo exchange is not part of it)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        copy_calibration()
+        return
 
-    from emma_amd import rt
-
-    precision = args.precision or ("mixed" if args.workload == "c3" else "fp64")
+    precision = args.precision or ("fp64" if args.workload == "c2" else "mixed")
     W = make_workload(args.workload, args.seed)
-    lvl = W.tree.lmin
-    G = W.params.ngrp
-    ncell = W.tree.level(lvl).ncell
-    g = rt.RadGPU(W.params, rt.MIXED if precision == "mixed" else rt.FP64)
-    g.setup(W.params, W.tree, W.fields)
-    dt = W.params.dt_rad(lvl)
-    for _ in range(args.warmup):
-        g.radsolver(lvl, dt)
-    # timed region: K sub-cycles, HIP events around every level-step launch
-    g.enable_timing(True)
-    t0 = time.perf_counter()
-    g.coarse_step(1.0, args.steps)
-    t1 = time.perf_counter()
-    st = g.stats()
-    wall = t1 - t0
+    runner = run_amr if args.workload == "c4" else run_uniform
+    wall, st, kernel_s, bytes_per_unit, launches = runner(W, precision, args.steps, args.warmup)
     cell_updates = st["cell_updates"]
     value = cell_updates / wall
-    Bc = alg_bytes_per_cell(G, precision)
-    launches = st["level_steps"]
-    mean_ms = st["kernel_ms"] / max(launches, 1)
-    achieved = Bc * ncell / (mean_ms * 1e-3) / 1e9
+    if args.workload == "c4":
+        achieved = bytes_per_unit / kernel_s / 1e9  # one eager coarse step
+        mean_launch_us = kernel_s / max(launches, 1) * 1e6
+    else:
+        achieved = bytes_per_unit / kernel_s / 1e9
+        mean_launch_us = kernel_s / max(launches, 1) * 1e6
     out = {
-        "metric": "M1+chem cell-updates/s (1 GPU) and % HBM peak",
+        "metric": "M1+chem cell-updates/s (mixed+batched vs FP64 plain, 1 GPU) and % HBM peak",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -139,13 +211,13 @@ def main():
         "dtype": "fp32+fp64" if precision == "mixed" else "fp64",
         "data": "synthetic",
         "config": {
-            "workload": "128^3 periodic level, 3 groups, 64 sources, lognormal n_H (C3)" if args.workload == "c3"
-            else "64^3 periodic level, 1 group (C2)",
-            "cells": ncell,
-            "groups": G,
+            "workload": WORKLOAD_NAMES[args.workload],
+            "cells": sum(W.tree.level(l).ncell for l in range(W.tree.lmin, W.tree.lmax + 1)),
+            "groups": W.params.ngrp,
             "precision": precision,
-            "substeps_per_cell_update": st["chem_substeps"] / max(cell_updates, 1),
-            "kernel_ms_per_step": mean_ms,
+            "step": "coarse step (31 level steps, HIP graph)" if args.workload == "c4" else "radiative sub-cycle",
+            "chem_substeps_per_cell_update": st["chem_substeps"] / max(cell_updates, 1),
+            "mean_kernel_us": mean_launch_us,
             "parallelism": "1 GPU",
         },
         "roofline": {
@@ -155,9 +227,14 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": load_traffic(f"{args.workload}_{precision}"),
-            "alg_bytes_per_cell_update": Bc,
+            "alg_bytes_per_cell_update": alg_bytes_per_cell(W.params.ngrp, precision),
         },
     }
+    if precision == "mixed" and not args.no_fp64_plain:
+        nsteps = max(4, args.steps // 4)
+        w2, st2, _, _, _ = runner(W, "fp64", nsteps, 1)
+        v2 = st2["cell_updates"] / w2
+        out["fp64_plain"] = {"value": v2, "unit": "cell-updates/s", "steps": nsteps, "speedup_of_value": value / v2}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
     print(json.dumps(out), flush=True)

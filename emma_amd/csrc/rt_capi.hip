@@ -57,7 +57,9 @@ struct GraphKey {
   double aexp;
   std::vector<int> parity;
   int xion;
+  int mode;
   bool operator<(const GraphKey& o) const {
+    if (mode != o.mode) return mode < o.mode;
     if (ncoarse != o.ncoarse) return ncoarse < o.ncoarse;
     if (aexp != o.aexp) return aexp < o.aexp;
     if (xion != o.xion) return xion < o.xion;
@@ -72,6 +74,7 @@ struct emma_rt_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int precision = EMMA_RT_FP64;
+  int mode = EMMA_RT_RADSTEP;
   int rbytes = 8;
   int lmin = 0, lmax = -1;
   bool has_tree = false;
@@ -279,7 +282,7 @@ static hipError_t dalloc(T** p, size_t n) {
 extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int* noct, const int* nei_oct,
                                    const int* child, int precision) {
   if (c == nullptr || noct == nullptr || nei_oct == nullptr || child == nullptr) return EMMA_RT_ERR_ARG;
-  if (lmin < 1 || lmax < lmin || lmax > 30) return EMMA_RT_ERR_ARG;
+  if (lmin < 1 || lmax < lmin || lmax > 30 || lmax - lmin + 1 > kMaxLevels) return EMMA_RT_ERR_ARG;
   if (precision != EMMA_RT_FP64 && precision != EMMA_RT_MIXED) return EMMA_RT_ERR_ARG;
   free_tree(c);
   const int nl = lmax - lmin + 1;
@@ -502,12 +505,22 @@ static LevelArgs<R> level_args(emma_rt_ctx* c, int k, double dtnew, double aexp,
   A.ng = c->p.ngrp;
   A.level = L.level;
   A.chemonly = chemonly;
+  if (c->mode == EMMA_RT_COARSERAD) {
+    A.transport = (k == 0) && !chemonly;
+    A.keep_split_flux = 1;
+    A.update_split = A.transport;
+  } else {
+    A.transport = 1;
+    A.keep_split_flux = 0;
+    A.update_split = 0;
+  }
   A.e = (const R*)L.e[a];
   A.fx = (const R*)L.fx[a];
   A.fy = (const R*)L.fy[a];
   A.fz = (const R*)L.fz[a];
   A.nhplus = L.nhplus[a];
   A.eint = L.eint[a];
+  A.temp = L.temp[a];
   A.src = (R*)L.src;
   A.nh = L.nh;
   A.eo = (R*)L.e[b];
@@ -629,6 +642,56 @@ static int end_call(emma_rt_ctx* c) {
   return status[0];
 }
 
+template <typename R>
+static TreeView<R> tree_view(emma_rt_ctx* c) {
+  TreeView<R> t;
+  memset(&t, 0, sizeof(t));
+  t.nlev = (int)c->lev.size();
+  t.ng = c->p.ngrp;
+  t.lcoarse = c->lmin;
+  for (int k = 0; k < t.nlev; k++) {
+    DevLevel& L = c->lev[k];
+    t.ncell[k] = L.ncell;
+    t.child[k] = L.child;
+    t.e[k] = (R*)L.e[L.cur];
+    t.fx[k] = (R*)L.fx[L.cur];
+    t.fy[k] = (R*)L.fy[L.cur];
+    t.fz[k] = (R*)L.fz[L.cur];
+  }
+  return t;
+}
+
+// FillRad's COARSERAD block (src_utils.c:717-733): split cells of the coarse
+// level gather the volume-weighted E, F of their leaves
+static hipError_t launch_put(emma_rt_ctx* c) {
+  if (c->rbytes == 8) return launch_put_e2coarse<double>(tree_view<double>(c), c->stream);
+  return launch_put_e2coarse<float>(tree_view<float>(c), c->stream);
+}
+
+// RadSolver's COARSERAD push of the coarse values to the leaves (rad_utils.c:2009-2013)
+static hipError_t launch_distrib(emma_rt_ctx* c) {
+  if (c->rbytes == 8) return launch_distrib_e<double>(tree_view<double>(c), c->stream);
+  return launch_distrib_e<float>(tree_view<float>(c), c->stream);
+}
+
+extern "C" int emma_rt_set_mode(emma_rt_ctx* c, int mode) {
+  if (c == nullptr) return EMMA_RT_ERR_ARG;
+  if (mode != EMMA_RT_RADSTEP && mode != EMMA_RT_COARSERAD) return EMMA_RT_ERR_ARG;
+  c->mode = mode;
+  drop_graphs(c);
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_fillrad(emma_rt_ctx* c, int level, double aexp) {
+  (void)aexp;
+  DevLevel* L = level_of(c, level);
+  if (L == nullptr) return EMMA_RT_ERR_STATE;
+  if (c->mode != EMMA_RT_COARSERAD || level != c->lmin || c->lev.size() < 2) return EMMA_RT_OK;
+  HIPCHK(launch_put(c));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return EMMA_RT_OK;
+}
+
 extern "C" int emma_rt_radsolver(emma_rt_ctx* c, int level, double dtnew, double aexp, int chemonly) {
   DevLevel* L = level_of(c, level);
   if (L == nullptr) return EMMA_RT_ERR_STATE;
@@ -636,12 +699,30 @@ extern "C" int emma_rt_radsolver(emma_rt_ctx* c, int level, double dtnew, double
   if (st) return st;
   st = timed_launch(c, level - c->lmin, dtnew, aexp, chemonly);
   if (st) return st;
+  if (c->mode == EMMA_RT_COARSERAD && level == c->lmin && c->lev.size() > 1) HIPCHK(launch_distrib(c));
   return end_call(c);
+}
+
+// COARSERAD recursion of Advance_level_RAD (advanceamr.c:1309-1537): every
+// level takes one step of the coarse level's dt, finer levels first; the
+// coarse level gathers its leaves before its step and pushes back after it.
+static int recurse_coarserad(emma_rt_ctx* c, int k, double aexp) {
+  const int nl = (int)c->lev.size();
+  if (k + 1 < nl && c->lev[k + 1].noct > 0) {
+    int st = recurse_coarserad(c, k + 1, aexp);
+    if (st) return st;
+  }
+  if (k == 0 && nl > 1) HIPCHK(launch_put(c));
+  int st = timed_launch(c, k, dtrad_of(c->p, c->lmin, aexp), aexp, 0);
+  if (st) return st;
+  if (k == 0 && nl > 1) HIPCHK(launch_distrib(c));
+  return EMMA_RT_OK;
 }
 
 // RADSTEP recursion of Advance_level (advanceamr.c:243-1300): per step of
 // level l, the finer level takes two steps first, then level l steps.
 static int recurse(emma_rt_ctx* c, int k, double aexp) {
+  if (c->mode == EMMA_RT_COARSERAD) return recurse_coarserad(c, k, aexp);
   const int nl = (int)c->lev.size();
   const DevLevel& L = c->lev[k];
   const double dt = dtrad_of(c->p, L.level, aexp);
@@ -673,6 +754,7 @@ extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
   key.ncoarse = ncoarse;
   key.aexp = aexp;
   key.xion = c->xion;
+  key.mode = c->mode;
   for (auto& L : c->lev) key.parity.push_back(L.cur);
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
@@ -698,7 +780,7 @@ extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
   for (size_t k = 0; k < c->lev.size(); k++) c->lev[k].cur = it->second.second[k];
   long long steps = 0;
   for (int k = 0; k < (int)c->lev.size(); k++)
-    if (c->lev[k].noct > 0) steps += (long long)ncoarse << k;
+    if (c->lev[k].noct > 0) steps += (c->mode == EMMA_RT_COARSERAD) ? (long long)ncoarse : ((long long)ncoarse << k);
   c->stats.level_steps = steps;
   return end_call(c);
 }

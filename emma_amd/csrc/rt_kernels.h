@@ -13,6 +13,11 @@ template <typename R>
 struct LevelArgs {
   int ncell, ncoarse, nfine, ng;
   int level, chemonly, nblocks, pad;
+  // RADSTEP: transport=1, keep_split_flux=0, update_split=0.  COARSERAD
+  // (rad_utils.c:518-526,1548-1552,1808-1812): transport only on the coarse
+  // level and not for chemonly calls, fluxes of split neighbours kept, split
+  // cells transport-updated (their chemistry still skipped).
+  int transport, keep_split_flux, update_split, pad2;
   // current state (read)
   const R* e;
   const R* fx;
@@ -20,6 +25,7 @@ struct LevelArgs {
   const R* fz;
   const double* nhplus;
   const double* eint;
+  const double* temp;
   // fields that are not double-buffered (written only for split cells)
   R* src;
   double* nh;
@@ -72,6 +78,28 @@ constexpr int kCounterStride = 16;  // unsigned long longs per slot (128 B)
 
 template <typename R>
 hipError_t launch_level_step(const LevelArgs<R>& a, const ChemConst& K, hipStream_t s);
+
+// Levels of a tree for the COARSERAD transfers between the coarse level and
+// the leaves (putE2coarse / distribE): per level (index level - lcoarse) the
+// child table and the current photon density / flux.
+constexpr int kMaxLevels = 16;
+template <typename R>
+struct TreeView {
+  int nlev, ng, lcoarse, pad;
+  int ncell[kMaxLevels];
+  const int* child[kMaxLevels];
+  R* e[kMaxLevels];
+  R* fx[kMaxLevels];
+  R* fy[kMaxLevels];
+  R* fz[kMaxLevels];
+};
+
+// putE2coarse (src_utils.c:127-150) on every split cell of the coarse level
+template <typename R>
+hipError_t launch_put_e2coarse(const TreeView<R>& t, hipStream_t s);
+// distribE (rad_utils.c:26-45) from every split cell of the coarse level
+template <typename R>
+hipError_t launch_distrib_e(const TreeView<R>& t, hipStream_t s);
 
 template <typename R>
 hipError_t launch_photon_sum(const R* e, const int* child, int ncell, int ng, double vol, double* out,

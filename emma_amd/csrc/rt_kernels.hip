@@ -46,9 +46,192 @@ __device__ __forceinline__ T sel(int i, T a, T b, T c) {
   return i == 0 ? a : (i == 1 ? b : c);
 }
 
+// ------------------------------------------------------------------ pieces
+// reionisation-time bookkeeping of RadSolver (rad_utils.c:1901-1913)
+template <typename R>
+__device__ __forceinline__ void xion_times(const LevelArgs<R>& a, int cell) {
+  const double xion = a.nhplus[cell] / a.nh[cell];
+  if (xion < 0.5) a.t_last[cell] = a.tphy;
+  if ((xion >= 0.5) && (a.t_first[cell] == -1)) a.t_first[cell] = a.tphy;
+}
+
+// split cell: its next state is the mean of its 8 children (rad_utils.c:1955-2006)
+template <typename R>
+__device__ __forceinline__ void restrict_cell(const LevelArgs<R>& a, int cell, int ch, int ng) {
+  const int n = a.ncell, nf = a.nfine, j0 = ch * 8;
+  for (int g = 0; g < ng; g++) {
+    R se = R(0), sx = R(0), sy = R(0), sz = R(0), ss = R(0);
+    for (int k = 0; k < 8; k++) {
+      const int j = g * nf + j0 + k;
+      se += a.fe[j] * R(0.125);
+      sx += a.ffx[j] * R(0.125);
+      sy += a.ffy[j] * R(0.125);
+      sz += a.ffz[j] * R(0.125);
+      ss += a.fsrc[j] * R(0.125);
+    }
+    a.eo[g * n + cell] = se;
+    a.fxo[g * n + cell] = sx;
+    a.fyo[g * n + cell] = sy;
+    a.fzo[g * n + cell] = sz;
+    a.src[g * n + cell] = ss;
+  }
+  double snp = 0., sei = 0., stp = 0., snh = 0., stf = 0., stl = 0.;
+  for (int k = 0; k < 8; k++) {
+    const int j = j0 + k;
+    if (a.t_first != nullptr) {
+      stf += a.ft_first[j] * 0.125;
+      stl += a.ft_last[j] * 0.125;
+    }
+    snp += a.fnhplus[j] * 0.125;
+    sei += a.feint[j] * 0.125;
+    stp += a.ftemp[j] * 0.125;
+    snh += a.fnh[j] * 0.125;
+  }
+  a.nhpluso[cell] = snp;
+  a.einto[cell] = sei;
+  a.tempo[cell] = stp;
+  a.nh[cell] = snh;
+  if (a.t_first != nullptr) {
+    a.t_first[cell] = stf;
+    a.t_last[cell] = stl;
+  }
+}
+
+// reflux of one face flux into the coarse leaf j (scatterstencilrad,
+// rad_utils.c:1738-1762): the coarse cell gains -/+ F dt/dx / 8
+template <typename R, int GA>
+__device__ __forceinline__ void reflux(const LevelArgs<R>& a, int j, int f, int g, const R F[4], R dts) {
+  const int cch = a.cchild[j];
+  if (cch <= -2) {
+    R* acc = a.cracc + (size_t)(-2 - cch) * 4 * GA;
+    const R w = (f & 1) ? R(1) : R(-1);  // pow(-1, face+1)
+    atomicAdd(acc + 0 * GA + g, F[0] * dts * w * R(0.125));
+    atomicAdd(acc + 1 * GA + g, F[1] * dts * w * R(0.125));
+    atomicAdd(acc + 2 * GA + g, F[2] * dts * w * R(0.125));
+    atomicAdd(acc + 3 * GA + g, F[3] * dts * w * R(0.125));
+  }
+}
+
+// leaf cell after its fluxes: base state (+ refluxes from the finer level),
+// update, N<0/NaN guard, chemistry, write of the next state.  Returns the
+// accepted chemistry sub-steps.
+template <typename R, int G, bool Fast>
+__device__ __forceinline__ int finish_leaf(const LevelArgs<R>& a, const ChemConst& K, int cell, int ch, int ng,
+                                           R* Ue, R* Ux, R* Uy, R* Uz, const R* De, const R* Dx, const R* Dy,
+                                           const R* Dz) {
+  constexpr int GA = G > 0 ? G : kMaxGrp;
+  const int n = a.ncell;
+  if (ch <= -2) {
+    R* acc = a.racc + (size_t)(-2 - ch) * 4 * GA;
+#pragma unroll
+    for (int g = 0; g < GA; g++) {
+      if (g < ng) {
+        Ue[g] += acc[0 * GA + g];
+        Ux[g] += acc[1 * GA + g];
+        Uy[g] += acc[2 * GA + g];
+        Uz[g] += acc[3 * GA + g];
+        acc[0 * GA + g] = R(0);
+        acc[1 * GA + g] = R(0);
+        acc[2 * GA + g] = R(0);
+        acc[3 * GA + g] = R(0);
+      }
+    }
+  }
+  double e[GA], fx[GA], fy[GA], fz[GA], src[GA];
+#pragma unroll
+  for (int g = 0; g < GA; g++) {
+    if (g < ng) {
+      e[g] = (double)(R)(Ue[g] + De[g]);
+      fx[g] = (double)(R)(Ux[g] + Dx[g]);
+      fy[g] = (double)(R)(Uy[g] + Dy[g]);
+      fz[g] = (double)(R)(Uz[g] + Dz[g]);
+      src[g] = (double)a.src[g * n + cell];
+    }
+  }
+  // rad_utils.c:1611-1650 (the second test reads e[1], i.e. fx[0] when G=1)
+  if (e[0] < 0 || isnan(e[0]) || isnan(ng > 1 ? e[1 < GA ? 1 : 0] : fx[0])) raise_status(a.status, -3, a.level, cell);
+  double nhplus = a.nhplus[cell];
+  double eint = a.eint[cell];
+  const double nh = a.nh[cell];
+  double temp = 0.;
+  int nit = 0;
+  int st;
+  if constexpr (Fast)
+    st = chem_cell_fast<G>(K, ng, e, fx, fy, fz, src, nhplus, eint, nh, temp, a.chemonly, nit);
+  else
+    st = chem_cell<G>(K, ng, e, fx, fy, fz, src, nhplus, eint, nh, temp, a.chemonly, nit);
+  if (st != 0) raise_status(a.status, st, a.level, cell);
+#pragma unroll
+  for (int g = 0; g < GA; g++) {
+    if (g < ng) {
+      a.eo[g * n + cell] = (R)e[g];
+      a.fxo[g * n + cell] = (R)fx[g];
+      a.fyo[g * n + cell] = (R)fy[g];
+      a.fzo[g * n + cell] = (R)fz[g];
+    }
+  }
+  a.nhpluso[cell] = nhplus;
+  a.einto[cell] = eint;
+  a.tempo[cell] = temp;
+  return nit;
+}
+
+// COARSERAD coarse level: a split cell takes the transport update of its own
+// state (updatefieldrad without the split test, rad_utils.c:1548-1552) and
+// keeps x, eint, T (chemrad skips it)
+template <typename R, int GA>
+__device__ __forceinline__ void transport_split_cell(const LevelArgs<R>& a, int cell, int ng, const R* Ue, const R* Ux,
+                                                     const R* Uy, const R* Uz, const R* De, const R* Dx, const R* Dy,
+                                                     const R* Dz) {
+  const int n = a.ncell;
+#pragma unroll
+  for (int g = 0; g < GA; g++) {
+    if (g < ng) {
+      a.eo[g * n + cell] = Ue[g] + De[g];
+      a.fxo[g * n + cell] = Ux[g] + Dx[g];
+      a.fyo[g * n + cell] = Uy[g] + Dy[g];
+      a.fzo[g * n + cell] = Uz[g] + Dz[g];
+    }
+  }
+  const double e0 = (double)(R)(Ue[0] + De[0]);
+  const double e1 = ng > 1 ? (double)(R)(Ue[1 < GA ? 1 : 0] + De[1 < GA ? 1 : 0]) : (double)(R)(Ux[0] + Dx[0]);
+  if (e0 < 0 || isnan(e0) || isnan(e1)) raise_status(a.status, -3, a.level, cell);
+  a.nhpluso[cell] = a.nhplus[cell];
+  a.einto[cell] = a.eint[cell];
+  a.tempo[cell] = a.temp[cell];
+}
+
+// per-block counters: accepted chemistry sub-steps and leaf cell-updates,
+// one atomic pair per block into a spread slot
+template <int NW>
+__device__ __forceinline__ void block_counters(unsigned long long* counters, unsigned int v0, unsigned int v1) {
+  __shared__ unsigned int red[2][NW];
+  for (int o = 32; o > 0; o >>= 1) {
+    v0 += __shfl_xor(v0, o, 64);
+    v1 += __shfl_xor(v1, o, 64);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = v0;
+    red[1][w] = v1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long s0 = 0, s1 = 0;
+    for (int i = 0; i < NW; i++) {
+      s0 += red[0][i];
+      s1 += red[1][i];
+    }
+    unsigned long long* slot = counters + (size_t)(blockIdx.x % kCounterSlots) * kCounterStride;
+    atomicAdd(slot + 0, s0);
+    atomicAdd(slot + 1, s1);
+  }
+}
+
 // ------------------------------------------------------------------ level step
-// Fast selects the mixed-precision physics (rt_physics_fast.h); the FP64
-// kernel (Fast = false) keeps the reference's operation order.
+// One thread per cell; neighbours straight from global memory.  Fast selects
+// the mixed-precision physics (rt_physics_fast.h); the FP64 kernel
+// (Fast = false) keeps the reference's operation order.
 template <typename R, int G, bool Fast>
 __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a, ChemConst K) {
   constexpr int GA = G > 0 ? G : kMaxGrp;
@@ -60,57 +243,11 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a, ChemCon
     const int n = a.ncell;
     const int oct = cell >> 3, ic = cell & 7;
     const int ch = a.child[cell];
-
-    if (a.t_first != nullptr) {  // rad_utils.c:1901-1913
-      const double xion = a.nhplus[cell] / a.nh[cell];
-      if (xion < 0.5) a.t_last[cell] = a.tphy;
-      if ((xion >= 0.5) && (a.t_first[cell] == -1)) a.t_first[cell] = a.tphy;
-    }
-
-    if (ch >= 0) {
-      // split cell: mean of the 8 children (rad_utils.c:1955-2006)
-      const int nf = a.nfine;
-      const int j0 = ch * 8;
-      for (int g = 0; g < ng; g++) {
-        R se = R(0), sx = R(0), sy = R(0), sz = R(0), ss = R(0);
-        for (int k = 0; k < 8; k++) {
-          const int j = g * nf + j0 + k;
-          se += a.fe[j] * R(0.125);
-          sx += a.ffx[j] * R(0.125);
-          sy += a.ffy[j] * R(0.125);
-          sz += a.ffz[j] * R(0.125);
-          ss += a.fsrc[j] * R(0.125);
-        }
-        a.eo[g * n + cell] = se;
-        a.fxo[g * n + cell] = sx;
-        a.fyo[g * n + cell] = sy;
-        a.fzo[g * n + cell] = sz;
-        a.src[g * n + cell] = ss;
-      }
-      double snp = 0., sei = 0., stp = 0., snh = 0., stf = 0., stl = 0.;
-      for (int k = 0; k < 8; k++) {
-        const int j = j0 + k;
-        if (a.t_first != nullptr) {
-          stf += a.ft_first[j] * 0.125;
-          stl += a.ft_last[j] * 0.125;
-        }
-        snp += a.fnhplus[j] * 0.125;
-        sei += a.feint[j] * 0.125;
-        stp += a.ftemp[j] * 0.125;
-        snh += a.fnh[j] * 0.125;
-      }
-      a.nhpluso[cell] = snp;
-      a.einto[cell] = sei;
-      a.tempo[cell] = stp;
-      a.nh[cell] = snh;
-      if (a.t_first != nullptr) {
-        a.t_first[cell] = stf;
-        a.t_last[cell] = stl;
-      }
+    if (a.t_first != nullptr) xion_times(a, cell);
+    if (ch >= 0 && !a.update_split) {
+      restrict_cell(a, cell, ch, ng);
     } else {
-      leaf = 1;
       const R c = a.c, hc = a.hc, dts = a.dtsurdx;
-      // own state and its closure, per group
       R Ue[GA], Ux[GA], Uy[GA], Uz[GA];
       R De[GA], Dx[GA], Dy[GA], Dz[GA];
       Closure<R> Q[GA];
@@ -125,17 +262,13 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a, ChemCon
             Q[g] = closure_fast(Ux[g], Uy[g], Uz[g], Ue[g], c);
           else
             Q[g] = closure<R>(Ux[g], Uy[g], Uz[g], Ue[g], c);
-          De[g] = R(0);
-          Dx[g] = R(0);
-          Dy[g] = R(0);
-          Dz[g] = R(0);
+          De[g] = Dx[g] = Dy[g] = Dz[g] = R(0);
         }
       }
       // six faces in the reference order (rad_utils.c:1566-1574): f = 2d+s
 #pragma unroll
-      for (int f = 0; f < 6; f++) {
+      for (int f = 0; f < (a.transport ? 6 : 0); f++) {
         const int d = f >> 1, s = f & 1;
-        const int bit = (ic >> d) & 1;
         const int nc = ic ^ (1 << d);
         const R* Ne = a.e;
         const R* Nx = a.fx;
@@ -143,7 +276,7 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a, ChemCon
         const R* Nz = a.fz;
         int nn = n, j, split;
         int coarse_face = 0;
-        if (bit != s) {  // neighbour inside the oct
+        if (((ic >> d) & 1) != s) {  // neighbour inside the oct
           j = oct * 8 + nc;
           split = a.child[j] >= 0;
         } else {
@@ -162,7 +295,7 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a, ChemCon
             coarse_face = 1;
           }
         }
-        const R ffact = split ? R(0) : R(1);
+        const R ffact = (split && !a.keep_split_flux) ? R(0) : R(1);
         const R one = (f & 1) ? R(-1) : R(1);
 #pragma unroll
         for (int g = 0; g < GA; g++) {
@@ -182,101 +315,236 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a, ChemCon
             Dx[g] += F[1] * dts * one;
             Dy[g] += F[2] * dts * one;
             Dz[g] += F[3] * dts * one;
-            if (coarse_face) {  // scatterstencilrad refluxing (rad_utils.c:1738-1762)
-              const int cch = a.cchild[j];
-              if (cch <= -2) {
-                R* acc = a.cracc + (size_t)(-2 - cch) * 4 * GA;
-                const R w = -one;  // pow(-1, face+1)
-                atomicAdd(acc + 0 * GA + g, F[0] * dts * w * R(0.125));
-                atomicAdd(acc + 1 * GA + g, F[1] * dts * w * R(0.125));
-                atomicAdd(acc + 2 * GA + g, F[2] * dts * w * R(0.125));
-                atomicAdd(acc + 3 * GA + g, F[3] * dts * w * R(0.125));
-              }
-            }
+            if (coarse_face) reflux<R, GA>(a, j, f, g, F, dts);
           }
         }
       }
-      // base state: rfieldnew = rfield (+ refluxes from the finer level)
-      if (ch <= -2) {
-        R* acc = a.racc + (size_t)(-2 - ch) * 4 * GA;
-#pragma unroll
-        for (int g = 0; g < GA; g++) {
-          if (g < ng) {
-            Ue[g] += acc[0 * GA + g];
-            Ux[g] += acc[1 * GA + g];
-            Uy[g] += acc[2 * GA + g];
-            Uz[g] += acc[3 * GA + g];
-            acc[0 * GA + g] = R(0);
-            acc[1 * GA + g] = R(0);
-            acc[2 * GA + g] = R(0);
-            acc[3 * GA + g] = R(0);
-          }
-        }
+      if (ch >= 0) {
+        transport_split_cell<R, GA>(a, cell, ng, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz);
+      } else {
+        leaf = 1;
+        nit_acc = (unsigned)finish_leaf<R, G, Fast>(a, K, cell, ch, ng, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz);
       }
-      double e[GA], fx[GA], fy[GA], fz[GA], src[GA];
-#pragma unroll
-      for (int g = 0; g < GA; g++) {
-        if (g < ng) {
-          const R ue = Ue[g] + De[g];
-          const R ux = Ux[g] + Dx[g];
-          e[g] = (double)ue;
-          fx[g] = (double)ux;
-          fy[g] = (double)(Uy[g] + Dy[g]);
-          fz[g] = (double)(Uz[g] + Dz[g]);
-          src[g] = (double)a.src[g * n + cell];
-        }
-      }
-      // rad_utils.c:1611-1650 (the second test reads e[1], i.e. fx[0] when G=1)
-      if (e[0] < 0 || isnan(e[0]) || isnan(ng > 1 ? e[1 < GA ? 1 : 0] : fx[0])) {
-        raise_status(a.status, -3, a.level, cell);
-      }
-      double nhplus = a.nhplus[cell];
-      double eint = a.eint[cell];
-      const double nh = a.nh[cell];
-      double temp = 0.;
-      int nit = 0;
-      int st;
-      if constexpr (Fast)
-        st = chem_cell_fast<G>(K, ng, e, fx, fy, fz, src, nhplus, eint, nh, temp, a.chemonly, nit);
-      else
-        st = chem_cell<G>(K, ng, e, fx, fy, fz, src, nhplus, eint, nh, temp, a.chemonly, nit);
-      if (st != 0) raise_status(a.status, st, a.level, cell);
-      nit_acc = (unsigned)nit;
-#pragma unroll
-      for (int g = 0; g < GA; g++) {
-        if (g < ng) {
-          a.eo[g * n + cell] = (R)e[g];
-          a.fxo[g * n + cell] = (R)fx[g];
-          a.fyo[g * n + cell] = (R)fy[g];
-          a.fzo[g * n + cell] = (R)fz[g];
-        }
-      }
-      a.nhpluso[cell] = nhplus;
-      a.einto[cell] = eint;
-      a.tempo[cell] = temp;
     }
   }
-  // per-block counters: accepted chemistry sub-steps and leaf cell-updates
-  if (a.counters != nullptr) {
-    __shared__ unsigned int red[2][4];
-    unsigned int v0 = nit_acc, v1 = leaf;
-    for (int o = 32; o > 0; o >>= 1) {
-      v0 += __shfl_xor(v0, o, 64);
-      v1 += __shfl_xor(v1, o, 64);
+  if (a.counters != nullptr) block_counters<4>(a.counters, nit_acc, leaf);
+}
+
+// ------------------------------------------------------------------ tiled level step
+// Mixed precision, G fixed.  A 512-thread block owns 64 consecutive octs
+// (Morton order: a 4x4x4 cube of octs on a complete level).  Every thread
+// puts its cell's state and closure in LDS; face neighbours inside the tile
+// come from LDS, so each closure is computed once instead of once per
+// neighbouring face; neighbours outside the tile (and injected coarse
+// leaves) are read from global memory as in level_step_kernel.
+constexpr int kTileCells = 512;
+constexpr int kTileOcts = kTileCells / 8;
+
+template <int G>
+__global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4))) void level_step_tiled(LevelArgs<float> a, ChemConst K) {
+  __shared__ float sE[G][kTileCells], sX[G][kTileCells], sY[G][kTileCells], sZ[G][kTileCells];
+  __shared__ float sDiag[G][kTileCells], sArg[G][kTileCells];
+  __shared__ float sN0[G][kTileCells], sN1[G][kTileCells], sN2[G][kTileCells];
+  __shared__ unsigned char sSplit[kTileCells];
+  const int tile = xcd_block(a.nblocks);
+  const int lc = (int)threadIdx.x;
+  const int cell = tile * kTileCells + lc;
+  const int oct0 = tile * kTileOcts;
+  const bool valid = cell < a.ncell;
+  const int n = a.ncell;
+  const float c = a.c, hc = a.hc, dts = a.dtsurdx;
+  float Ue[G], Ux[G], Uy[G], Uz[G];
+  Closure<float> Q[G];
+  int ch = -1;
+  if (valid) {
+    ch = a.child[cell];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      Ue[g] = a.e[g * n + cell];
+      Ux[g] = a.fx[g * n + cell];
+      Uy[g] = a.fy[g * n + cell];
+      Uz[g] = a.fz[g * n + cell];
+      Q[g] = closure_fast(Ux[g], Uy[g], Uz[g], Ue[g], c);
+      sE[g][lc] = Ue[g];
+      sX[g][lc] = Ux[g];
+      sY[g][lc] = Uy[g];
+      sZ[g][lc] = Uz[g];
+      sDiag[g][lc] = Q[g].diag;
+      sArg[g][lc] = Q[g].arg;
+      sN0[g][lc] = Q[g].n[0];
+      sN1[g][lc] = Q[g].n[1];
+      sN2[g][lc] = Q[g].n[2];
     }
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-      red[0][w] = v0;
-      red[1][w] = v1;
+  }
+  sSplit[lc] = (unsigned char)(ch >= 0);
+  __syncthreads();
+  unsigned int nit_acc = 0, leaf = 0;
+  if (valid) {
+    const int oct = cell >> 3, ic = cell & 7;
+    if (a.t_first != nullptr) xion_times(a, cell);
+    if (ch >= 0 && !a.update_split) {
+      restrict_cell(a, cell, ch, G);
+    } else {
+      float De[G], Dx[G], Dy[G], Dz[G];
+#pragma unroll
+      for (int g = 0; g < G; g++) De[g] = Dx[g] = Dy[g] = Dz[g] = 0.f;
+#pragma unroll
+      for (int f = 0; f < (a.transport ? 6 : 0); f++) {
+        const int d = f >> 1, s = f & 1;
+        const int nc = ic ^ (1 << d);
+        int jl = -1, j = 0, coarse_face = 0;
+        if (((ic >> d) & 1) != s) {
+          jl = lc ^ (1 << d);  // same oct
+        } else {
+          const int no = a.nei[oct * 6 + f];
+          if (no >= oct0 && no < oct0 + kTileOcts) {
+            jl = (no - oct0) * 8 + nc;
+          } else if (no >= 0) {
+            j = no * 8 + nc;
+          } else {
+            j = -1 - no;
+            coarse_face = 1;
+          }
+        }
+        const float one = (f & 1) ? -1.f : 1.f;
+        if (jl >= 0) {
+          const float ffact = (sSplit[jl] && !a.keep_split_flux) ? 0.f : 1.f;
+#pragma unroll
+          for (int g = 0; g < G; g++) {
+            Closure<float> qn;
+            qn.diag = sDiag[g][jl];
+            qn.arg = sArg[g][jl];
+            qn.n[0] = sN0[g][jl];
+            qn.n[1] = sN1[g][jl];
+            qn.n[2] = sN2[g][jl];
+            float F[4];
+            lf_flux<float>(d, s, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], sE[g][jl], sX[g][jl], sY[g][jl], sZ[g][jl], qn,
+                           hc, F);
+            De[g] += F[0] * ffact * dts * one;
+            Dx[g] += F[1] * ffact * dts * one;
+            Dy[g] += F[2] * ffact * dts * one;
+            Dz[g] += F[3] * ffact * dts * one;
+          }
+        } else {
+          const float* Ne = coarse_face ? a.ce : a.e;
+          const float* Nx = coarse_face ? a.cfx : a.fx;
+          const float* Ny = coarse_face ? a.cfy : a.fy;
+          const float* Nz = coarse_face ? a.cfz : a.fz;
+          const int nn = coarse_face ? a.ncoarse : n;
+          const float ffact = (!coarse_face && !a.keep_split_flux && a.child[j] >= 0) ? 0.f : 1.f;
+#pragma unroll
+          for (int g = 0; g < G; g++) {
+            const float ve = Ne[g * nn + j], vx = Nx[g * nn + j], vy = Ny[g * nn + j], vz = Nz[g * nn + j];
+            const Closure<float> qn = closure_fast(vx, vy, vz, ve, c);
+            float F[4];
+            lf_flux<float>(d, s, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], ve, vx, vy, vz, qn, hc, F);
+#pragma unroll
+            for (int v = 0; v < 4; v++) F[v] *= ffact;
+            De[g] += F[0] * dts * one;
+            Dx[g] += F[1] * dts * one;
+            Dy[g] += F[2] * dts * one;
+            Dz[g] += F[3] * dts * one;
+            if (coarse_face) reflux<float, G>(a, j, f, g, F, dts);
+          }
+        }
+      }
+      if (ch >= 0) {
+        transport_split_cell<float, G>(a, cell, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz);
+      } else {
+        leaf = 1;
+        nit_acc = (unsigned)finish_leaf<float, G, true>(a, K, cell, ch, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz);
+      }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned long long* slot = a.counters + (size_t)(blockIdx.x % kCounterSlots) * kCounterStride;
-      atomicAdd(slot + 0, (unsigned long long)(red[0][0] + red[0][1] + red[0][2] + red[0][3]));
-      atomicAdd(slot + 1, (unsigned long long)(red[1][0] + red[1][1] + red[1][2] + red[1][3]));
+  }
+  if (a.counters != nullptr) block_counters<kTileCells / 64>(a.counters, nit_acc, leaf);
+}
+
+// ------------------------------------------------------------------ COARSERAD transfers
+// One thread per cell of the coarse level; a split cell walks its subtree
+// depth first, children in index order like the reference's recursion
+// (collectE, src_utils.c:94-123; distribE, rad_utils.c:26-45).
+template <typename R, bool Put>
+__global__ __launch_bounds__(256) void coarse_transfer_kernel(TreeView<R> t) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= t.ncell[0]) return;
+  const int ch0 = t.child[0][i];
+  if (ch0 < 0 || t.nlev < 2) return;
+  const int n0 = t.ncell[0];
+  R acc[4][kMaxGrp];
+  if (Put) {
+    for (int g = 0; g < t.ng; g++) acc[0][g] = acc[1][g] = acc[2][g] = acc[3][g] = R(0);
+  } else {
+    for (int g = 0; g < t.ng; g++) {
+      acc[0][g] = t.e[0][g * n0 + i];
+      acc[1][g] = t.fx[0][g * n0 + i];
+      acc[2][g] = t.fy[0][g * n0 + i];
+      acc[3][g] = t.fz[0][g * n0 + i];
+    }
+  }
+  // explicit stack: octs of levels 1..nlev-1 and the next child to visit
+  int oct[kMaxLevels], next[kMaxLevels];
+  int k = 1;
+  oct[1] = ch0;
+  next[1] = 0;
+  while (k >= 1) {
+    if (next[k] == 8) {
+      k--;
+      continue;
+    }
+    const int cell = oct[k] * 8 + next[k];
+    next[k]++;
+    const int ch = t.child[k][cell];
+    if (ch >= 0 && k + 1 < t.nlev) {
+      k++;
+      oct[k] = ch;
+      next[k] = 0;
+      continue;
+    }
+    if (ch >= 0) continue;  // split cell on the last level (no children stored)
+    const int n = t.ncell[k];
+    if (Put) {
+      const R fact = (R)ldexp(1.0, -3 * k);  // pow(2, 3 (lcoarse - level))
+      for (int g = 0; g < t.ng; g++) {
+        acc[0][g] += t.e[k][g * n + cell] * fact;
+        acc[1][g] += t.fx[k][g * n + cell] * fact;
+        acc[2][g] += t.fy[k][g * n + cell] * fact;
+        acc[3][g] += t.fz[k][g * n + cell] * fact;
+      }
+    } else {
+      for (int g = 0; g < t.ng; g++) {
+        t.e[k][g * n + cell] = acc[0][g];
+        t.fx[k][g * n + cell] = acc[1][g];
+        t.fy[k][g * n + cell] = acc[2][g];
+        t.fz[k][g * n + cell] = acc[3][g];
+      }
+    }
+  }
+  if (Put) {
+    for (int g = 0; g < t.ng; g++) {
+      t.e[0][g * n0 + i] = acc[0][g];
+      t.fx[0][g * n0 + i] = acc[1][g];
+      t.fy[0][g * n0 + i] = acc[2][g];
+      t.fz[0][g * n0 + i] = acc[3][g];
     }
   }
 }
+
+template <typename R>
+hipError_t launch_put_e2coarse(const TreeView<R>& t, hipStream_t s) {
+  hipLaunchKernelGGL((coarse_transfer_kernel<R, true>), dim3((t.ncell[0] + 255) / 256), dim3(256), 0, s, t);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_distrib_e(const TreeView<R>& t, hipStream_t s) {
+  hipLaunchKernelGGL((coarse_transfer_kernel<R, false>), dim3((t.ncell[0] + 255) / 256), dim3(256), 0, s, t);
+  return hipGetLastError();
+}
+
+template hipError_t launch_put_e2coarse<double>(const TreeView<double>&, hipStream_t);
+template hipError_t launch_put_e2coarse<float>(const TreeView<float>&, hipStream_t);
+template hipError_t launch_distrib_e<double>(const TreeView<double>&, hipStream_t);
+template hipError_t launch_distrib_e<float>(const TreeView<float>&, hipStream_t);
 
 // ------------------------------------------------------------------ photon budget
 // FP64 sum over leaf cells of sum_g N_g * dx^3 (wavefront shuffles, one
@@ -308,6 +576,18 @@ __global__ void fill_kernel(unsigned int* p, int n, unsigned int v) {
 template <typename R>
 hipError_t launch_level_step(const LevelArgs<R>& a0, const ChemConst& K, hipStream_t s) {
   LevelArgs<R> a = a0;
+  if constexpr (sizeof(R) == 4) {
+    if (a.ng == 1 || a.ng == 3) {
+      int nb = (a.ncell + kTileCells - 1) / kTileCells;
+      nb = (nb + 7) & ~7;
+      a.nblocks = nb;
+      if (a.ng == 1)
+        hipLaunchKernelGGL((level_step_tiled<1>), dim3(nb), dim3(kTileCells), 0, s, a, K);
+      else
+        hipLaunchKernelGGL((level_step_tiled<3>), dim3(nb), dim3(kTileCells), 0, s, a, K);
+      return hipGetLastError();
+    }
+  }
   int nb = (a.ncell + 255) / 256;
   nb = (nb + 7) & ~7;
   a.nblocks = nb;

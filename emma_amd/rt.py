@@ -44,6 +44,9 @@ ERR_STATE = -9
 FP64 = 0
 MIXED = 1
 
+RADSTEP = 0
+COARSERAD = 1
+
 EXPORTS = [
     "emma_rt_countdevices", "emma_rt_initlocaldevice", "emma_rt_checkdevice", "emma_rt_strerror",
     "emma_rt_create", "emma_rt_set_params", "emma_rt_destroy",
@@ -51,7 +54,7 @@ EXPORTS = [
     "emma_rt_advance_stencil",
     "emma_rt_tree_create", "emma_rt_set_level", "emma_rt_get_level", "emma_rt_enable_xion_times",
     "emma_rt_get_xion_times", "emma_rt_set_tphy", "emma_rt_radsolver", "emma_rt_coarse_step",
-    "emma_rt_get_stats", "emma_rt_enable_timing",
+    "emma_rt_get_stats", "emma_rt_enable_timing", "emma_rt_set_mode", "emma_rt_fillrad",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -116,6 +119,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.emma_rt_coarse_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
     L.emma_rt_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
     L.emma_rt_enable_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.emma_rt_set_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.emma_rt_fillrad.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
     _lib = L
     return L
 
@@ -188,9 +193,10 @@ class RadGPU:
                                                  self.precision))
         self.tree = tree
 
-    def setup(self, params: RunParams, tree: T.Tree, fields: dict, tphy: float = 0.0):
+    def setup(self, params: RunParams, tree: T.Tree, fields: dict, tphy: float = 0.0, mode: int = RADSTEP):
         if params is not self.params:
             self.set_params(params)
+        self.set_mode(mode)
         self.load_tree(tree)
         for l, F in fields.items():
             self.set(l, 0, F)
@@ -223,7 +229,16 @@ class RadGPU:
     def enable_timing(self, on: bool = True):
         self._check(self.lib.emma_rt_enable_timing(self.h, int(on)))
 
+    def set_mode(self, mode: int):
+        """RADSTEP (per-level transport) or COARSERAD (coarse-level transport)"""
+        self._check(self.lib.emma_rt_set_mode(self.h, int(mode)))
+        self.mode = mode
+
     # ---- the reference's calls
+    def fillrad(self, level: int, aexp: float = 1.0):
+        """FillRad's COARSERAD gather of the leaves' E, F into the coarse level"""
+        self._check(self.lib.emma_rt_fillrad(self.h, level, aexp))
+
     def clean_new(self, level: int):
         """clean_new_rad: no device work (double-buffered state)."""
         return None

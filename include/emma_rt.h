@@ -52,6 +52,11 @@ extern "C" {
 #define EMMA_RT_ERR_NODEVICE (-8)   /* no GPU visible */
 #define EMMA_RT_ERR_STATE (-9)      /* call out of order (no tree / no stencil buffer) */
 
+/* level-stepping mode of the resident path (compile-time COARSERAD flag of
+   the reference, prototypes.h:81-83) */
+#define EMMA_RT_RADSTEP 0   /* every level transports with its own dt (advanceamr.c:243) */
+#define EMMA_RT_COARSERAD 1 /* transport on the coarse level only (advanceamr.c:1309) */
+
 /* precision of the resident path */
 #define EMMA_RT_FP64 0  /* everything FP64 */
 #define EMMA_RT_MIXED 1 /* photon density, flux and source FP32; x, eint, T, n_H FP64 */
@@ -148,6 +153,19 @@ int emma_rt_radsolver(emma_rt_ctx *ctx, int level, double dtnew, double aexp, in
    level l takes 2^(l-lmin) steps of dt_rad(l) (advanceamr.c:226) per coarse
    step, finer levels first.  Captured once as a HIP graph and replayed. */
 int emma_rt_coarse_step(emma_rt_ctx *ctx, double aexp, int ncoarse);
+
+/* COARSERAD (EMMA_RT_COARSERAD): the RadSolver step of the coarse level
+   transports every cell (split cells included, fluxes of split neighbours
+   kept, no refluxing; rad_utils.c:518-526, 1548-1552), finer levels run the
+   chemistry only, and after the coarse step the coarse E, F are pushed down to
+   the leaves (distribE, rad_utils.c:26-45, 2009-2013).  emma_rt_fillrad is
+   FillRad's COARSERAD block (src_utils.c:717-733): before the coarse step,
+   every split coarse cell gathers the volume-weighted E, F of its leaves
+   (putE2coarse/collectE, src_utils.c:94-150); a no-op in RADSTEP mode and on
+   finer levels.  emma_rt_coarse_step follows Advance_level_RAD
+   (advanceamr.c:1309-1537) in this mode: one step of the coarse dt per level. */
+int emma_rt_set_mode(emma_rt_ctx *ctx, int mode);
+int emma_rt_fillrad(emma_rt_ctx *ctx, int level, double aexp);
 
 int emma_rt_get_stats(emma_rt_ctx *ctx, emma_rt_stats *out);
 /* record HIP events around every level-step kernel (adds the per-launch

@@ -27,6 +27,10 @@ from emma_amd.synth import Fields
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "_build", "libemma_oracle.so")
 REF_SO = {1: os.path.join(HERE, "_ref", "libemma_ref_g1.so"), 3: os.path.join(HERE, "_ref", "libemma_ref_g3.so")}
+REF_SO_CR = {1: os.path.join(HERE, "_ref", "libemma_ref_g1_cr.so"), 3: os.path.join(HERE, "_ref", "libemma_ref_g3_cr.so")}
+
+RADSTEP = 0
+COARSERAD = 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -89,8 +93,19 @@ class Oracle:
         L.orc_dtrad.restype = ctypes.c_double
         L.orc_dtrad.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
         L.orc_getcellnei.argtypes = [ctypes.c_int, _ip, _ip]
+        L.orc_level_advance_mode.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip,
+                                             ctypes.POINTER(_OrcFields), ctypes.POINTER(_OrcFields),
+                                             ctypes.POINTER(_OrcFields), ctypes.POINTER(_OrcFields), ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_int, _ip, _ip]
+        L.orc_level_final_mode.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
+                                           ctypes.POINTER(_OrcFields), ctypes.POINTER(_OrcFields),
+                                           ctypes.POINTER(_OrcFields), ctypes.c_double, _dp, _dp, _dp, _dp]
+        L.orc_putE2coarse.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.POINTER(_OrcFields)]
+        L.orc_distribE.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         self.params = None
         self.tree = None
+        self.mode = RADSTEP
 
     # ---- scalar / vector helpers
     def eddington(self, fx, fy, fz, e, c, i, j):
@@ -154,10 +169,11 @@ class Oracle:
         return out.reshape(L.ncell, 6, G, 5)
 
     # ---- level state interface (mirrors the reference driver)
-    def setup(self, params, tree, fields: dict, tphy: float = 0.0):
+    def setup(self, params, tree, fields: dict, tphy: float = 0.0, mode: int = RADSTEP):
         self.params = params
         self.cparams = params.to_c()
         self.tree = tree
+        self.mode = mode
         self.old = {l: f.copy() for l, f in fields.items()}
         self.new = {l: f.copy() for l, f in fields.items()}
         self.t_first = {l: np.full(f.ncell, -1.0) for l, f in fields.items()}
@@ -188,16 +204,16 @@ class Oracle:
         nei = np.ascontiguousarray(L.nei_oct.reshape(-1), dtype=np.int32)
         ch = np.ascontiguousarray(L.child, dtype=np.int32)
         dx = 0.5**level
-        st = self.lib.orc_level_advance(ctypes.byref(self.cparams), L.noct, _i(nei), _i(ch), ctypes.byref(a),
-                                        ctypes.byref(b), ctypes.byref(ca) if ca else None,
-                                        ctypes.byref(cb) if cb else None, dx, dtnew, aexp, chemonly, _i(nsub),
-                                        _i(bad))
+        st = self.lib.orc_level_advance_mode(ctypes.byref(self.cparams), self.mode, int(level == t.lmin), L.noct,
+                                             _i(nei), _i(ch), ctypes.byref(a), ctypes.byref(b),
+                                             ctypes.byref(ca) if ca else None, ctypes.byref(cb) if cb else None, dx,
+                                             dtnew, aexp, chemonly, _i(nsub), _i(bad))
         self.nsub[level] = nsub
         if st != 0:
             raise OracleError(st, int(bad[0]))
         return L.noct
 
-    def final(self, level):
+    def final(self, level, chemonly=0):
         t = self.tree
         L = t.level(level)
         fo = self.old.get(level + 1)
@@ -207,17 +223,40 @@ class Oracle:
         tf, tl = self.t_first[level], self.t_last[level]
         ftf = self.t_first.get(level + 1)
         ftl = self.t_last.get(level + 1)
-        self.lib.orc_level_final(self.params.ngrp, L.noct, _i(ch), ctypes.byref(a), ctypes.byref(b),
-                                 ctypes.byref(c) if c else None, self.tphy, _d(tf), _d(tl),
-                                 _d(ftf) if ftf is not None else None, _d(ftl) if ftl is not None else None)
+        self.lib.orc_level_final_mode(self.params.ngrp, self.mode, int(level == t.lmin), chemonly, L.noct, _i(ch),
+                                      ctypes.byref(a), ctypes.byref(b), ctypes.byref(c) if c else None, self.tphy,
+                                      _d(tf), _d(tl), _d(ftf) if ftf is not None else None,
+                                      _d(ftl) if ftl is not None else None)
+        if self.mode == COARSERAD and level == t.lmin:
+            self._levels_call(self.lib.orc_distribE)
+
+    def _levels_call(self, fn, *extra):
+        t = self.tree
+        ls = range(t.lmin, t.lmax + 1)
+        childs = [np.ascontiguousarray(t.level(l).child, dtype=np.int32) for l in ls]
+        cf = [_cfields(self.old[l]) for l in ls]
+        arr_c = (ctypes.POINTER(ctypes.c_int) * len(childs))(*[_i(c) for c in childs])
+        arr_f = (ctypes.POINTER(_OrcFields) * len(cf))(*[ctypes.pointer(x) for x in cf])
+        self._keep = (childs, cf, arr_c, arr_f)
+        fn(self.params.ngrp, t.lmin, len(childs), ctypes.cast(arr_c, ctypes.c_void_p),
+           ctypes.cast(arr_f, ctypes.c_void_p), *extra)
+
+    def fillrad(self, level, aexp=1.0):
+        """FillRad's COARSERAD block: split coarse cells gather their leaves' E, F"""
+        if self.mode == COARSERAD and level == self.tree.lmin:
+            nc = _cfields(self.new[level])
+            self._levels_call(self.lib.orc_putE2coarse, ctypes.byref(nc))
 
     def radsolver(self, level, dtnew, aexp=1.0, chemonly=0):
         n = self.advancerad(level, dtnew, aexp, chemonly)
         if n > 0:
-            self.final(level)
+            self.final(level, chemonly)
 
     def coarse_step(self, aexp=1.0):
-        coarse_step(self, self.tree, self.params, aexp)
+        if self.mode == COARSERAD:
+            coarse_step_coarserad(self, self.tree, self.params, aexp)
+        else:
+            coarse_step(self, self.tree, self.params, aexp)
 
 
 class OracleError(RuntimeError):
@@ -243,15 +282,34 @@ def coarse_step(solver, tree, params, aexp=1.0):
     adv(tree.lmin)
 
 
-def ref_available(ngrp: int) -> bool:
-    return os.path.exists(REF_SO.get(ngrp, ""))
+def coarse_step_coarserad(solver, tree, params, aexp=1.0):
+    """One coarse step of the COARSERAD level recursion (Advance_level_RAD,
+    advanceamr.c:1309-1537): every level takes one step of the coarse level's
+    dt_rad; per level: clean, recurse into the finer level, FillRad (the
+    coarse level gathers its leaves' E, F), RadSolver with chemonly=0 (the
+    transport runs on the coarse level only)."""
+    dt = params.dt_rad(tree.lmin, aexp)
+
+    def adv(l):
+        solver.clean_new(l)
+        if l < tree.lmax and tree.level(l + 1).noct > 0:
+            adv(l + 1)
+        solver.fillrad(l, aexp)
+        solver.radsolver(l, dt, aexp, 0)
+
+    adv(tree.lmin)
+
+
+def ref_available(ngrp: int, coarserad: bool = False) -> bool:
+    return os.path.exists((REF_SO_CR if coarserad else REF_SO).get(ngrp, ""))
 
 
 class Reference:
     """The reference's own CPU functions, driven through ref_driver.c."""
 
-    def __init__(self, ngrp: int):
-        path = REF_SO[ngrp]
+    def __init__(self, ngrp: int, coarserad: bool = False):
+        path = (REF_SO_CR if coarserad else REF_SO)[ngrp]
+        self.mode = COARSERAD if coarserad else RADSTEP
         if not os.path.exists(path):
             raise FileNotFoundError(path)
         self.lib = ctypes.CDLL(path)
@@ -280,6 +338,8 @@ class Reference:
                                           ctypes.c_double, ctypes.c_int]
         L.refdrv_set_tphy.argtypes = [ctypes.c_double]
         L.refdrv_rates_n.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.refdrv_fillrad_coarse.argtypes = [ctypes.c_int, ctypes.c_double]
+        assert L.refdrv_coarserad() == (1 if coarserad else 0)
         self.stride = 4096
 
     def layout(self):
@@ -334,7 +394,14 @@ class Reference:
         return self.lib.refdrv_advancerad(level, dtnew, aexp, chemonly, self.stride)
 
     def coarse_step(self, aexp=1.0):
-        coarse_step(self, self.tree, self.params, aexp)
+        if self.mode == COARSERAD:
+            coarse_step_coarserad(self, self.tree, self.params, aexp)
+        else:
+            coarse_step(self, self.tree, self.params, aexp)
+
+    def fillrad(self, level, aexp=1.0):
+        if self.mode == COARSERAD:
+            self.lib.refdrv_fillrad_coarse(level, aexp)
 
     def eddington(self, fx, fy, fz, e, c, i, j):
         fx, fy, fz, e = [np.ascontiguousarray(a, dtype=np.float64) for a in (fx, fy, fz, e)]

@@ -412,7 +412,20 @@ int orc_level_fluxes(int G, int noct, const int *nei_oct, const int *child, cons
 int orc_level_advance(const orc_params *P, int noct, const int *nei_oct, const int *child,
                       const orc_fields *old, orc_fields *newf, const orc_fields *cold, orc_fields *cnew,
                       double dx, double dt, double aexp, int chemonly, int *nsub, int *bad_cell) {
+  return orc_level_advance_mode(P, 0, 1, noct, nei_oct, child, old, newf, cold, cnew, dx, dt, aexp, chemonly, nsub,
+                                bad_cell);
+}
+
+/* mode 0 = RADSTEP; mode 1 = COARSERAD, where transport runs only on the
+   coarse level (is_coarse) and not for chemonly calls (rad_utils.c:1808-1812),
+   fluxes of split neighbours are kept (condsplit=1, 518-526), split cells are
+   updated too (1548-1552) and nothing is refluxed (1740) */
+int orc_level_advance_mode(const orc_params *P, int mode, int is_coarse, int noct, const int *nei_oct,
+                           const int *child, const orc_fields *old, orc_fields *newf, const orc_fields *cold,
+                           orc_fields *cnew, double dx, double dt, double aexp, int chemonly, int *nsub,
+                           int *bad_cell) {
   const int G = P->ngrp;
+  const int condadvec = (mode == 0) ? 1 : (is_coarse && !chemonly);
   const double c = orc_cloc(aexp, P);
   const double dtsurdx = dt / dx;
   double F[30 * ORC_MAXGRP];
@@ -420,19 +433,21 @@ int orc_level_advance(const orc_params *P, int noct, const int *nei_oct, const i
     for (int icell = 0; icell < 8; icell++) {
       int i = oct * 8 + icell;
       slot_state C, N[6];
-      int st = cell_gather(G, nei_oct, child, old, cold, oct, icell, &C, N);
-      if (st != ORC_OK) {
-        if (bad_cell) *bad_cell = i;
-        return st;
+      if (condadvec) {
+        int st = cell_gather(G, nei_oct, child, old, cold, oct, icell, &C, N);
+        if (st != ORC_OK) {
+          if (bad_cell) *bad_cell = i;
+          return st;
+        }
+        if (mode == 1)
+          for (int f = 0; f < 6; f++) N[f].split = 0;
+        cell_fluxes(G, &C, N, c, F);
       }
-      cell_fluxes(G, &C, N, c, F);
-      if (child[i] >= 0) {
-        if (nsub) nsub[i] = 0;
-      } else {
-        orc_cell R;
+      orc_cell R;
+      int updated = 0;
+      if (condadvec && (mode == 1 || child[i] < 0)) { /* updatefieldrad */
         double Re[ORC_MAXGRP] = {0}, Rx[ORC_MAXGRP] = {0}, Ry[ORC_MAXGRP] = {0}, Rz[ORC_MAXGRP] = {0};
         double one = 1.;
-        int ns = 0;
         for (int flx = 0; flx < 6; flx++) { /* 1566-1574 */
           for (int g = 0; g < G; g++) {
             Re[g] += F[0 + g * 5 + flx * 5 * G] * dtsurdx * one;
@@ -455,7 +470,16 @@ int orc_level_advance(const orc_params *P, int noct, const int *nei_oct, const i
           if (bad_cell) *bad_cell = i;
           return ORC_ERR_NEG_PHOTON;
         }
-        st = orc_chem_cell(P, &R, dt, aexp, chemonly, &ns);
+        updated = 1;
+      } else {
+        load_cell(G, newf, i, &R);
+      }
+      if (child[i] >= 0) { /* chemrad skips split cells (chem_utils.c:302) */
+        if (nsub) nsub[i] = 0;
+        if (updated) store_cell(G, newf, i, &R);
+      } else {
+        int ns = 0;
+        int st = orc_chem_cell(P, &R, dt, aexp, chemonly, &ns);
         if (nsub) nsub[i] = ns;
         if (st != ORC_OK) {
           if (bad_cell) *bad_cell = i;
@@ -463,7 +487,7 @@ int orc_level_advance(const orc_params *P, int noct, const int *nei_oct, const i
         }
         store_cell(G, newf, i, &R);
       }
-      if (cnew != NULL) { /* refluxing, 1738-1762 */
+      if (mode == 0 && cnew != NULL) { /* refluxing, 1738-1762 */
         int vnei[6], vcell[6];
         orc_getcellnei(icell, vnei, vcell);
         for (int inei = 0; inei < 6; inei++) {
@@ -490,6 +514,16 @@ int orc_level_advance(const orc_params *P, int noct, const int *nei_oct, const i
 void orc_level_final(int G, int noct, const int *child, orc_fields *old, const orc_fields *newf,
                      const orc_fields *fine_old, double tphy, double *t_first, double *t_last,
                      const double *fine_t_first, const double *fine_t_last) {
+  orc_level_final_mode(G, 0, 0, 0, noct, child, old, newf, fine_old, tphy, t_first, t_last, fine_t_first,
+                       fine_t_last);
+}
+
+/* COARSERAD (mode 1): on the coarse level, unless chemonly, every cell takes
+   its rfieldnew (rad_utils.c:1917-1929); the push of the coarse values down
+   to the leaves (distribE) is orc_distribE */
+void orc_level_final_mode(int G, int mode, int is_coarse, int chemonly, int noct, const int *child, orc_fields *old,
+                          const orc_fields *newf, const orc_fields *fine_old, double tphy, double *t_first,
+                          double *t_last, const double *fine_t_first, const double *fine_t_last) {
   const int ncell = noct * 8;
   for (int i = 0; i < ncell; i++) {
     if (t_first && t_last) { /* 1901-1913 */
@@ -498,7 +532,8 @@ void orc_level_final(int G, int noct, const int *child, orc_fields *old, const o
       if ((xion >= 0.5) && (t_first[i] == -1)) t_first[i] = tphy;
     }
     orc_cell R;
-    if (child[i] < 0) { /* 1934-1940 */
+    const int unsplit = (mode == 0 || chemonly) ? (child[i] < 0) : ((child[i] < 0) || is_coarse);
+    if (unsplit) { /* 1934-1940 */
       load_cell(G, newf, i, &R);
       for (int g = 0; g < G; g++) R.src[g] = old->src[g * ncell + i];
       R.nh = old->nh[i];
@@ -558,5 +593,85 @@ void orc_e2t_n(const orc_params *P, int n, const double *nhplus, const double *e
     R.nh = nh[k];
     orc_e2t(&R, aexp, P);
     temp[k] = R.temp;
+  }
+}
+
+/* ---- COARSERAD transfers between the coarse level and the leaves ----
+   Levels are passed as arrays indexed by level - lcoarse. */
+
+/* collectE (src_utils.c:94-123): depth-first over the subtree, leaves add
+   their E, F (and nhplus, eint) times the volume fraction 2^(3(lcoarse-l)) */
+static void collect_rec(int G, int lcoarse, int l, int nlev, int cell, const int *const *child,
+                        const orc_fields *const *F, orc_fields *C, int ci) {
+  const int k = l - lcoarse;
+  const int ch = child[k][cell];
+  if (ch < 0) {
+    const double fact = pow(2., 3 * (lcoarse - l));
+    const orc_fields *L = F[k];
+    const int n = L->ncell, nc = C->ncell;
+    for (int g = 0; g < G; g++) {
+      C->e[g * nc + ci] += L->e[g * n + cell] * fact;
+      C->fx[g * nc + ci] += L->fx[g * n + cell] * fact;
+      C->fy[g * nc + ci] += L->fy[g * n + cell] * fact;
+      C->fz[g * nc + ci] += L->fz[g * n + cell] * fact;
+    }
+    C->nhplus[ci] += L->nhplus[cell] * fact;
+    C->eint[ci] += L->eint[cell] * fact;
+  } else if (k + 1 < nlev) {
+    for (int c = 0; c < 8; c++) collect_rec(G, lcoarse, l + 1, nlev, ch * 8 + c, child, F, C, ci);
+  }
+}
+
+/* putE2coarse (src_utils.c:127-150) for every split cell of the coarse level,
+   as FillRad does under COARSERAD (src_utils.c:717-733) */
+void orc_putE2coarse(int G, int lcoarse, int nlev, const int *const *child, orc_fields *const *old,
+                     orc_fields *newc) {
+  orc_fields *C = old[0];
+  const int nc = C->ncell;
+  for (int i = 0; i < nc; i++) {
+    if (child[0][i] < 0) continue;
+    for (int g = 0; g < G; g++) {
+      C->e[g * nc + i] = 0.;
+      C->fx[g * nc + i] = 0.;
+      C->fy[g * nc + i] = 0.;
+      C->fz[g * nc + i] = 0.;
+    }
+    for (int c = 0; c < 8; c++)
+      if (nlev > 1) collect_rec(G, lcoarse, lcoarse + 1, nlev, child[0][i] * 8 + c, child, (const orc_fields *const *)old, C, i);
+    for (int g = 0; g < G; g++) {
+      newc->e[g * nc + i] = C->e[g * nc + i];
+      newc->fx[g * nc + i] = C->fx[g * nc + i];
+      newc->fy[g * nc + i] = C->fy[g * nc + i];
+      newc->fz[g * nc + i] = C->fz[g * nc + i];
+    }
+  }
+}
+
+/* distribE (rad_utils.c:26-45): every leaf below a split coarse cell takes the
+   coarse cell's E and F */
+static void distrib_rec(int G, int lcoarse, int l, int nlev, int cell, const int *const *child, orc_fields *const *F,
+                        const orc_fields *C, int ci) {
+  const int k = l - lcoarse;
+  const int ch = child[k][cell];
+  if (ch < 0) {
+    orc_fields *L = F[k];
+    const int n = L->ncell, nc = C->ncell;
+    for (int g = 0; g < G; g++) {
+      L->e[g * n + cell] = C->e[g * nc + ci];
+      L->fx[g * n + cell] = C->fx[g * nc + ci];
+      L->fy[g * n + cell] = C->fy[g * nc + ci];
+      L->fz[g * n + cell] = C->fz[g * nc + ci];
+    }
+  } else if (k + 1 < nlev) {
+    for (int c = 0; c < 8; c++) distrib_rec(G, lcoarse, l + 1, nlev, ch * 8 + c, child, F, C, ci);
+  }
+}
+
+void orc_distribE(int G, int lcoarse, int nlev, const int *const *child, orc_fields *const *old) {
+  const orc_fields *C = old[0];
+  for (int i = 0; i < C->ncell; i++) {
+    if (child[0][i] < 0) continue;
+    for (int c = 0; c < 8; c++)
+      if (nlev > 1) distrib_rec(G, lcoarse, lcoarse + 1, nlev, child[0][i] * 8 + c, child, old, C, i);
   }
 }

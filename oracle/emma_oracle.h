@@ -87,6 +87,19 @@ int orc_level_advance(const orc_params *P, int noct, const int *nei_oct, const i
                       const orc_fields *old, orc_fields *newf, const orc_fields *cold, orc_fields *cnew,
                       double dx, double dt, double aexp, int chemonly, int *nsub, int *bad_cell);
 
+/* the same with mode 0 = RADSTEP, 1 = COARSERAD (is_coarse: level == lcoarse) */
+int orc_level_advance_mode(const orc_params *P, int mode, int is_coarse, int noct, const int *nei_oct,
+                           const int *child, const orc_fields *old, orc_fields *newf, const orc_fields *cold,
+                           orc_fields *cnew, double dx, double dt, double aexp, int chemonly, int *nsub,
+                           int *bad_cell);
+void orc_level_final_mode(int G, int mode, int is_coarse, int chemonly, int noct, const int *child, orc_fields *old,
+                          const orc_fields *newf, const orc_fields *fine_old, double tphy, double *t_first,
+                          double *t_last, const double *fine_t_first, const double *fine_t_last);
+/* COARSERAD: putE2coarse on every split coarse cell, and distribE */
+void orc_putE2coarse(int G, int lcoarse, int nlev, const int *const *child, orc_fields *const *old,
+                     orc_fields *newc);
+void orc_distribE(int G, int lcoarse, int nlev, const int *const *child, orc_fields *const *old);
+
 /* RadSolver's final update on one level (rfield := rfieldnew, restriction of split cells) */
 void orc_level_final(int ngrp, int noct, const int *child, orc_fields *old, const orc_fields *newf,
                      const orc_fields *fine_old, double tphy, double *t_first, double *t_last,

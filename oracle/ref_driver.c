@@ -38,6 +38,10 @@ REAL cucompute_alpha_a(REAL temp, REAL unit_number, REAL aexp);
 REAL cucompute_beta(REAL temp, REAL unit_number, REAL aexp);
 void cuCompCooling(REAL temp, REAL x, REAL nH, REAL *lambda, REAL *tcool, REAL aexp, REAL CLUMPF, int src);
 
+#ifdef COARSERAD
+int putE2coarse(struct CELL *cell, struct RUNPARAMS *param, int level, REAL aexp, REAL tcur);
+#endif
+
 #define MAXLEV 24
 
 static struct RUNPARAMS g_param;
@@ -59,6 +63,15 @@ static void ensure_mpi(void) {
 }
 
 int refdrv_ngrp(void) { return NGRP; }
+
+/* 1 when built with COARSERAD (transport on the coarse level only) */
+int refdrv_coarserad(void) {
+#ifdef COARSERAD
+  return 1;
+#else
+  return 0;
+#endif
+}
 int refdrv_real_bytes(void) { return (int)sizeof(REAL); }
 
 /* layout table of the reference structs for the library's layout self-check */
@@ -331,4 +344,21 @@ void refdrv_rates_n(int n, const double *temp, const double *x, const double *nH
     lambda[k] = l;
     tcool[k] = t;
   }
+}
+
+/* FillRad's COARSERAD block (src_utils.c:717-733): on the coarse level, every
+   split cell gathers the volume-weighted E and F of the leaves below it */
+int refdrv_fillrad_coarse(int level, double aexp) {
+#ifdef COARSERAD
+  if (level != g_param.lcoarse) return 0;
+  int ns = 0;
+  for (int i = 0; i < g_noct[level]; i++)
+    for (int c = 0; c < 8; c++)
+      if (g_lev[level][i].cell[c].child != NULL) ns += putE2coarse(&g_lev[level][i].cell[c], &g_param, level, aexp, 0.);
+  return ns;
+#else
+  (void)level;
+  (void)aexp;
+  return -1;
+#endif
 }

@@ -83,6 +83,19 @@ def test_mixed_bound(gpu_lib, oracle):
     assert max(err["fx"], err["fy"], err["fz"]) < MIXED_TOL, err
 
 
+@pytest.mark.parametrize("table,G", [("SB99_equiv_mono_TH", 1), ("Starburst99_1grpT_3grpS", 3)])
+def test_amr_mixed_bound(gpu_lib, oracle, table, G):
+    """mixed precision on a 4-level tree: partial LDS tiles, coarse faces,
+    FP32 reflux accumulators"""
+    W = synth.amr_workload(3, [60, 100, 80], table, seed=11, sigma=1.5, nsrc=2, grid=16)
+    g = run_pair(oracle, W, rt.MIXED, steps=2, mode="graph")
+    for l in range(W.tree.lmin, W.tree.lmax + 1):
+        err = field_errors(g.get(l), oracle.get(l), W.params.cloc())
+        print("amr mixed G=%d level %d" % (G, l), err)
+        assert err["x"] < MIXED_TOL and err["temp"] < MIXED_TOL and err["e"] < MIXED_TOL, (l, err)
+        assert max(err["fx"], err["fy"], err["fz"]) < MIXED_TOL, (l, err)
+
+
 def test_photon_budget_transport_only(gpu_lib):
     """Transport conserves sum N dx^3 on a periodic level up to the chemistry
     absorption: with n_H -> tiny and x -> 1 the absorption vanishes and the FP64
@@ -132,3 +145,29 @@ def test_status_negative_photons(gpu_lib):
     with pytest.raises(rt.EmmaRTError) as ei:
         g.radsolver(3, W.params.dt_rad(3))
     assert ei.value.status == rt.ERR_NEG_PHOTON
+
+
+@pytest.mark.parametrize("table,G", [("SB99_equiv_mono_TH", 1), ("Starburst99_1grpT_3grpS", 3)])
+@pytest.mark.parametrize("mode", ["host", "graph"])
+@pytest.mark.parametrize("precision", [rt.FP64, rt.MIXED])
+def test_coarserad(gpu_lib, oracle, table, G, mode, precision):
+    """COARSERAD: transport on the coarse level only, chemistry on every leaf,
+    putE2coarse before and distribE after the coarse step."""
+    from oracle import COARSERAD, coarse_step_coarserad
+
+    W = synth.amr_workload(3, [60, 100, 80], table, seed=20 + G, sigma=1.5, nsrc=3, grid=16)
+    g = rt.RadGPU(W.params, precision)
+    g.setup(W.params, W.tree, W.fields, mode=rt.COARSERAD)
+    oracle.setup(W.params, W.tree, W.fields, mode=COARSERAD)
+    for _ in range(3):
+        if mode == "graph":
+            g.coarse_step(1.0, 1)
+        else:
+            coarse_step_coarserad(g, W.tree, W.params)
+        oracle.coarse_step()
+    tol = FP64_TOL if precision == rt.FP64 else MIXED_TOL
+    for l in range(W.tree.lmin, W.tree.lmax + 1):
+        err = field_errors(g.get(l), oracle.get(l), W.params.cloc())
+        print("coarserad G=%d prec=%d level %d" % (G, precision, l), err)
+        assert err["x"] < tol and err["temp"] < tol and err["e"] < tol and err["eint"] < tol, (l, err)
+        assert max(err["fx"], err["fy"], err["fz"]) < tol, (l, err)

@@ -117,3 +117,22 @@ def test_reference_layout_matches_numpy_view():
     for G in (1, 3):
         need(G)
         assert np.array_equal(Reference(G).layout(), rgrid.layout(G))
+
+
+@pytest.mark.parametrize("G", [1, 3])
+def test_coarserad_steps(G):
+    """COARSERAD: transport on the coarse level with split-cell fluxes kept,
+    chemistry only below, putE2coarse / distribE between the coarse level and
+    the leaves (rad_utils.c:26-45, src_utils.c:94-150, advanceamr.c:1309)."""
+    from oracle import COARSERAD
+
+    if not ref_available(G, coarserad=True):
+        pytest.skip("reference COARSERAD build not present")
+    W = synth.amr_workload(3, [60, 100, 80], TABLES[G], seed=20 + G, sigma=1.5, nsrc=3, grid=16)
+    o, r = Oracle(), Reference(G, coarserad=True)
+    o.setup(W.params, W.tree, W.fields, mode=COARSERAD)
+    r.setup(W.params, W.tree, W.fields)
+    for _ in range(3):
+        o.coarse_step()
+        r.coarse_step()
+    same_state(o, r, W.tree)
