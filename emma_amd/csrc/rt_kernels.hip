@@ -408,7 +408,8 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
         }
         const float one = (f & 1) ? -1.f : 1.f;
         if (jl >= 0) {
-          const float ffact = (sSplit[jl] && !a.keep_split_flux) ? 0.f : 1.f;
+          // split factor (0/1), face sign and dt/dx folded into one exact factor
+          const float w = (sSplit[jl] && !a.keep_split_flux) ? 0.f : dts * one;
 #pragma unroll
           for (int g = 0; g < G; g++) {
             Closure<float> qn;
@@ -418,12 +419,12 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
             qn.n[1] = sN1[g][jl];
             qn.n[2] = sN2[g][jl];
             float F[4];
-            lf_flux<float>(d, s, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], sE[g][jl], sX[g][jl], sY[g][jl], sZ[g][jl], qn,
-                           hc, F);
-            De[g] += F[0] * ffact * dts * one;
-            Dx[g] += F[1] * ffact * dts * one;
-            Dy[g] += F[2] * ffact * dts * one;
-            Dz[g] += F[3] * ffact * dts * one;
+            lf_flux_fast(d, s, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], sE[g][jl], sX[g][jl], sY[g][jl], sZ[g][jl], qn, hc,
+                         F);
+            De[g] = fmaf(F[0], w, De[g]);
+            Dx[g] = fmaf(F[1], w, Dx[g]);
+            Dy[g] = fmaf(F[2], w, Dy[g]);
+            Dz[g] = fmaf(F[3], w, Dz[g]);
           }
         } else {
           const float* Ne = coarse_face ? a.ce : a.e;
@@ -437,13 +438,14 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
             const float ve = Ne[g * nn + j], vx = Nx[g * nn + j], vy = Ny[g * nn + j], vz = Nz[g * nn + j];
             const Closure<float> qn = closure_fast(vx, vy, vz, ve, c);
             float F[4];
-            lf_flux<float>(d, s, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], ve, vx, vy, vz, qn, hc, F);
+            lf_flux_fast(d, s, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], ve, vx, vy, vz, qn, hc, F);
 #pragma unroll
             for (int v = 0; v < 4; v++) F[v] *= ffact;
-            De[g] += F[0] * dts * one;
-            Dx[g] += F[1] * dts * one;
-            Dy[g] += F[2] * dts * one;
-            Dz[g] += F[3] * dts * one;
+            const float w = dts * one;
+            De[g] = fmaf(F[0], w, De[g]);
+            Dx[g] = fmaf(F[1], w, Dx[g]);
+            Dy[g] = fmaf(F[2], w, Dy[g]);
+            Dz[g] = fmaf(F[3], w, Dz[g]);
             if (coarse_face) reflux<float, G>(a, j, f, g, F, dts);
           }
         }

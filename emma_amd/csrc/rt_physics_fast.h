@@ -30,6 +30,7 @@ __device__ __forceinline__ double fdiv64(double a, double b) {
 
 // Levermore closure (rad_utils.c:448-475) with fast FP32 reciprocals
 __device__ __forceinline__ Closure<float> closure_fast(float fx, float fy, float fz, float ee, float c) {
+#pragma clang fp contract(fast)
   Closure<float> q;
   const float c2e = ee * c * c;
   float ff = 0.f;
@@ -55,6 +56,7 @@ __device__ __forceinline__ Closure<float> closure_fast(float fx, float fy, float
 // the rate fits of chem_utils.c:37-187 in FP32: pow(x, a) = exp2(a log2 x)
 // with log2(lambda) shared, exp(-E/T) = exp2(-E log2(e) / T)
 __device__ __forceinline__ Rates rates_fast(double Td) {
+#pragma clang fp contract(fast)
   const float T = (float)Td;
   const float rT = frcp32(T);
   const float sT = __builtin_amdgcn_sqrtf(T);
@@ -80,6 +82,7 @@ template <int G>
 __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fx, double* fy, double* fz,
                               const double* src, double& nhplus, double& eint_code, double nh, double& temp,
                               int chemonly, int& nit) {
+#pragma clang fp contract(fast)
   constexpr int GA = G > 0 ? G : kMaxGrp;
   double egy[GA], fl0[GA], fl1[GA], fl2[GA], srcl[GA], et[GA], p[GA];
   const double notchem = (double)(!chemonly);
@@ -218,6 +221,29 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fx,
   // E2T: eint_code back in physical units is eint itself
   temp = eint * kT * frcp64((nh * K.unit_N / a3f) * (1. + x0));
   return 0;
+}
+
+// the Lax-Friedrichs face flux of lf_flux with contraction allowed (mixed path)
+__device__ __forceinline__ void lf_flux_fast(int d, int s, float Ue, float Ux, float Uy, float Uz,
+                                             const Closure<float>& qu, float Ve, float Vx, float Vy, float Vz,
+                                             const Closure<float>& qv, float hc, float F[4]) {
+#pragma clang fp contract(fast)
+  const float Me = s ? Ue : Ve, Pe = s ? Ve : Ue;
+  const float Mf[3] = {s ? Ux : Vx, s ? Uy : Vy, s ? Uz : Vz};
+  const float Pf[3] = {s ? Vx : Ux, s ? Vy : Uy, s ? Vz : Uz};
+  const Closure<float>& qm = s ? qu : qv;
+  const Closure<float>& qp = s ? qv : qu;
+  F[0] = 0.5f * (Pf[d] + Mf[d]) + hc * (Me - Pe);
+  const float wp = qp.arg * qp.n[d], wm = qm.arg * qm.n[d];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    float pp = wp * qp.n[i], pm = wm * qm.n[i];
+    if (i == d) {
+      pp += qp.diag;
+      pm += qm.diag;
+    }
+    F[1 + i] = 0.5f * (pp + pm) + hc * (Mf[i] - Pf[i]);
+  }
 }
 
 }  // namespace emma

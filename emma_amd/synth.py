@@ -103,7 +103,7 @@ def gaussian_field(n: int, rng: np.random.Generator, smooth_cells: float = 2.0) 
     k = np.fft.fftfreq(n) * n
     kx, ky, kz = np.meshgrid(k, k, k[: n // 2 + 1], indexing="ij")
     filt = np.exp(-0.5 * (kx**2 + ky**2 + kz**2) * (2 * np.pi * smooth_cells / n) ** 2)
-    g = np.fft.irfftn(np.fft.rfftn(w) * filt, s=(n, n, n))
+    g = np.fft.irfftn(np.fft.rfftn(w) * filt, s=(n, n, n), axes=(0, 1, 2))
     g -= g.mean()
     g /= g.std()
     return g
