@@ -46,6 +46,25 @@ __device__ __forceinline__ T sel(int i, T a, T b, T c) {
   return i == 0 ? a : (i == 1 ? b : c);
 }
 
+// The level-step kernels take (LevelArgs, ChemConst) by value but read them
+// through the kernarg segment pointer where they are used.  By-value kernel
+// arguments are all loaded into SGPRs at kernel entry; a level step has more
+// of them than there are SGPRs, and the overflow is spilled to VGPR lanes and
+// read back with a v_readlane per use.  Loads at the point of use keep only
+// the live ones in SGPRs.  The explicit arguments sit at offset 0 and right
+// after the LevelArgs block (both 8-byte aligned).
+template <typename R>
+struct KernArgs {
+  const LevelArgs<R>& a;
+  const ChemConst& K;
+};
+template <typename R>
+__device__ __forceinline__ KernArgs<R> kernargs() {
+  static_assert(sizeof(LevelArgs<R>) % alignof(ChemConst) == 0, "ChemConst follows LevelArgs unpadded");
+  const char* kp = (const char*)__builtin_amdgcn_kernarg_segment_ptr();  // constant address space -> generic
+  return {*reinterpret_cast<const LevelArgs<R>*>(kp), *reinterpret_cast<const ChemConst*>(kp + sizeof(LevelArgs<R>))};
+}
+
 // ------------------------------------------------------------------ pieces
 // reionisation-time bookkeeping of RadSolver (rad_utils.c:1901-1913)
 template <typename R>
@@ -233,7 +252,10 @@ __device__ __forceinline__ void block_counters(unsigned long long* counters, uns
 // the mixed-precision physics (rt_physics_fast.h); the FP64 kernel
 // (Fast = false) keeps the reference's operation order.
 template <typename R, int G, bool Fast>
-__global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a, ChemConst K) {
+__global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value, ChemConst K_by_value) {
+  const KernArgs<R> args = kernargs<R>();
+  const LevelArgs<R>& a = args.a;
+  const ChemConst& K = args.K;
   constexpr int GA = G > 0 ? G : kMaxGrp;
   const int ng = G > 0 ? G : a.ng;
   const int cell = xcd_block(a.nblocks) * 256 + (int)threadIdx.x;
@@ -341,7 +363,11 @@ constexpr int kTileCells = 512;
 constexpr int kTileOcts = kTileCells / 8;
 
 template <int G>
-__global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4))) void level_step_tiled(LevelArgs<float> a, ChemConst K) {
+__global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4))) void level_step_tiled(
+    LevelArgs<float> a_by_value, ChemConst K_by_value) {
+  const KernArgs<float> args = kernargs<float>();
+  const LevelArgs<float>& a = args.a;
+  const ChemConst& K = args.K;
   __shared__ float sE[G][kTileCells], sX[G][kTileCells], sY[G][kTileCells], sZ[G][kTileCells];
   __shared__ float sDiag[G][kTileCells], sArg[G][kTileCells];
   __shared__ float sN0[G][kTileCells], sN1[G][kTileCells], sN2[G][kTileCells];
