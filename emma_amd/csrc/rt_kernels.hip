@@ -354,13 +354,48 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value
 
 // ------------------------------------------------------------------ tiled level step
 // Mixed precision, G fixed.  A 512-thread block owns 64 consecutive octs
-// (Morton order: a 4x4x4 cube of octs on a complete level).  Every thread
-// puts its cell's state and closure in LDS; face neighbours inside the tile
-// come from LDS, so each closure is computed once instead of once per
-// neighbouring face; neighbours outside the tile (and injected coarse
-// leaves) are read from global memory as in level_step_kernel.
+// (Morton order: a 4x4x4 cube of octs on a complete level), one cell per
+// thread.
+//
+// The Lax-Friedrichs flux through a face normal to d splits into one share
+// per side: F = A(M) + B(P) with A = h + hc U on the minus side and
+// B = h - hc U on the plus side, h the halved physical flux (lf_share).  So
+// each cell computes its own A_d and B_d once per direction and a face costs
+// one LDS read of the neighbour's share plus an add.  The block runs the
+// three directions as three passes over one LDS share buffer.
+//
+// Neighbours outside the tile (same-level octs of other tiles, injected
+// coarse leaves) are the halo.  Each wave lists its halo faces (lane,
+// direction) and evaluates them lane-parallel in one pass before the
+// direction passes: the neighbour's share and whether the face flux is
+// cancelled (split neighbour, RADSTEP) go to LDS slots of that wave, so an
+// in-tile and a halo face differ only in the LDS address read.  A complete
+// level has 48 halo faces per wave (16 per direction); faces beyond kHaloCap
+// (sparse AMR tiles) evaluate their neighbour's share from global memory.
 constexpr int kTileCells = 512;
 constexpr int kTileOcts = kTileCells / 8;
+constexpr int kTileWaves = kTileCells / 64;
+constexpr int kHaloCap = 64;
+
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// the share (A if side = +1, B if side = -1) in direction d of cell j of
+// this level (coarse = 0) or of the coarser level (coarse = 1), group g
+template <int G>
+__device__ __forceinline__ float4 share_of(const LevelArgs<float>& a, bool coarse, int j, int g, int d, float side) {
+  const float* Ne = coarse ? a.ce : a.e;
+  const float* Nx = coarse ? a.cfx : a.fx;
+  const float* Ny = coarse ? a.cfy : a.fy;
+  const float* Nz = coarse ? a.cfz : a.fz;
+  const int nn = coarse ? a.ncoarse : a.ncell;
+  const float ve = Ne[g * nn + j], vx = Nx[g * nn + j], vy = Ny[g * nn + j], vz = Nz[g * nn + j];
+  const Closure<float> q = closure_fast(vx, vy, vz, ve, a.c);
+  float X[4];
+  lf_share(d, side, ve, vx, vy, vz, q, a.hc, X);
+  return make_float4(X[0], X[1], X[2], X[3]);
+}
 
 template <int G>
 __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4))) void level_step_tiled(
@@ -368,12 +403,16 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   const KernArgs<float> args = kernargs<float>();
   const LevelArgs<float>& a = args.a;
   const ChemConst& K = args.K;
-  __shared__ float sE[G][kTileCells], sX[G][kTileCells], sY[G][kTileCells], sZ[G][kTileCells];
-  __shared__ float sDiag[G][kTileCells], sArg[G][kTileCells];
-  __shared__ float sN0[G][kTileCells], sN1[G][kTileCells], sN2[G][kTileCells];
-  __shared__ unsigned char sSplit[kTileCells];
+  // shares: [g][A or B][cell] of the current direction, then the halo slots
+  // [wave][slot][g]
+  constexpr int kShares = G * 2 * kTileCells;
+  __shared__ float4 sShare[kShares + kTileWaves * kHaloCap * G];
+  // 1: the flux through the face is cancelled; [cell], then [wave][slot]
+  __shared__ unsigned char sCancel[kTileCells + kTileWaves * kHaloCap];
+  __shared__ unsigned char sItem[kTileWaves][kHaloCap];  // halo slot -> lane*4 + direction
   const int tile = xcd_block(a.nblocks);
   const int lc = (int)threadIdx.x;
+  const int wv = lc >> 6, lane = lc & 63;
   const int cell = tile * kTileCells + lc;
   const int oct0 = tile * kTileOcts;
   const bool valid = cell < a.ncell;
@@ -391,100 +430,140 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
       Uy[g] = a.fy[g * n + cell];
       Uz[g] = a.fz[g * n + cell];
       Q[g] = closure_fast(Ux[g], Uy[g], Uz[g], Ue[g], c);
-      sE[g][lc] = Ue[g];
-      sX[g][lc] = Ux[g];
-      sY[g][lc] = Uy[g];
-      sZ[g][lc] = Uz[g];
-      sDiag[g][lc] = Q[g].diag;
-      sArg[g][lc] = Q[g].arg;
-      sN0[g][lc] = Q[g].n[0];
-      sN1[g][lc] = Q[g].n[1];
-      sN2[g][lc] = Q[g].n[2];
     }
   }
-  sSplit[lc] = (unsigned char)(ch >= 0);
-  __syncthreads();
+  sCancel[lc] = (unsigned char)(ch >= 0 && !a.keep_split_flux);
+
+  // ---- halo: per direction, the one face of the cell that leaves its oct
+  const int oct = cell >> 3, ic = cell & 7;
+  const bool transport = a.transport != 0;
+  const bool moves = valid && transport && !(ch >= 0 && !a.update_split);
+  int noD[3];                   // nei_oct entry of that face
+  unsigned long long hball[3];  // lanes whose face in direction d is a halo face
+  int hbase[3];                 // first halo slot of direction d
+  int nhalo = 0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int f = 2 * d + ((ic >> d) & 1);
+    noD[d] = moves ? a.nei[oct * 6 + f] : 0;
+    const bool out = moves && !(noD[d] >= oct0 && noD[d] < oct0 + kTileOcts);
+    hball[d] = __ballot(out);
+    hbase[d] = nhalo;
+    nhalo += __popcll(hball[d]);
+    if (out) {
+      const int slot = hbase[d] + lanes_below(hball[d]);
+      if (slot < kHaloCap) sItem[wv][slot] = (unsigned char)(lane * 4 + d);
+    }
+  }
+  // the list is written and read by lanes of the same wave
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane < nhalo && lane < kHaloCap) {
+    const int it = sItem[wv][lane];
+    const int d = it & 3;
+    const int hcell = tile * kTileCells + wv * 64 + (it >> 2);
+    const int hic = hcell & 7;
+    const int s = (hic >> d) & 1;
+    const int no = a.nei[(hcell >> 3) * 6 + 2 * d + s];
+    const bool coarse = no < 0;
+    const int j = coarse ? -1 - no : no * 8 + (hic ^ (1 << d));
+    const int hs = kTileCells + wv * kHaloCap + lane;
+    sCancel[hs] = (unsigned char)(!coarse && !a.keep_split_flux && a.child[j] >= 0);
+    // a neighbour on the plus side (s = 1) contributes B, on the minus side A
+#pragma unroll
+    for (int g = 0; g < G; g++)
+      sShare[kShares + (wv * kHaloCap + lane) * G + g] = share_of<G>(a, coarse, j, g, d, s ? -1.f : 1.f);
+  }
+
+  float De[G], Dx[G], Dy[G], Dz[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) De[g] = Dx[g] = Dy[g] = Dz[g] = 0.f;
+  if (transport) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      float4 A[G], B[G];
+      if (valid) {
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+          float X[4];
+          lf_share(d, 1.f, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], hc, X);
+          A[g] = make_float4(X[0], X[1], X[2], X[3]);
+          lf_share(d, -1.f, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], hc, X);
+          B[g] = make_float4(X[0], X[1], X[2], X[3]);
+          sShare[(g * 2 + 0) * kTileCells + lc] = A[g];
+          sShare[(g * 2 + 1) * kTileCells + lc] = B[g];
+        }
+      }
+      __syncthreads();
+      if (moves) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+          // neighbour on the minus side (s = 0) gives A, on the plus side B;
+          // the face flux is the neighbour's share plus this cell's other one
+          const int nc = ic ^ (1 << d);
+          const float one = s ? -1.f : 1.f;
+          int idx, stride, cidx, slot = 0;
+          bool halo = false;
+          if (((ic >> d) & 1) != s) {
+            idx = s * kTileCells + (lc ^ (1 << d));  // same oct
+            stride = 2 * kTileCells;
+            cidx = lc ^ (1 << d);
+          } else if (noD[d] >= oct0 && noD[d] < oct0 + kTileOcts) {
+            const int jl = (noD[d] - oct0) * 8 + nc;
+            idx = s * kTileCells + jl;
+            stride = 2 * kTileCells;
+            cidx = jl;
+          } else {
+            halo = true;
+            slot = hbase[d] + lanes_below(hball[d]);
+            idx = kShares + (wv * kHaloCap + slot) * G;
+            stride = 1;
+            cidx = kTileCells + wv * kHaloCap + slot;
+          }
+          const bool overflow = halo && slot >= kHaloCap;
+          const bool coarse_face = halo && noD[d] < 0;
+          const int j = coarse_face ? -1 - noD[d] : noD[d] * 8 + nc;
+          float w;
+          if (!overflow) {
+            w = sCancel[cidx] ? 0.f : dts * one;
+          } else {
+            w = (!coarse_face && !a.keep_split_flux && a.child[j] >= 0) ? 0.f : dts * one;
+          }
+#pragma unroll
+          for (int g = 0; g < G; g++) {
+            float4 X;
+            if (!overflow)
+              X = sShare[idx + g * stride];
+            else
+              X = share_of<G>(a, coarse_face, j, g, d, one);
+            const float4 O = s ? A[g] : B[g];
+            float F[4] = {X.x + O.x, X.y + O.y, X.z + O.z, X.w + O.w};
+            De[g] = fmaf(F[0], w, De[g]);
+            Dx[g] = fmaf(F[1], w, Dx[g]);
+            Dy[g] = fmaf(F[2], w, Dy[g]);
+            Dz[g] = fmaf(F[3], w, Dz[g]);
+            if (coarse_face) reflux<float, G>(a, j, 2 * d + s, g, F, dts);
+          }
+        }
+      }
+      if (d < 2) __syncthreads();  // the next direction overwrites the shares
+    }
+  }
+
   unsigned int nit_acc = 0, leaf = 0;
   if (valid) {
-    const int oct = cell >> 3, ic = cell & 7;
     if (a.t_first != nullptr) xion_times(a, cell);
     if (ch >= 0 && !a.update_split) {
       restrict_cell(a, cell, ch, G);
+    } else if (ch >= 0) {
+      transport_split_cell<float, G>(a, cell, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz);
     } else {
-      float De[G], Dx[G], Dy[G], Dz[G];
-#pragma unroll
-      for (int g = 0; g < G; g++) De[g] = Dx[g] = Dy[g] = Dz[g] = 0.f;
-#pragma unroll
-      for (int f = 0; f < (a.transport ? 6 : 0); f++) {
-        const int d = f >> 1, s = f & 1;
-        const int nc = ic ^ (1 << d);
-        int jl = -1, j = 0, coarse_face = 0;
-        if (((ic >> d) & 1) != s) {
-          jl = lc ^ (1 << d);  // same oct
-        } else {
-          const int no = a.nei[oct * 6 + f];
-          if (no >= oct0 && no < oct0 + kTileOcts) {
-            jl = (no - oct0) * 8 + nc;
-          } else if (no >= 0) {
-            j = no * 8 + nc;
-          } else {
-            j = -1 - no;
-            coarse_face = 1;
-          }
-        }
-        const float one = (f & 1) ? -1.f : 1.f;
-        if (jl >= 0) {
-          // split factor (0/1), face sign and dt/dx folded into one exact factor
-          const float w = (sSplit[jl] && !a.keep_split_flux) ? 0.f : dts * one;
-#pragma unroll
-          for (int g = 0; g < G; g++) {
-            Closure<float> qn;
-            qn.diag = sDiag[g][jl];
-            qn.arg = sArg[g][jl];
-            qn.n[0] = sN0[g][jl];
-            qn.n[1] = sN1[g][jl];
-            qn.n[2] = sN2[g][jl];
-            float F[4];
-            lf_flux_fast(d, s, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], sE[g][jl], sX[g][jl], sY[g][jl], sZ[g][jl], qn, hc,
-                         F);
-            De[g] = fmaf(F[0], w, De[g]);
-            Dx[g] = fmaf(F[1], w, Dx[g]);
-            Dy[g] = fmaf(F[2], w, Dy[g]);
-            Dz[g] = fmaf(F[3], w, Dz[g]);
-          }
-        } else {
-          const float* Ne = coarse_face ? a.ce : a.e;
-          const float* Nx = coarse_face ? a.cfx : a.fx;
-          const float* Ny = coarse_face ? a.cfy : a.fy;
-          const float* Nz = coarse_face ? a.cfz : a.fz;
-          const int nn = coarse_face ? a.ncoarse : n;
-          const float ffact = (!coarse_face && !a.keep_split_flux && a.child[j] >= 0) ? 0.f : 1.f;
-#pragma unroll
-          for (int g = 0; g < G; g++) {
-            const float ve = Ne[g * nn + j], vx = Nx[g * nn + j], vy = Ny[g * nn + j], vz = Nz[g * nn + j];
-            const Closure<float> qn = closure_fast(vx, vy, vz, ve, c);
-            float F[4];
-            lf_flux_fast(d, s, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], ve, vx, vy, vz, qn, hc, F);
-#pragma unroll
-            for (int v = 0; v < 4; v++) F[v] *= ffact;
-            const float w = dts * one;
-            De[g] = fmaf(F[0], w, De[g]);
-            Dx[g] = fmaf(F[1], w, Dx[g]);
-            Dy[g] = fmaf(F[2], w, Dy[g]);
-            Dz[g] = fmaf(F[3], w, Dz[g]);
-            if (coarse_face) reflux<float, G>(a, j, f, g, F, dts);
-          }
-        }
-      }
-      if (ch >= 0) {
-        transport_split_cell<float, G>(a, cell, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz);
-      } else {
-        leaf = 1;
-        nit_acc = (unsigned)finish_leaf<float, G, true>(a, K, cell, ch, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz);
-      }
+      leaf = 1;
+      nit_acc = (unsigned)finish_leaf<float, G, true>(a, K, cell, ch, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz);
     }
   }
-  if (a.counters != nullptr) block_counters<kTileCells / 64>(a.counters, nit_acc, leaf);
+  if (a.counters != nullptr) block_counters<kTileWaves>(a.counters, nit_acc, leaf);
 }
 
 // ------------------------------------------------------------------ COARSERAD transfers

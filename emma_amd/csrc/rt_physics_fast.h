@@ -31,20 +31,16 @@ __device__ __forceinline__ double fdiv64(double a, double b) {
 // Levermore closure (rad_utils.c:448-475) with fast FP32 reciprocals
 __device__ __forceinline__ Closure<float> closure_fast(float fx, float fy, float fz, float ee, float c) {
 #pragma clang fp contract(fast)
+  // branch-free: the isotropic case (E <= 0 or F = 0: n = 0, f = 0) by selects
   Closure<float> q;
   const float c2e = ee * c * c;
-  float ff = 0.f;
-  q.n[0] = q.n[1] = q.n[2] = 0.f;
-  if (ee > 0.f) {
-    const float f2 = fx * fx + fy * fy + fz * fz;
-    if (f2 > 1e-36f) {
-      const float ri = __builtin_amdgcn_rsqf(f2);
-      q.n[0] = fx * ri;
-      q.n[1] = fy * ri;
-      q.n[2] = fz * ri;
-      ff = f2 * ri * frcp32(c * ee);
-    }
-  }
+  const float f2 = fx * fx + fy * fy + fz * fz;
+  const bool aniso = (ee > 0.f) & (f2 > 1e-36f);
+  const float ri = aniso ? __builtin_amdgcn_rsqf(f2) : 0.f;
+  q.n[0] = fx * ri;
+  q.n[1] = fy * ri;
+  q.n[2] = fz * ri;
+  const float ff = aniso ? f2 * ri * frcp32(c * ee) : 0.f;
   const float ff2 = ff * ff;
   const float arg = fmaxf(4.f - 3.f * ff2, 0.f);
   const float chi = (3.f + 4.f * ff2) * frcp32(5.f + 2.f * __builtin_amdgcn_sqrtf(arg));
@@ -243,6 +239,26 @@ __device__ __forceinline__ void lf_flux_fast(int d, int s, float Ue, float Ux, f
       pm += qm.diag;
     }
     F[1 + i] = 0.5f * (pp + pm) + hc * (Mf[i] - Pf[i]);
+  }
+}
+
+// One side's share of the Lax-Friedrichs face flux in direction d.  With
+// h(U) = (F_d, P_d0, P_d1, P_d2) / 2 the physical flux halved (P_di = diag
+// delta_di + arg n_d n_i), the face flux of lf_flux is A(M) + B(P) where
+// A = h + hc U for the state on the minus side and B = h - hc U for the
+// state on the plus side.  side = +1 gives A, -1 gives B.
+__device__ __forceinline__ void lf_share(int d, float side, float Ue, float Ux, float Uy, float Uz,
+                                         const Closure<float>& q, float hc, float X[4]) {
+#pragma clang fp contract(fast)
+  const float U[4] = {Ue, Ux, Uy, Uz};
+  const float hs = side * hc;
+  X[0] = fmaf(hs, Ue, 0.5f * U[1 + d]);
+  const float w = 0.5f * q.arg * q.n[d];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    float h = w * q.n[i];
+    if (i == d) h = fmaf(0.5f, q.diag, h);
+    X[1 + i] = fmaf(hs, U[1 + i], h);
   }
 }
 
