@@ -410,6 +410,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   // 1: the flux through the face is cancelled; [cell], then [wave][slot]
   __shared__ unsigned char sCancel[kTileCells + kTileWaves * kHaloCap];
   __shared__ unsigned char sItem[kTileWaves][kHaloCap];  // halo slot -> lane*4 + direction
+  __shared__ int sItemNei[kTileWaves][kHaloCap];         // halo slot -> nei_oct entry of the face
   const int tile = xcd_block(a.nblocks);
   const int lc = (int)threadIdx.x;
   const int wv = lc >> 6, lane = lc & 63;
@@ -452,7 +453,10 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
     nhalo += __popcll(hball[d]);
     if (out) {
       const int slot = hbase[d] + lanes_below(hball[d]);
-      if (slot < kHaloCap) sItem[wv][slot] = (unsigned char)(lane * 4 + d);
+      if (slot < kHaloCap) {
+        sItem[wv][slot] = (unsigned char)(lane * 4 + d);
+        sItemNei[wv][slot] = noD[d];
+      }
     }
   }
   // the list is written and read by lanes of the same wave
@@ -465,7 +469,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
     const int hcell = tile * kTileCells + wv * 64 + (it >> 2);
     const int hic = hcell & 7;
     const int s = (hic >> d) & 1;
-    const int no = a.nei[(hcell >> 3) * 6 + 2 * d + s];
+    const int no = sItemNei[wv][lane];
     const bool coarse = no < 0;
     const int j = coarse ? -1 - no : no * 8 + (hic ^ (1 << d));
     const int hs = kTileCells + wv * kHaloCap + lane;
@@ -482,17 +486,14 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   if (transport) {
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      float4 A[G], B[G];
       if (valid) {
 #pragma unroll
         for (int g = 0; g < G; g++) {
           float X[4];
           lf_share(d, 1.f, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], hc, X);
-          A[g] = make_float4(X[0], X[1], X[2], X[3]);
+          sShare[(g * 2 + 0) * kTileCells + lc] = make_float4(X[0], X[1], X[2], X[3]);
           lf_share(d, -1.f, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], hc, X);
-          B[g] = make_float4(X[0], X[1], X[2], X[3]);
-          sShare[(g * 2 + 0) * kTileCells + lc] = A[g];
-          sShare[(g * 2 + 1) * kTileCells + lc] = B[g];
+          sShare[(g * 2 + 1) * kTileCells + lc] = make_float4(X[0], X[1], X[2], X[3]);
         }
       }
       __syncthreads();
@@ -537,7 +538,9 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
               X = sShare[idx + g * stride];
             else
               X = share_of<G>(a, coarse_face, j, g, d, one);
-            const float4 O = s ? A[g] : B[g];
+            // this cell's own share (A on the face's minus side, B on its plus
+            // side), read back from LDS rather than held in registers
+            const float4 O = sShare[(g * 2 + (s ? 0 : 1)) * kTileCells + lc];
             float F[4] = {X.x + O.x, X.y + O.y, X.z + O.z, X.w + O.w};
             De[g] = fmaf(F[0], w, De[g]);
             Dx[g] = fmaf(F[1], w, Dx[g]);

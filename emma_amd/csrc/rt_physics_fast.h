@@ -73,6 +73,15 @@ __device__ __forceinline__ Rates rates_fast(double Td) {
   return r;
 }
 
+// cooling of rt_physics.h with the common factors taken out:
+// nH'^2 x [(A1 + A3)(1 + yHe)(1 - x) + (A6 + A4 (1 + yHe)^2) x] * 0.1
+__device__ __forceinline__ double cooling_fast(const Rates& r, double x, double nH) {
+#pragma clang fp contract(fast)
+  const double nh2 = nH * 1e-6;
+  const double yp = 1. + kyHe;
+  return (nh2 * nh2 * x) * ((r.A1 + r.A3) * yp * (1. - x) + (r.A6 + r.A4 * (yp * yp)) * x) * 0.1;
+}
+
 // chemrad for one unsplit cell, mixed-precision variant of chem_cell<G>
 template <int G>
 __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fx, double* fy, double* fz,
@@ -105,7 +114,7 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fx,
   const double dt = K.dt;
   const double kT = 1. / (1.5 * kKboltz);
   double tcached = -1.;
-  Rates r = rates_fast(1e4);
+  Rates r = {};  // evaluated on the first pass (tloc != tcached)
   int tries = 0;
   while (tcur < dt) {
     if (++tries > kMaxTries) return -5;
@@ -114,7 +123,7 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fx,
       r = rates_fast(tloc);
       tcached = tloc;
     }
-    const double Cool0 = cooling(r, x0, nH);
+    const double Cool0 = cooling_fast(r, x0, nH);
     if (fudgecool < 1e-20) return -5;
     double ai = 0.;
 #pragma unroll
@@ -148,7 +157,7 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fx,
       fudgecool *= 0.1;
       continue;
     }
-    const double Cool1 = cooling(r, xt, nH);
+    const double Cool1 = cooling_fast(r, xt, nH);
     ai = 0.;
 #pragma unroll
     for (int g = 0; g < GA; g++)
@@ -250,15 +259,21 @@ __device__ __forceinline__ void lf_flux_fast(int d, int s, float Ue, float Ux, f
 __device__ __forceinline__ void lf_share(int d, float side, float Ue, float Ux, float Uy, float Uz,
                                          const Closure<float>& q, float hc, float X[4]) {
 #pragma clang fp contract(fast)
-  const float U[4] = {Ue, Ux, Uy, Uz};
+  // d may differ between lanes: components are picked with 0/1 masks, not
+  // by indexing (a lane-varying index, also one the optimiser rebuilds from
+  // selects, puts the arrays in scratch memory); all terms are finite
   const float hs = side * hc;
-  X[0] = fmaf(hs, Ue, 0.5f * U[1 + d]);
-  const float w = 0.5f * q.arg * q.n[d];
+  const float m0 = d == 0 ? 1.f : 0.f, m1 = d == 1 ? 1.f : 0.f, m2 = d == 2 ? 1.f : 0.f;
+  const float Fd = fmaf(Ux, m0, fmaf(Uy, m1, Uz * m2));
+  const float nd = fmaf(q.n[0], m0, fmaf(q.n[1], m1, q.n[2] * m2));
+  X[0] = fmaf(hs, Ue, 0.5f * Fd);
+  const float w = 0.5f * q.arg * nd;
+  const float Uf[3] = {Ux, Uy, Uz};
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     float h = w * q.n[i];
-    if (i == d) h = fmaf(0.5f, q.diag, h);
-    X[1 + i] = fmaf(hs, U[1 + i], h);
+    h = (i == d) ? fmaf(0.5f, q.diag, h) : h;
+    X[1 + i] = fmaf(hs, Uf[i], h);
   }
 }
 
