@@ -156,37 +156,66 @@ __device__ __forceinline__ int finish_leaf(const LevelArgs<R>& a, const ChemCons
       }
     }
   }
-  double e[GA], fx[GA], fy[GA], fz[GA], src[GA];
+  double e[GA], src[GA];
+  R fx[GA], fy[GA], fz[GA];  // fluxes after transport, in the storage type
 #pragma unroll
   for (int g = 0; g < GA; g++) {
     if (g < ng) {
       e[g] = (double)(R)(Ue[g] + De[g]);
-      fx[g] = (double)(R)(Ux[g] + Dx[g]);
-      fy[g] = (double)(R)(Uy[g] + Dy[g]);
-      fz[g] = (double)(R)(Uz[g] + Dz[g]);
+      fx[g] = Ux[g] + Dx[g];
+      fy[g] = Uy[g] + Dy[g];
+      fz[g] = Uz[g] + Dz[g];
       src[g] = (double)a.src[g * n + cell];
     }
   }
   // rad_utils.c:1611-1650 (the second test reads e[1], i.e. fx[0] when G=1)
-  if (e[0] < 0 || isnan(e[0]) || isnan(ng > 1 ? e[1 < GA ? 1 : 0] : fx[0])) raise_status(a.status, -3, a.level, cell);
+  if (e[0] < 0 || isnan(e[0]) || isnan(ng > 1 ? e[1 < GA ? 1 : 0] : (double)fx[0]))
+    raise_status(a.status, -3, a.level, cell);
   double nhplus = a.nhplus[cell];
   double eint = a.eint[cell];
   const double nh = a.nh[cell];
   double temp = 0.;
   int nit = 0;
   int st;
-  if constexpr (Fast)
-    st = chem_cell_fast<G>(K, ng, e, fx, fy, fz, src, nhplus, eint, nh, temp, a.chemonly, nit);
-  else
-    st = chem_cell<G>(K, ng, e, fx, fy, fz, src, nhplus, eint, nh, temp, a.chemonly, nit);
+  if constexpr (Fast) {
+    double fscale[GA];
+    st = chem_cell_fast<G>(K, ng, e, fscale, src, nhplus, eint, nh, temp, a.chemonly, nit);
+#pragma unroll
+    for (int g = 0; g < GA; g++) {
+      if (g < ng) {
+        fx[g] = (R)((double)fx[g] * fscale[g]);
+        fy[g] = (R)((double)fy[g] * fscale[g]);
+        fz[g] = (R)((double)fz[g] * fscale[g]);
+      }
+    }
+  } else {
+    double dfx[GA], dfy[GA], dfz[GA];
+#pragma unroll
+    for (int g = 0; g < GA; g++) {
+      if (g < ng) {
+        dfx[g] = (double)fx[g];
+        dfy[g] = (double)fy[g];
+        dfz[g] = (double)fz[g];
+      }
+    }
+    st = chem_cell<G>(K, ng, e, dfx, dfy, dfz, src, nhplus, eint, nh, temp, a.chemonly, nit);
+#pragma unroll
+    for (int g = 0; g < GA; g++) {
+      if (g < ng) {
+        fx[g] = (R)dfx[g];
+        fy[g] = (R)dfy[g];
+        fz[g] = (R)dfz[g];
+      }
+    }
+  }
   if (st != 0) raise_status(a.status, st, a.level, cell);
 #pragma unroll
   for (int g = 0; g < GA; g++) {
     if (g < ng) {
       a.eo[g * n + cell] = (R)e[g];
-      a.fxo[g * n + cell] = (R)fx[g];
-      a.fyo[g * n + cell] = (R)fy[g];
-      a.fzo[g * n + cell] = (R)fz[g];
+      a.fxo[g * n + cell] = fx[g];
+      a.fyo[g * n + cell] = fy[g];
+      a.fzo[g * n + cell] = fz[g];
     }
   }
   a.nhpluso[cell] = nhplus;
@@ -221,29 +250,33 @@ __device__ __forceinline__ void transport_split_cell(const LevelArgs<R>& a, int 
 }
 
 // per-block counters: accepted chemistry sub-steps and leaf cell-updates,
-// one atomic pair per block into a spread slot
+// one atomic pair per block into a spread slot.  The wave sums come from
+// ballots (scalar popcounts): the leaf flags directly, the sub-step counts
+// bit plane by bit plane over their low 4 bits; a lane with 16 or more
+// sub-steps adds the rest with its own atomic.
 template <int NW>
-__device__ __forceinline__ void block_counters(unsigned long long* counters, unsigned int v0, unsigned int v1) {
+__device__ __forceinline__ void block_counters(unsigned long long* counters, unsigned int nit, unsigned int leaf) {
   __shared__ unsigned int red[2][NW];
-  for (int o = 32; o > 0; o >>= 1) {
-    v0 += __shfl_xor(v0, o, 64);
-    v1 += __shfl_xor(v1, o, 64);
-  }
+  unsigned long long* slot = counters + (size_t)(blockIdx.x % kCounterSlots) * kCounterStride;
+  unsigned int s0 = 0;
+#pragma unroll
+  for (int b = 0; b < 4; b++) s0 += (unsigned int)__popcll(__ballot((nit >> b) & 1u)) << b;
+  const unsigned int s1 = (unsigned int)__popcll(__ballot(leaf != 0u));
+  if (nit >= 16u) atomicAdd(slot, (unsigned long long)(nit & ~15u));
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
-    red[0][w] = v0;
-    red[1][w] = v1;
+    red[0][w] = s0;
+    red[1][w] = s1;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long s0 = 0, s1 = 0;
+    unsigned long long t0 = 0, t1 = 0;
     for (int i = 0; i < NW; i++) {
-      s0 += red[0][i];
-      s1 += red[1][i];
+      t0 += red[0][i];
+      t1 += red[1][i];
     }
-    unsigned long long* slot = counters + (size_t)(blockIdx.x % kCounterSlots) * kCounterStride;
-    atomicAdd(slot + 0, s0);
-    atomicAdd(slot + 1, s1);
+    atomicAdd(slot + 0, t0);
+    atomicAdd(slot + 1, t1);
   }
 }
 
@@ -393,7 +426,7 @@ __device__ __forceinline__ float4 share_of(const LevelArgs<float>& a, bool coars
   const float ve = Ne[g * nn + j], vx = Nx[g * nn + j], vy = Ny[g * nn + j], vz = Nz[g * nn + j];
   const Closure<float> q = closure_fast(vx, vy, vz, ve, a.c);
   float X[4];
-  lf_share(d, side, ve, vx, vy, vz, q, a.hc, X);
+  lf_share_lane(d, side, ve, vx, vy, vz, q, a.hc, X);
   return make_float4(X[0], X[1], X[2], X[3]);
 }
 
@@ -490,9 +523,9 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
         for (int g = 0; g < G; g++) {
           float X[4];
-          lf_share(d, 1.f, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], hc, X);
+          lf_share_unrolled(d, 1.f, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], hc, X);
           sShare[(g * 2 + 0) * kTileCells + lc] = make_float4(X[0], X[1], X[2], X[3]);
-          lf_share(d, -1.f, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], hc, X);
+          lf_share_unrolled(d, -1.f, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], hc, X);
           sShare[(g * 2 + 1) * kTileCells + lc] = make_float4(X[0], X[1], X[2], X[3]);
         }
       }

@@ -83,22 +83,24 @@ __device__ __forceinline__ double cooling_fast(const Rates& r, double x, double 
 }
 
 // chemrad for one unsplit cell, mixed-precision variant of chem_cell<G>
+// The flux F of each group only gets multiplied by the accepted sub-steps'
+// absorption factors; instead of carrying F through the loop the function
+// returns that product (unit conversions in and out included) in fscale, and
+// the caller forms F * fscale.
 template <int G>
-__device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fx, double* fy, double* fz,
-                              const double* src, double& nhplus, double& eint_code, double nh, double& temp,
-                              int chemonly, int& nit) {
+__device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fscale, const double* src,
+                              double& nhplus, double& eint_code, double nh, double& temp, int chemonly, int& nit) {
 #pragma clang fp contract(fast)
   constexpr int GA = G > 0 ? G : kMaxGrp;
-  double egy[GA], fl0[GA], fl1[GA], fl2[GA], srcl[GA], et[GA], p[GA];
+  double egy[GA], fsc[GA], srcl[GA], et[GA], p[GA];
   const double notchem = (double)(!chemonly);
 #pragma unroll
   for (int g = 0; g < GA; g++) {
     if (g < ng) {
       egy[g] = e[g] * K.k_e;
-      fl0[g] = fx[g] * K.k_f;
-      fl1[g] = fy[g] * K.k_f;
-      fl2[g] = fz[g] * K.k_f;
+      fsc[g] = 1.;
       srcl[g] = src[g] * K.k_src;
+      fscale[g] = 1.;
     }
   }
   const double rnh = frcp64(nh);
@@ -187,12 +189,7 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fx,
     for (int g = 0; g < GA; g++) {
       if (g < ng) {
         egy[g] = et[g] * r3;
-        if (!chemonly) {
-          const double q = frcp64(p[g]) * r4;
-          fl0[g] *= q;
-          fl1[g] *= q;
-          fl2[g] *= q;
-        }
+        if (!chemonly) fsc[g] *= frcp64(p[g]) * r4;
       }
     }
     eint = eintt * r5;
@@ -215,9 +212,7 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fx,
     for (int g = 0; g < GA; g++) {
       if (g < ng) {
         e[g] = fmax(egy[g] * ki_e, K.emin_grp[g]);
-        fx[g] = fl0[g] * ki_f;
-        fy[g] = fl1[g] * ki_f;
-        fz[g] = fl2[g] * ki_f;
+        fscale[g] = fsc[g] * (K.k_f * ki_f);
       }
     }
   }
@@ -251,30 +246,56 @@ __device__ __forceinline__ void lf_flux_fast(int d, int s, float Ue, float Ux, f
   }
 }
 
-// One side's share of the Lax-Friedrichs face flux in direction d.  With
-// h(U) = (F_d, P_d0, P_d1, P_d2) / 2 the physical flux halved (P_di = diag
-// delta_di + arg n_d n_i), the face flux of lf_flux is A(M) + B(P) where
+// One side's share of the Lax-Friedrichs face flux in direction D.  With
+// h(U) = (F_D, P_D0, P_D1, P_D2) / 2 the physical flux halved (P_Di = diag
+// delta_Di + arg n_D n_i), the face flux of lf_flux is A(M) + B(P) where
 // A = h + hc U for the state on the minus side and B = h - hc U for the
 // state on the plus side.  side = +1 gives A, -1 gives B.
-__device__ __forceinline__ void lf_share(int d, float side, float Ue, float Ux, float Uy, float Uz,
-                                         const Closure<float>& q, float hc, float X[4]) {
+template <int D>
+__device__ __forceinline__ void lf_share(float side, float Ue, float Ux, float Uy, float Uz, const Closure<float>& q,
+                                         float hc, float X[4]) {
 #pragma clang fp contract(fast)
-  // d may differ between lanes: components are picked with 0/1 masks, not
-  // by indexing (a lane-varying index, also one the optimiser rebuilds from
-  // selects, puts the arrays in scratch memory); all terms are finite
+  const float Uf[3] = {Ux, Uy, Uz};
+  const float hs = side * hc;
+  X[0] = fmaf(hs, Ue, 0.5f * Uf[D]);
+  const float w = 0.5f * q.arg * q.n[D];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    float h = w * q.n[i];
+    if (i == D) h = fmaf(0.5f, q.diag, h);
+    X[1 + i] = fmaf(hs, Uf[i], h);
+  }
+}
+
+// d uniform and known after unrolling: the branches fold away
+__device__ __forceinline__ void lf_share_unrolled(int d, float side, float Ue, float Ux, float Uy, float Uz,
+                                                  const Closure<float>& q, float hc, float X[4]) {
+  if (d == 0)
+    lf_share<0>(side, Ue, Ux, Uy, Uz, q, hc, X);
+  else if (d == 1)
+    lf_share<1>(side, Ue, Ux, Uy, Uz, q, hc, X);
+  else
+    lf_share<2>(side, Ue, Ux, Uy, Uz, q, hc, X);
+}
+
+// the same with a direction that differs between lanes: components are
+// picked with 0/1 masks, not by indexing (a lane-varying index, also one the
+// optimiser rebuilds from selects, puts the arrays in scratch memory); all
+// terms are finite
+__device__ __forceinline__ void lf_share_lane(int d, float side, float Ue, float Ux, float Uy, float Uz,
+                                              const Closure<float>& q, float hc, float X[4]) {
+#pragma clang fp contract(fast)
   const float hs = side * hc;
   const float m0 = d == 0 ? 1.f : 0.f, m1 = d == 1 ? 1.f : 0.f, m2 = d == 2 ? 1.f : 0.f;
   const float Fd = fmaf(Ux, m0, fmaf(Uy, m1, Uz * m2));
   const float nd = fmaf(q.n[0], m0, fmaf(q.n[1], m1, q.n[2] * m2));
   X[0] = fmaf(hs, Ue, 0.5f * Fd);
   const float w = 0.5f * q.arg * nd;
+  const float hd = 0.5f * q.diag;
   const float Uf[3] = {Ux, Uy, Uz};
+  const float mi[3] = {m0, m1, m2};
 #pragma unroll
-  for (int i = 0; i < 3; i++) {
-    float h = w * q.n[i];
-    h = (i == d) ? fmaf(0.5f, q.diag, h) : h;
-    X[1 + i] = fmaf(hs, Uf[i], h);
-  }
+  for (int i = 0; i < 3; i++) X[1 + i] = fmaf(hs, Uf[i], fmaf(hd, mi[i], w * q.n[i]));
 }
 
 }  // namespace emma
