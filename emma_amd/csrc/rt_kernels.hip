@@ -134,10 +134,12 @@ __device__ __forceinline__ void reflux(const LevelArgs<R>& a, int j, int f, int 
 // leaf cell after its fluxes: base state (+ refluxes from the finer level),
 // update, N<0/NaN guard, chemistry, write of the next state.  Returns the
 // accepted chemistry sub-steps.
+// The chemistry inputs (src, x n_H, eint, n_H) come from memory, or from
+// `pre` when the caller loaded them early (pre_src[GA], pre[3]).
 template <typename R, int G, bool Fast>
 __device__ __forceinline__ int finish_leaf(const LevelArgs<R>& a, const ChemConst& K, int cell, int ch, int ng,
                                            R* Ue, R* Ux, R* Uy, R* Uz, const R* De, const R* Dx, const R* Dy,
-                                           const R* Dz) {
+                                           const R* Dz, const R* pre_src = nullptr, const double* pre = nullptr) {
   constexpr int GA = G > 0 ? G : kMaxGrp;
   const int n = a.ncell;
   if (ch <= -2) {
@@ -165,15 +167,15 @@ __device__ __forceinline__ int finish_leaf(const LevelArgs<R>& a, const ChemCons
       fx[g] = Ux[g] + Dx[g];
       fy[g] = Uy[g] + Dy[g];
       fz[g] = Uz[g] + Dz[g];
-      src[g] = (double)a.src[g * n + cell];
+      src[g] = (double)(pre_src != nullptr ? pre_src[g] : a.src[g * n + cell]);
     }
   }
   // rad_utils.c:1611-1650 (the second test reads e[1], i.e. fx[0] when G=1)
   if (e[0] < 0 || isnan(e[0]) || isnan(ng > 1 ? e[1 < GA ? 1 : 0] : (double)fx[0]))
     raise_status(a.status, -3, a.level, cell);
-  double nhplus = a.nhplus[cell];
-  double eint = a.eint[cell];
-  const double nh = a.nh[cell];
+  double nhplus = pre != nullptr ? pre[0] : a.nhplus[cell];
+  double eint = pre != nullptr ? pre[1] : a.eint[cell];
+  const double nh = pre != nullptr ? pre[2] : a.nh[cell];
   double temp = 0.;
   int nit = 0;
   int st;
@@ -452,34 +454,42 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   const bool valid = cell < a.ncell;
   const int n = a.ncell;
   const float c = a.c, hc = a.hc, dts = a.dtsurdx;
-  float Ue[G], Ux[G], Uy[G], Uz[G];
-  Closure<float> Q[G];
+  // all loads of the cell are issued first: state, child, the oct's face
+  // neighbours and the chemistry inputs (their latency hides behind the
+  // transport)
+  const int oct = cell >> 3, ic = cell & 7;
+  float Ue[G], Ux[G], Uy[G], Uz[G], Src[G];
+  double pre[3] = {0., 0., 1.};  // x n_H, eint, n_H
+  int noD[3] = {0, 0, 0};        // nei_oct entry of the face that leaves the oct, per direction
   int ch = -1;
   if (valid) {
+    // in the order of use: topology (halo list), state, chemistry inputs
     ch = a.child[cell];
+#pragma unroll
+    for (int d = 0; d < 3; d++) noD[d] = a.nei[oct * 6 + 2 * d + ((ic >> d) & 1)];
 #pragma unroll
     for (int g = 0; g < G; g++) {
       Ue[g] = a.e[g * n + cell];
       Ux[g] = a.fx[g * n + cell];
       Uy[g] = a.fy[g * n + cell];
       Uz[g] = a.fz[g * n + cell];
-      Q[g] = closure_fast(Ux[g], Uy[g], Uz[g], Ue[g], c);
     }
+#pragma unroll
+    for (int g = 0; g < G; g++) Src[g] = a.src[g * n + cell];
+    pre[0] = a.nhplus[cell];
+    pre[1] = a.eint[cell];
+    pre[2] = a.nh[cell];
   }
   sCancel[lc] = (unsigned char)(ch >= 0 && !a.keep_split_flux);
 
-  // ---- halo: per direction, the one face of the cell that leaves its oct
-  const int oct = cell >> 3, ic = cell & 7;
+  // ---- halo faces: list them per wave
   const bool transport = a.transport != 0;
   const bool moves = valid && transport && !(ch >= 0 && !a.update_split);
-  int noD[3];                   // nei_oct entry of that face
   unsigned long long hball[3];  // lanes whose face in direction d is a halo face
   int hbase[3];                 // first halo slot of direction d
   int nhalo = 0;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    const int f = 2 * d + ((ic >> d) & 1);
-    noD[d] = moves ? a.nei[oct * 6 + f] : 0;
     const bool out = moves && !(noD[d] >= oct0 && noD[d] < oct0 + kTileOcts);
     hball[d] = __ballot(out);
     hbase[d] = nhalo;
@@ -496,21 +506,58 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (lane < nhalo && lane < kHaloCap) {
-    const int it = sItem[wv][lane];
-    const int d = it & 3;
-    const int hcell = tile * kTileCells + wv * 64 + (it >> 2);
-    const int hic = hcell & 7;
-    const int s = (hic >> d) & 1;
-    const int no = sItemNei[wv][lane];
-    const bool coarse = no < 0;
-    const int j = coarse ? -1 - no : no * 8 + (hic ^ (1 << d));
-    const int hs = kTileCells + wv * kHaloCap + lane;
-    sCancel[hs] = (unsigned char)(!coarse && !a.keep_split_flux && a.child[j] >= 0);
-    // a neighbour on the plus side (s = 1) contributes B, on the minus side A
+  // ---- one halo face per lane: issue the neighbour's loads ...  Straight-line
+  // code (every lane loads, lanes without a halo face from cell 0): a load
+  // under a divergent branch makes the wait before the closures below cover
+  // it as well.
+  const bool hlane = lane < nhalo && lane < kHaloCap;
+  const int it = sItem[wv][lane];         // read by every lane, used by halo lanes
+  const int hno = sItemNei[wv][lane];
+  const int hd = it & 3;
+  const int hic = (it >> 2) & 7;  // the owning cell's position in its oct
+  const int hside = (hic >> hd) & 1;
+  const bool hcoarse = hlane && hno < 0;
+  const int js = (hlane && !hcoarse) ? hno * 8 + (hic ^ (1 << hd)) : 0;
+  float hv[G][4];
 #pragma unroll
-    for (int g = 0; g < G; g++)
-      sShare[kShares + (wv * kHaloCap + lane) * G + g] = share_of<G>(a, coarse, j, g, d, s ? -1.f : 1.f);
+  for (int g = 0; g < G; g++) {
+    hv[g][0] = a.e[g * n + js];
+    hv[g][1] = a.fx[g * n + js];
+    hv[g][2] = a.fy[g * n + js];
+    hv[g][3] = a.fz[g * n + js];
+  }
+  const int hchild = a.child[js];
+  if (__ballot(hcoarse) != 0) {  // injected coarse leaves (levels with a coarser level only)
+    const int jc = hcoarse ? -1 - hno : 0;
+    const int nc = a.ncoarse;
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const float ce = a.ce[g * nc + jc], cx = a.cfx[g * nc + jc], cy = a.cfy[g * nc + jc], cz = a.cfz[g * nc + jc];
+      if (hcoarse) {
+        hv[g][0] = ce;
+        hv[g][1] = cx;
+        hv[g][2] = cy;
+        hv[g][3] = cz;
+      }
+    }
+  }
+  // ---- ... compute this cell's closures meanwhile ...
+  Closure<float> Q[G];
+  if (valid) {
+#pragma unroll
+    for (int g = 0; g < G; g++) Q[g] = closure_fast(Ux[g], Uy[g], Uz[g], Ue[g], c);
+  }
+  // ---- ... then the neighbour's share (a neighbour on the plus side of the
+  // face, side 1, contributes B, on the minus side A) and cancel flag
+  if (hlane) {
+    sCancel[kTileCells + wv * kHaloCap + lane] = (unsigned char)(!hcoarse && !a.keep_split_flux && hchild >= 0);
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const Closure<float> q = closure_fast(hv[g][1], hv[g][2], hv[g][3], hv[g][0], c);
+      float X[4];
+      lf_share_lane(hd, hside ? -1.f : 1.f, hv[g][0], hv[g][1], hv[g][2], hv[g][3], q, hc, X);
+      sShare[kShares + (wv * kHaloCap + lane) * G + g] = make_float4(X[0], X[1], X[2], X[3]);
+    }
   }
 
   float De[G], Dx[G], Dy[G], Dz[G];
@@ -596,7 +643,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
       transport_split_cell<float, G>(a, cell, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz);
     } else {
       leaf = 1;
-      nit_acc = (unsigned)finish_leaf<float, G, true>(a, K, cell, ch, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz);
+      nit_acc = (unsigned)finish_leaf<float, G, true>(a, K, cell, ch, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz, Src, pre);
     }
   }
   if (a.counters != nullptr) block_counters<kTileWaves>(a.counters, nit_acc, leaf);
