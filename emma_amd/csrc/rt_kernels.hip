@@ -185,9 +185,10 @@ __device__ __forceinline__ int finish_leaf(const LevelArgs<R>& a, const ChemCons
 #pragma unroll
     for (int g = 0; g < GA; g++) {
       if (g < ng) {
-        fx[g] = (R)((double)fx[g] * fscale[g]);
-        fy[g] = (R)((double)fy[g] * fscale[g]);
-        fz[g] = (R)((double)fz[g] * fscale[g]);
+        const R fs = (R)fscale[g];
+        fx[g] *= fs;
+        fy[g] *= fs;
+        fz[g] *= fs;
       }
     }
   } else {
@@ -388,9 +389,10 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value
 }
 
 // ------------------------------------------------------------------ tiled level step
-// Mixed precision, G fixed.  A 512-thread block owns 64 consecutive octs
-// (Morton order: a 4x4x4 cube of octs on a complete level), one cell per
-// thread.
+// Mixed precision, G fixed.  A 256-thread block owns 32 consecutive octs
+// (Morton order: a 4x4x2 brick of octs on a complete level), one cell per
+// thread.  With about 38 KB of LDS at G=3, four blocks share a CU, so the
+// barrier phases of one block overlap the memory phases of the others.
 //
 // The Lax-Friedrichs flux through a face normal to d splits into one share
 // per side: F = A(M) + B(P) with A = h + hc U on the minus side and
@@ -405,9 +407,10 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value
 // direction passes: the neighbour's share and whether the face flux is
 // cancelled (split neighbour, RADSTEP) go to LDS slots of that wave, so an
 // in-tile and a halo face differ only in the LDS address read.  A complete
-// level has 48 halo faces per wave (16 per direction); faces beyond kHaloCap
-// (sparse AMR tiles) evaluate their neighbour's share from global memory.
-constexpr int kTileCells = 512;
+// level has 64 halo faces per wave (16 in x and y, 32 in z); faces beyond
+// kHaloCap (sparse AMR tiles) evaluate their neighbour's share from global
+// memory.
+constexpr int kTileCells = 256;
 constexpr int kTileOcts = kTileCells / 8;
 constexpr int kTileWaves = kTileCells / 64;
 constexpr int kHaloCap = 64;
