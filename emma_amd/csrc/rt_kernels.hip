@@ -581,56 +581,58 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
       }
       __syncthreads();
       if (moves) {
+        // Each cell has one face in d inside its oct and one leaving it; bit
+        // says which is which (bit 1: the cell sits on the plus side of its
+        // oct, so its minus face is the inside one).  The face on the cell's
+        // minus side takes the neighbour's A plus the cell's B, the face on
+        // its plus side the cell's A plus the neighbour's B.
+        const int bit = (ic >> d) & 1;
+        const int jin = lc ^ (1 << d);
+        const int iin = (1 - bit) * kTileCells + jin;  // neighbour share, inside face
+        const bool inT = noD[d] >= oct0 && noD[d] < oct0 + kTileOcts;
+        const int jl = (noD[d] - oct0) * 8 + (ic ^ (1 << d));
+        const int slot = hbase[d] + lanes_below(hball[d]);
+        const bool overflow = !inT && slot >= kHaloCap;
+        const bool coarse_face = !inT && noD[d] < 0;
+        int iout, sout, cout;  // neighbour share, stride over groups, cancel flag: leaving face
+        if (inT) {
+          iout = bit * kTileCells + jl;
+          sout = 2 * kTileCells;
+          cout = jl;
+        } else {
+          iout = overflow ? 0 : kShares + (wv * kHaloCap + slot) * G;
+          sout = 1;
+          cout = overflow ? 0 : kTileCells + wv * kHaloCap + slot;
+        }
+        const int j = coarse_face ? -1 - noD[d] : noD[d] * 8 + (ic ^ (1 << d));
+        bool cancel_out = sCancel[cout] != 0;
+        if (overflow) cancel_out = !coarse_face && !a.keep_split_flux && a.child[j] >= 0;
+        const bool cancel_in = sCancel[jin] != 0;
+        const int im = bit ? iin : iout, sm = bit ? 2 * kTileCells : sout;
+        const int ip = bit ? iout : iin, sp = bit ? sout : 2 * kTileCells;
+        const float wm = (bit ? cancel_in : cancel_out) ? 0.f : dts;
+        const float wp = (bit ? cancel_out : cancel_in) ? 0.f : -dts;
 #pragma unroll
-        for (int s = 0; s < 2; s++) {
-          // neighbour on the minus side (s = 0) gives A, on the plus side B;
-          // the face flux is the neighbour's share plus this cell's other one
-          const int nc = ic ^ (1 << d);
-          const float one = s ? -1.f : 1.f;
-          int idx, stride, cidx, slot = 0;
-          bool halo = false;
-          if (((ic >> d) & 1) != s) {
-            idx = s * kTileCells + (lc ^ (1 << d));  // same oct
-            stride = 2 * kTileCells;
-            cidx = lc ^ (1 << d);
-          } else if (noD[d] >= oct0 && noD[d] < oct0 + kTileOcts) {
-            const int jl = (noD[d] - oct0) * 8 + nc;
-            idx = s * kTileCells + jl;
-            stride = 2 * kTileCells;
-            cidx = jl;
-          } else {
-            halo = true;
-            slot = hbase[d] + lanes_below(hball[d]);
-            idx = kShares + (wv * kHaloCap + slot) * G;
-            stride = 1;
-            cidx = kTileCells + wv * kHaloCap + slot;
-          }
-          const bool overflow = halo && slot >= kHaloCap;
-          const bool coarse_face = halo && noD[d] < 0;
-          const int j = coarse_face ? -1 - noD[d] : noD[d] * 8 + nc;
-          float w;
-          if (!overflow) {
-            w = sCancel[cidx] ? 0.f : dts * one;
-          } else {
-            w = (!coarse_face && !a.keep_split_flux && a.child[j] >= 0) ? 0.f : dts * one;
-          }
-#pragma unroll
-          for (int g = 0; g < G; g++) {
-            float4 X;
-            if (!overflow)
-              X = sShare[idx + g * stride];
+        for (int g = 0; g < G; g++) {
+          // this cell's shares are read back rather than held across the barrier
+          const float4 Ag = sShare[(g * 2 + 0) * kTileCells + lc];
+          const float4 Bg = sShare[(g * 2 + 1) * kTileCells + lc];
+          float4 Xm = sShare[im + g * sm];
+          float4 Xp = sShare[ip + g * sp];
+          if (overflow) {
+            const float4 Xo = share_of<G>(a, coarse_face, j, g, d, bit ? -1.f : 1.f);
+            if (bit)
+              Xp = Xo;
             else
-              X = share_of<G>(a, coarse_face, j, g, d, one);
-            // this cell's own share (A on the face's minus side, B on its plus
-            // side), read back from LDS rather than held in registers
-            const float4 O = sShare[(g * 2 + (s ? 0 : 1)) * kTileCells + lc];
-            float F[4] = {X.x + O.x, X.y + O.y, X.z + O.z, X.w + O.w};
-            De[g] = fmaf(F[0], w, De[g]);
-            Dx[g] = fmaf(F[1], w, Dx[g]);
-            Dy[g] = fmaf(F[2], w, Dy[g]);
-            Dz[g] = fmaf(F[3], w, Dz[g]);
-            if (coarse_face) reflux<float, G>(a, j, 2 * d + s, g, F, dts);
+              Xm = Xo;
           }
+          const float Fm[4] = {Xm.x + Bg.x, Xm.y + Bg.y, Xm.z + Bg.z, Xm.w + Bg.w};
+          const float Fp[4] = {Ag.x + Xp.x, Ag.y + Xp.y, Ag.z + Xp.z, Ag.w + Xp.w};
+          De[g] = fmaf(Fp[0], wp, fmaf(Fm[0], wm, De[g]));
+          Dx[g] = fmaf(Fp[1], wp, fmaf(Fm[1], wm, Dx[g]));
+          Dy[g] = fmaf(Fp[2], wp, fmaf(Fm[2], wm, Dy[g]));
+          Dz[g] = fmaf(Fp[3], wp, fmaf(Fm[3], wm, Dz[g]));
+          if (coarse_face) reflux<float, G>(a, j, 2 * d + bit, g, bit ? Fp : Fm, dts);
         }
       }
       if (d < 2) __syncthreads();  // the next direction overwrites the shares
