@@ -395,11 +395,11 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value
 // barrier phases of one block overlap the memory phases of the others.
 //
 // The Lax-Friedrichs flux through a face normal to d splits into one share
-// per side: F = A(M) + B(P) with A = h + hc U on the minus side and
-// B = h - hc U on the plus side, h the halved physical flux (lf_share).  So
-// each cell computes its own A_d and B_d once per direction and a face costs
-// one LDS read of the neighbour's share plus an add.  The block runs the
-// three directions as three passes over one LDS share buffer.
+// per side: 2F = A(M) + B(P) with A = P + c U on the minus side and
+// B = P - c U on the plus side, P the physical flux (lf_share).  So each
+// cell computes its own A_d and B_d once per direction and a face costs one
+// LDS read of the neighbour's share plus an add.  The block runs the three
+// directions as three passes over one LDS share buffer.
 //
 // Neighbours outside the tile (same-level octs of other tiles, injected
 // coarse leaves) are the halo.  Each wave lists its halo faces (lane,
@@ -431,7 +431,7 @@ __device__ __forceinline__ float4 share_of(const LevelArgs<float>& a, bool coars
   const float ve = Ne[g * nn + j], vx = Nx[g * nn + j], vy = Ny[g * nn + j], vz = Nz[g * nn + j];
   const Closure<float> q = closure_fast(vx, vy, vz, ve, a.c);
   float X[4];
-  lf_share_lane(d, side, ve, vx, vy, vz, q, a.hc, X);
+  lf_share_lane(d, side, ve, vx, vy, vz, q, a.c, X);
   return make_float4(X[0], X[1], X[2], X[3]);
 }
 
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   const int oct0 = tile * kTileOcts;
   const bool valid = cell < a.ncell;
   const int n = a.ncell;
-  const float c = a.c, hc = a.hc, dts = a.dtsurdx;
+  const float c = a.c, dts = a.dtsurdx;
   // all loads of the cell are issued first: state, child, the oct's face
   // neighbours and the chemistry inputs (their latency hides behind the
   // transport)
@@ -558,7 +558,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
     for (int g = 0; g < G; g++) {
       const Closure<float> q = closure_fast(hv[g][1], hv[g][2], hv[g][3], hv[g][0], c);
       float X[4];
-      lf_share_lane(hd, hside ? -1.f : 1.f, hv[g][0], hv[g][1], hv[g][2], hv[g][3], q, hc, X);
+      lf_share_lane(hd, hside ? -1.f : 1.f, hv[g][0], hv[g][1], hv[g][2], hv[g][3], q, c, X);
       sShare[kShares + (wv * kHaloCap + lane) * G + g] = make_float4(X[0], X[1], X[2], X[3]);
     }
   }
@@ -573,9 +573,9 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
         for (int g = 0; g < G; g++) {
           float X[4];
-          lf_share_unrolled(d, 1.f, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], hc, X);
+          lf_share_unrolled(d, 1.f, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], c, X);
           sShare[(g * 2 + 0) * kTileCells + lc] = make_float4(X[0], X[1], X[2], X[3]);
-          lf_share_unrolled(d, -1.f, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], hc, X);
+          lf_share_unrolled(d, -1.f, Ue[g], Ux[g], Uy[g], Uz[g], Q[g], c, X);
           sShare[(g * 2 + 1) * kTileCells + lc] = make_float4(X[0], X[1], X[2], X[3]);
         }
       }
@@ -610,8 +610,9 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
         const bool cancel_in = sCancel[jin] != 0;
         const int im = bit ? iin : iout, sm = bit ? 2 * kTileCells : sout;
         const int ip = bit ? iout : iin, sp = bit ? sout : 2 * kTileCells;
-        const float wm = (bit ? cancel_in : cancel_out) ? 0.f : dts;
-        const float wp = (bit ? cancel_out : cancel_in) ? 0.f : -dts;
+        // the shares are doubled: the face fluxes below are 2F
+        const float wm = (bit ? cancel_in : cancel_out) ? 0.f : 0.5f * dts;
+        const float wp = (bit ? cancel_out : cancel_in) ? 0.f : -0.5f * dts;
 #pragma unroll
         for (int g = 0; g < G; g++) {
           // this cell's shares are read back rather than held across the barrier
@@ -632,7 +633,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
           Dx[g] = fmaf(Fp[1], wp, fmaf(Fm[1], wm, Dx[g]));
           Dy[g] = fmaf(Fp[2], wp, fmaf(Fm[2], wm, Dy[g]));
           Dz[g] = fmaf(Fp[3], wp, fmaf(Fm[3], wm, Dz[g]));
-          if (coarse_face) reflux<float, G>(a, j, 2 * d + bit, g, bit ? Fp : Fm, dts);
+          if (coarse_face) reflux<float, G>(a, j, 2 * d + bit, g, bit ? Fp : Fm, 0.5f * dts);
         }
       }
       if (d < 2) __syncthreads();  // the next direction overwrites the shares

@@ -170,29 +170,34 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fsc
       fudgecool *= 0.1;
       continue;
     }
-    if (fabs(fdiv64(eintt, eint) - 1.) > kFracVar) {
+    if (fabs(eintt - eint) > kFracVar * eint) {  // |eintt/eint - 1| > FRAC_VAR with eint > 0
       fudgecool *= 0.1;
       continue;
     }
     eintt = fmax(K.emin, eintt);
-    double r3 = 1., r4 = 1., r5 = 1.;
-    if (!K.static_cosmo) {  // chem_utils.c:737-756
+#pragma unroll
+    for (int g = 0; g < GA; g++) {
+      if (g < ng) {
+        egy[g] = et[g];
+        if (!chemonly) fsc[g] *= frcp64(p[g]);
+      }
+    }
+    eint = eintt;
+    if (!K.static_cosmo) {  // chem_utils.c:737-756: N, F, eint scale by (a_old/a)^3,4,5
       const double aold = aexp;
       const double hubblet = K.H0 * sqrt(K.om / aexp + K.ov * (aexp * aexp)) / aexp * (1e3 / (1e6 * kParsec));
       aexp += hubblet * dtcool * aexp;
       const double ra = aold / aexp;
-      r3 = ra * ra * ra;
-      r4 = r3 * ra;
-      r5 = r4 * ra;
-    }
+      const double r3 = ra * ra * ra, r4 = r3 * ra, r5 = r4 * ra;
 #pragma unroll
-    for (int g = 0; g < GA; g++) {
-      if (g < ng) {
-        egy[g] = et[g] * r3;
-        if (!chemonly) fsc[g] *= frcp64(p[g]) * r4;
+      for (int g = 0; g < GA; g++) {
+        if (g < ng) {
+          egy[g] *= r3;
+          fsc[g] *= r4;
+        }
       }
+      eint *= r5;
     }
-    eint = eintt * r5;
     x0 = xt;
     tcur += dtcool;
     fudgecool = K.fudgecool;
@@ -246,36 +251,35 @@ __device__ __forceinline__ void lf_flux_fast(int d, int s, float Ue, float Ux, f
   }
 }
 
-// One side's share of the Lax-Friedrichs face flux in direction D.  With
-// h(U) = (F_D, P_D0, P_D1, P_D2) / 2 the physical flux halved (P_Di = diag
-// delta_Di + arg n_D n_i), the face flux of lf_flux is A(M) + B(P) where
-// A = h + hc U for the state on the minus side and B = h - hc U for the
-// state on the plus side.  side = +1 gives A, -1 gives B.
+// Twice one side's share of the Lax-Friedrichs face flux in direction D.
+// With P(U) = (F_D, P_D0, P_D1, P_D2) the physical flux (P_Di = diag
+// delta_Di + arg n_D n_i) and c the wave speed, the face flux of lf_flux is
+// (A(M) + B(P)) / 2 where A = P + c U for the state on the minus side and
+// B = P - c U for the state on the plus side.  side = +1 gives A, -1 gives B.
 template <int D>
 __device__ __forceinline__ void lf_share(float side, float Ue, float Ux, float Uy, float Uz, const Closure<float>& q,
-                                         float hc, float X[4]) {
+                                         float c, float X[4]) {
 #pragma clang fp contract(fast)
   const float Uf[3] = {Ux, Uy, Uz};
-  const float hs = side * hc;
-  X[0] = fmaf(hs, Ue, 0.5f * Uf[D]);
-  const float w = 0.5f * q.arg * q.n[D];
+  const float sc = side * c;
+  X[0] = fmaf(sc, Ue, Uf[D]);
+  const float w = q.arg * q.n[D];
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    float h = w * q.n[i];
-    if (i == D) h = fmaf(0.5f, q.diag, h);
-    X[1 + i] = fmaf(hs, Uf[i], h);
+    const float h = (i == D) ? fmaf(w, q.n[i], q.diag) : w * q.n[i];
+    X[1 + i] = fmaf(sc, Uf[i], h);
   }
 }
 
 // d uniform and known after unrolling: the branches fold away
 __device__ __forceinline__ void lf_share_unrolled(int d, float side, float Ue, float Ux, float Uy, float Uz,
-                                                  const Closure<float>& q, float hc, float X[4]) {
+                                                  const Closure<float>& q, float c, float X[4]) {
   if (d == 0)
-    lf_share<0>(side, Ue, Ux, Uy, Uz, q, hc, X);
+    lf_share<0>(side, Ue, Ux, Uy, Uz, q, c, X);
   else if (d == 1)
-    lf_share<1>(side, Ue, Ux, Uy, Uz, q, hc, X);
+    lf_share<1>(side, Ue, Ux, Uy, Uz, q, c, X);
   else
-    lf_share<2>(side, Ue, Ux, Uy, Uz, q, hc, X);
+    lf_share<2>(side, Ue, Ux, Uy, Uz, q, c, X);
 }
 
 // the same with a direction that differs between lanes: components are
@@ -283,19 +287,18 @@ __device__ __forceinline__ void lf_share_unrolled(int d, float side, float Ue, f
 // optimiser rebuilds from selects, puts the arrays in scratch memory); all
 // terms are finite
 __device__ __forceinline__ void lf_share_lane(int d, float side, float Ue, float Ux, float Uy, float Uz,
-                                              const Closure<float>& q, float hc, float X[4]) {
+                                              const Closure<float>& q, float c, float X[4]) {
 #pragma clang fp contract(fast)
-  const float hs = side * hc;
+  const float sc = side * c;
   const float m0 = d == 0 ? 1.f : 0.f, m1 = d == 1 ? 1.f : 0.f, m2 = d == 2 ? 1.f : 0.f;
   const float Fd = fmaf(Ux, m0, fmaf(Uy, m1, Uz * m2));
   const float nd = fmaf(q.n[0], m0, fmaf(q.n[1], m1, q.n[2] * m2));
-  X[0] = fmaf(hs, Ue, 0.5f * Fd);
-  const float w = 0.5f * q.arg * nd;
-  const float hd = 0.5f * q.diag;
+  X[0] = fmaf(sc, Ue, Fd);
+  const float w = q.arg * nd;
   const float Uf[3] = {Ux, Uy, Uz};
   const float mi[3] = {m0, m1, m2};
 #pragma unroll
-  for (int i = 0; i < 3; i++) X[1 + i] = fmaf(hs, Uf[i], fmaf(hd, mi[i], w * q.n[i]));
+  for (int i = 0; i < 3; i++) X[1 + i] = fmaf(sc, Uf[i], fmaf(q.diag, mi[i], w * q.n[i]));
 }
 
 }  // namespace emma
