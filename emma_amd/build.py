@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(LIBDIR, "libemma_rt.so")
-SOURCES = ["rt_kernels.hip", "rt_stencil.hip", "rt_capi.hip"]
+SOURCES = ["rt_kernels.hip", "rt_stencil.hip", "rt_fill.hip", "rt_capi.hip"]
 ARCH = os.environ.get("EMMA_RT_ARCH", "gfx950")
 
 

@@ -50,6 +50,10 @@ struct DevLevel {
   double* t_first = nullptr;
   double* t_last = nullptr;
   std::vector<int> hchild;  // host copy with reflux slots encoded
+  // hydro state (struct Wtype) for the WRADHYD coupling, and the octs' lower
+  // corners for the test source; allocated on first use
+  double* hyd[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // d u v w p dX E
+  double* opos = nullptr;
 };
 
 struct GraphKey {
@@ -86,6 +90,11 @@ struct emma_rt_ctx {
   int timing = 0;
   int xion = 0;
   double tphy = 0.;
+  // FillRad / final-update coupling (WRADHYD) and the Stromgren test source
+  int hydro = 0;
+  int star = 0;
+  double star_x = 0., star_y = 0., star_z = 0., star_mass = 0., star_lifetime = 0., star_srcint = 0.,
+         star_tb0 = 0.;
   std::vector<hipEvent_t> ev;  // pairs
   size_t ev_used = 0;
   std::map<GraphKey, std::pair<hipGraphExec_t, std::vector<int>>> graphs;
@@ -113,6 +122,8 @@ static void free_tree(emma_rt_ctx* c) {
     (void)hipFree(L.racc);
     (void)hipFree(L.t_first);
     (void)hipFree(L.t_last);
+    for (int k = 0; k < 7; k++) (void)hipFree(L.hyd[k]);
+    (void)hipFree(L.opos);
   }
   c->lev.clear();
   for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.first);
@@ -674,6 +685,159 @@ static hipError_t launch_distrib(emma_rt_ctx* c) {
   return launch_distrib_e<float>(tree_view<float>(c), c->stream);
 }
 
+// ---------------------------------------------------------------- FillRad / hydro coupling
+static bool hydro_ready(emma_rt_ctx* c) {
+  for (auto& L : c->lev)
+    if (L.noct > 0 && L.hyd[0] == nullptr) return false;
+  return true;
+}
+
+static bool opos_ready(emma_rt_ctx* c) {
+  for (auto& L : c->lev)
+    if (L.noct > 0 && L.opos == nullptr) return false;
+  return true;
+}
+
+// FillRad's per-cell block on level index k (src_utils.c:738-838); a no-op
+// unless the hydro coupling or the test source is on
+template <typename R>
+static hipError_t launch_fill_cells(emma_rt_ctx* c, int k, double aexp) {
+  DevLevel& L = c->lev[k];
+  if (!(c->hydro || c->star) || L.noct == 0) return hipSuccess;
+  FillArgs<R> f;
+  memset(&f, 0, sizeof(f));
+  f.ncell = L.ncell;
+  f.ng = c->p.ngrp;
+  f.hydro = c->hydro;
+  f.star = c->star;
+  f.child = L.child;
+  f.nh = L.nh;
+  f.eint = L.eint[L.cur];
+  f.nhplus = L.nhplus[L.cur];
+  f.temp = L.temp[L.cur];
+  f.src = (R*)L.src;
+  f.d = L.hyd[0];
+  f.p = L.hyd[4];
+  f.dX = L.hyd[5];
+  f.opos = L.opos;
+  if (c->star) {
+    // stromgren_source (src_utils.c:339-412): the uniform parts on the host,
+    // in the reference's operation order (glibc pow and fmod)
+    const double MYR = 3.1536e13, SOLAR_MASS = 1.989e30;  // param.h:31, 37
+    const int nsub = 1;
+    f.X0 = 1. / (double)(1ULL << c->lmin);
+    f.dxcur = pow(0.5, L.level);
+    f.xs = c->star_x;
+    f.ys = c->star_y;
+    f.zs = c->star_z;
+    const double tcur = c->tphy;
+    const double tcur_in_yrs = tcur * c->p.unit_t / MYR * 1e6;
+    f.star_active = (fmod(tcur_in_yrs, c->star_lifetime) < c->star_lifetime / nsub) &&
+                    (tcur_in_yrs >= c->star_tb0) && (tcur_in_yrs < INFINITY);
+    const double srcint = c->star_mass * SOLAR_MASS * c->star_srcint * (tcur > 0 ? 1. : 0) * nsub / 8;
+    f.star_src = srcint / pow(f.X0 * c->p.unit_l, 3) * c->p.unit_t / c->p.unit_N * pow(aexp, 2);
+  }
+  f.pa3 = pow(aexp, 3);
+  f.pa5 = pow(aexp, 5);
+  f.pstar = c->p.unit_n * c->p.unit_d * pow(c->p.unit_v, 2);
+  f.unit_N = c->p.unit_N;
+  return launch_fill_level<R>(f, c->stream);
+}
+
+static hipError_t fill_cells(emma_rt_ctx* c, int k, double aexp) {
+  if (c->rbytes == 8) return launch_fill_cells<double>(c, k, aexp);
+  return launch_fill_cells<float>(c, k, aexp);
+}
+
+// RadSolver's injection into the hydro state after the level's final update
+static hipError_t to_hydro(emma_rt_ctx* c, int k) {
+  DevLevel& L = c->lev[k];
+  if (!c->hydro || L.noct == 0) return hipSuccess;
+  HydroArgs h;
+  h.ncell = L.ncell;
+  h.eint = L.eint[L.cur];
+  h.nhplus = L.nhplus[L.cur];
+  h.nh = L.nh;
+  h.d = L.hyd[0];
+  h.u = L.hyd[1];
+  h.v = L.hyd[2];
+  h.w = L.hyd[3];
+  h.p = L.hyd[4];
+  h.dX = L.hyd[5];
+  h.E = L.hyd[6];
+  return launch_rad_to_hydro(h, c->stream);
+}
+
+extern "C" int emma_rt_set_hydro(emma_rt_ctx* c, int level, const double* d, const double* u, const double* v,
+                                 const double* w, const double* p, const double* dX) {
+  DevLevel* L = level_of(c, level);
+  if (L == nullptr) return EMMA_RT_ERR_STATE;
+  if (L->ncell == 0) return EMMA_RT_OK;
+  if (!d || !u || !v || !w || !p || !dX) return EMMA_RT_ERR_ARG;
+  const size_t nb = (size_t)L->ncell * sizeof(double);
+  for (int k = 0; k < 7; k++)
+    if (L->hyd[k] == nullptr) HIPCHK(hipMalloc((void**)&L->hyd[k], nb));
+  const double* in[6] = {d, u, v, w, p, dX};
+  for (int k = 0; k < 6; k++) HIPCHK(hipMemcpy(L->hyd[k], in[k], nb, hipMemcpyHostToDevice));
+  HIPCHK(launch_fill((unsigned int*)L->hyd[6], 2 * L->ncell, 0u, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_get_hydro(emma_rt_ctx* c, int level, double* p, double* dX, double* E) {
+  DevLevel* L = level_of(c, level);
+  if (L == nullptr) return EMMA_RT_ERR_STATE;
+  if (L->ncell == 0) return EMMA_RT_OK;
+  if (L->hyd[0] == nullptr) return EMMA_RT_ERR_STATE;
+  const size_t nb = (size_t)L->ncell * sizeof(double);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (p) HIPCHK(hipMemcpy(p, L->hyd[4], nb, hipMemcpyDeviceToHost));
+  if (dX) HIPCHK(hipMemcpy(dX, L->hyd[5], nb, hipMemcpyDeviceToHost));
+  if (E) HIPCHK(hipMemcpy(E, L->hyd[6], nb, hipMemcpyDeviceToHost));
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_set_oct_pos(emma_rt_ctx* c, int level, const double* xyz) {
+  DevLevel* L = level_of(c, level);
+  if (L == nullptr) return EMMA_RT_ERR_STATE;
+  if (L->noct == 0) return EMMA_RT_OK;
+  if (!xyz) return EMMA_RT_ERR_ARG;
+  const size_t nb = (size_t)L->noct * 3 * sizeof(double);
+  if (L->opos == nullptr) HIPCHK(hipMalloc((void**)&L->opos, nb));
+  HIPCHK(hipMemcpy(L->opos, xyz, nb, hipMemcpyHostToDevice));
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_set_stromgren_source(emma_rt_ctx* c, int enable, double x, double y, double z, double mass,
+                                            double lifetime, double srcint, double time_bound0) {
+  if (c == nullptr) return EMMA_RT_ERR_ARG;
+  if (enable && !(lifetime > 0)) return EMMA_RT_ERR_ARG;
+  c->star = enable ? 1 : 0;
+  c->star_x = x;
+  c->star_y = y;
+  c->star_z = z;
+  c->star_mass = mass;
+  c->star_lifetime = lifetime;
+  c->star_srcint = srcint;
+  c->star_tb0 = time_bound0;
+  drop_graphs(c);
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_set_coupling(emma_rt_ctx* c, int hydro) {
+  if (c == nullptr) return EMMA_RT_ERR_ARG;
+  c->hydro = hydro ? 1 : 0;
+  drop_graphs(c);
+  return EMMA_RT_OK;
+}
+
+// every level set up for what the coupling and the source read
+static int coupling_ready(emma_rt_ctx* c) {
+  if (c->hydro && !hydro_ready(c)) return EMMA_RT_ERR_STATE;
+  if (c->star && !opos_ready(c)) return EMMA_RT_ERR_STATE;
+  return EMMA_RT_OK;
+}
+
 extern "C" int emma_rt_set_mode(emma_rt_ctx* c, int mode) {
   if (c == nullptr) return EMMA_RT_ERR_ARG;
   if (mode != EMMA_RT_RADSTEP && mode != EMMA_RT_COARSERAD) return EMMA_RT_ERR_ARG;
@@ -683,11 +847,12 @@ extern "C" int emma_rt_set_mode(emma_rt_ctx* c, int mode) {
 }
 
 extern "C" int emma_rt_fillrad(emma_rt_ctx* c, int level, double aexp) {
-  (void)aexp;
   DevLevel* L = level_of(c, level);
   if (L == nullptr) return EMMA_RT_ERR_STATE;
-  if (c->mode != EMMA_RT_COARSERAD || level != c->lmin || c->lev.size() < 2) return EMMA_RT_OK;
-  HIPCHK(launch_put(c));
+  int st = coupling_ready(c);
+  if (st) return st;
+  if (c->mode == EMMA_RT_COARSERAD && level == c->lmin && c->lev.size() > 1) HIPCHK(launch_put(c));
+  HIPCHK(fill_cells(c, level - c->lmin, aexp));
   HIPCHK(hipStreamSynchronize(c->stream));
   return EMMA_RT_OK;
 }
@@ -695,11 +860,14 @@ extern "C" int emma_rt_fillrad(emma_rt_ctx* c, int level, double aexp) {
 extern "C" int emma_rt_radsolver(emma_rt_ctx* c, int level, double dtnew, double aexp, int chemonly) {
   DevLevel* L = level_of(c, level);
   if (L == nullptr) return EMMA_RT_ERR_STATE;
-  int st = begin_call(c);
+  int st = coupling_ready(c);
+  if (st) return st;
+  st = begin_call(c);
   if (st) return st;
   st = timed_launch(c, level - c->lmin, dtnew, aexp, chemonly);
   if (st) return st;
   if (c->mode == EMMA_RT_COARSERAD && level == c->lmin && c->lev.size() > 1) HIPCHK(launch_distrib(c));
+  HIPCHK(to_hydro(c, level - c->lmin));
   return end_call(c);
 }
 
@@ -713,9 +881,11 @@ static int recurse_coarserad(emma_rt_ctx* c, int k, double aexp) {
     if (st) return st;
   }
   if (k == 0 && nl > 1) HIPCHK(launch_put(c));
+  HIPCHK(fill_cells(c, k, aexp));
   int st = timed_launch(c, k, dtrad_of(c->p, c->lmin, aexp), aexp, 0);
   if (st) return st;
   if (k == 0 && nl > 1) HIPCHK(launch_distrib(c));
+  HIPCHK(to_hydro(c, k));
   return EMMA_RT_OK;
 }
 
@@ -732,8 +902,10 @@ static int recurse(emma_rt_ctx* c, int k, double aexp) {
       int st = recurse(c, k + 1, aexp);
       if (st) return st;
     }
+    HIPCHK(fill_cells(c, k, aexp));  // FillRad before RadSolver (advanceamr.c:1035)
     int st = timed_launch(c, k, dt, aexp, 0);
     if (st) return st;
+    HIPCHK(to_hydro(c, k));
   }
   return EMMA_RT_OK;
 }
@@ -741,7 +913,9 @@ static int recurse(emma_rt_ctx* c, int k, double aexp) {
 extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
   if (c == nullptr || !c->has_tree) return EMMA_RT_ERR_STATE;
   if (ncoarse < 1) return EMMA_RT_ERR_ARG;
-  int st = begin_call(c);
+  int st = coupling_ready(c);
+  if (st) return st;
+  st = begin_call(c);
   if (st) return st;
   if (c->timing) {  // eager launches with per-kernel events
     for (int i = 0; i < ncoarse; i++) {

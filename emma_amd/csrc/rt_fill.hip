@@ -1,0 +1,81 @@
+// rt_fill.hip -- the work either side of a level step that couples the
+// radiation state to the hydro state and places the test source: FillRad's
+// per-cell block (src_utils.c:692-849) and RadSolver's injection back into
+// the hydro fields (rad_utils.c:1941-1951, 1993-2004).  FP64 throughout with
+// the reference's operation order (this file is compiled without
+// contraction like the rest of the library).
+#include <hip/hip_runtime.h>
+
+#include "rt_kernels.h"
+
+namespace emma {
+
+constexpr double kGammaAd = 5. / 3.;  // GAMMA, param.h:4
+
+// E2T (chem_utils.c:21-32); pow(aexp,3), pow(aexp,5) and the pressure unit
+// come from the host
+__device__ __forceinline__ double e2t(double nhplus, double eint, double nh, double pa3, double pa5, double pstar,
+                                      double unit_N) {
+  const double x = nhplus / nh;
+  const double nH = nh / pa3 * unit_N;
+  const double ei = eint / pa5 * pstar;
+  return ei / (1.5 * nH * kKboltz * (1. + x) * (1. + kyHe));
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void fill_level_kernel(FillArgs<R> f) {
+  const int cell = blockIdx.x * 256 + (int)threadIdx.x;
+  if (cell >= f.ncell) return;
+  const int n = f.ncell;
+  double nh = f.nh[cell], eint = f.eint[cell], nhplus = f.nhplus[cell];
+  if (f.hydro) {  // src_utils.c:738-752 (WRADHYD)
+    nh = f.d[cell] * (1. - kYHe);
+    eint = f.p[cell] / (kGammaAd - 1.);
+    nhplus = f.dX[cell];
+    f.nh[cell] = nh;
+    f.eint[cell] = eint;
+    f.nhplus[cell] = nhplus;
+  }
+  if (f.child[cell] >= 0) {  // src_utils.c:765-772: split cells carry no source
+    for (int g = 0; g < f.ng; g++) f.src[g * n + cell] = R(0);
+    return;
+  }
+  if (f.star) {  // putsource -> stromgren_source (src_utils.c:339-412, 446-466)
+    const int oct = cell >> 3, ic = cell & 7;
+    const double xc = f.opos[3 * oct + 0] + (ic & 1) * f.dxcur + f.dxcur * 0.5;
+    const double yc = f.opos[3 * oct + 1] + ((ic >> 1) & 1) * f.dxcur + f.dxcur * 0.5;
+    const double zc = f.opos[3 * oct + 2] + (ic >> 2) * f.dxcur + f.dxcur * 0.5;
+    const double rx = xc - f.xs, ry = yc - f.ys, rz = zc - f.zs;
+    const double Rd = sqrt(rx * rx + ry * ry + rz * rz);
+    const bool on = (Rd <= f.X0) && f.star_active && (xc > 0.) && (yc > 0.) && (zc > 0.);
+    for (int g = 0; g < f.ng; g++) f.src[g * n + cell] = on ? (R)f.star_src : R(0);
+  }
+  if (f.hydro) f.temp[cell] = e2t(nhplus, eint, nh, f.pa3, f.pa5, f.pstar, f.unit_N);  // src_utils.c:835-838
+}
+
+__global__ __launch_bounds__(256) void rad_to_hydro_kernel(HydroArgs h) {
+  const int i = blockIdx.x * 256 + (int)threadIdx.x;
+  if (i >= h.ncell) return;
+  const double d = h.d[i], u = h.u[i], v = h.v[i], w = h.w[i];
+  const double p = (kGammaAd - 1.) * h.eint[i];
+  h.p[i] = p;
+  h.dX[i] = h.nhplus[i] / h.nh[i] * d * (1. - kYHe);
+  h.E[i] = p / (kGammaAd - 1.) + 0.5 * d * (u * u + v * v + w * w);  // getE, hydro_utils.c:29-31
+}
+
+template <typename R>
+hipError_t launch_fill_level(const FillArgs<R>& f, hipStream_t s) {
+  if (f.ncell == 0) return hipSuccess;
+  hipLaunchKernelGGL((fill_level_kernel<R>), dim3((f.ncell + 255) / 256), dim3(256), 0, s, f);
+  return hipGetLastError();
+}
+template hipError_t launch_fill_level<double>(const FillArgs<double>&, hipStream_t);
+template hipError_t launch_fill_level<float>(const FillArgs<float>&, hipStream_t);
+
+hipError_t launch_rad_to_hydro(const HydroArgs& h, hipStream_t s) {
+  if (h.ncell == 0) return hipSuccess;
+  hipLaunchKernelGGL(rad_to_hydro_kernel, dim3((h.ncell + 255) / 256), dim3(256), 0, s, h);
+  return hipGetLastError();
+}
+
+}  // namespace emma

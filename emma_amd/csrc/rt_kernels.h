@@ -101,6 +101,49 @@ hipError_t launch_put_e2coarse(const TreeView<R>& t, hipStream_t s);
 template <typename R>
 hipError_t launch_distrib_e(const TreeView<R>& t, hipStream_t s);
 
+// FillRad's per-cell block (src_utils.c:738-838, init = 0) on one level's
+// current state: with hydro, n_H = d (1-YHE), eint = p/(GAMMA-1), x n_H = dX;
+// split cells src = 0; unsplit cells the Stromgren test source (star) and,
+// with hydro, E2T.  The source term is uniform: the host evaluates its
+// value and the time tests, the kernel only selects the cells.
+template <typename R>
+struct FillArgs {
+  int ncell, ng, hydro, star;
+  const int* child;
+  double* nh;
+  double* eint;
+  double* nhplus;
+  double* temp;
+  R* src;
+  const double* d;
+  const double* p;
+  const double* dX;
+  const double* opos;  // octs' lower corners, 3 per oct
+  double dxcur, X0, xs, ys, zs;
+  int star_active;  // time group and lifetime tests passed (stromgren_source)
+  double star_src;  // source value of every group when active
+  double pa3, pa5, pstar, unit_N;  // E2T
+};
+template <typename R>
+hipError_t launch_fill_level(const FillArgs<R>& f, hipStream_t s);
+
+// RadSolver's injection into the hydro state (rad_utils.c:1941-1951,
+// 1993-2004), every cell: p = (GAMMA-1) eint, dX = x d (1-YHE), E = getE.
+struct HydroArgs {
+  int ncell;
+  const double* eint;
+  const double* nhplus;
+  const double* nh;
+  const double* d;
+  const double* u;
+  const double* v;
+  const double* w;
+  double* p;
+  double* dX;
+  double* E;
+};
+hipError_t launch_rad_to_hydro(const HydroArgs& h, hipStream_t s);
+
 template <typename R>
 hipError_t launch_photon_sum(const R* e, const int* child, int ncell, int ng, double vol, double* out,
                              hipStream_t s);

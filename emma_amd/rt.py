@@ -55,6 +55,8 @@ EXPORTS = [
     "emma_rt_tree_create", "emma_rt_set_level", "emma_rt_get_level", "emma_rt_enable_xion_times",
     "emma_rt_get_xion_times", "emma_rt_set_tphy", "emma_rt_radsolver", "emma_rt_coarse_step",
     "emma_rt_get_stats", "emma_rt_enable_timing", "emma_rt_set_mode", "emma_rt_fillrad",
+    "emma_rt_set_hydro", "emma_rt_get_hydro", "emma_rt_set_coupling", "emma_rt_set_oct_pos",
+    "emma_rt_set_stromgren_source",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -121,6 +123,11 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.emma_rt_enable_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.emma_rt_set_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.emma_rt_fillrad.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+    L.emma_rt_set_hydro.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_dp] * 6
+    L.emma_rt_get_hydro.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_dp] * 3
+    L.emma_rt_set_coupling.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.emma_rt_set_oct_pos.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    L.emma_rt_set_stromgren_source.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_double] * 7
     _lib = L
     return L
 
@@ -234,9 +241,38 @@ class RadGPU:
         self._check(self.lib.emma_rt_set_mode(self.h, int(mode)))
         self.mode = mode
 
+    # ---- coupling to the hydro state and the test source
+    def set_hydro(self, level: int, H):
+        """struct Wtype fields of a level (emma_amd.synth.Hydro)"""
+        a = [np.ascontiguousarray(getattr(H, k), dtype=np.float64) for k in ("d", "u", "v", "w", "p", "dX")]
+        self._check(self.lib.emma_rt_set_hydro(self.h, level, *[_d(x) for x in a]))
+
+    def get_hydro(self, level: int):
+        """(p, dX, E) of a level after the last radsolver / coarse step"""
+        n = self.tree.level(level).ncell
+        p, dX, E = np.empty(n), np.empty(n), np.empty(n)
+        self._check(self.lib.emma_rt_get_hydro(self.h, level, _d(p), _d(dX), _d(E)))
+        return p, dX, E
+
+    def set_coupling(self, hydro: bool = True):
+        self._check(self.lib.emma_rt_set_coupling(self.h, int(hydro)))
+
+    def set_oct_pos(self, level: int, xyz: np.ndarray):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self._check(self.lib.emma_rt_set_oct_pos(self.h, level, _d(xyz)))
+
+    def set_stromgren_source(self, star):
+        """star: emma_amd.synth.Star, or None to switch the source off"""
+        if star is None:
+            self._check(self.lib.emma_rt_set_stromgren_source(self.h, 0, *([0.0] * 7)))
+            return
+        self._check(self.lib.emma_rt_set_stromgren_source(self.h, 1, star.x, star.y, star.z, star.mass,
+                                                          star.lifetime, star.srcint, star.time_bound0))
+
     # ---- the reference's calls
     def fillrad(self, level: int, aexp: float = 1.0):
-        """FillRad's COARSERAD gather of the leaves' E, F into the coarse level"""
+        """FillRad: the COARSERAD gather of the leaves' E, F into the coarse
+        level, then the hydro copy / test source when enabled"""
         self._check(self.lib.emma_rt_fillrad(self.h, level, aexp))
 
     def clean_new(self, level: int):

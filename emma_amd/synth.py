@@ -56,6 +56,58 @@ class Fields:
         return Fields(*[getattr(self, k)[..., idx] for k in self.NAMES])
 
 
+@dataclasses.dataclass
+class Hydro:
+    """One level's hydro state as the radiation step reads and writes it
+    (struct Wtype, float64 SoA [ncell]): density d (= n_H in code units with
+    YHE = 0), velocity u, v, w, pressure p, ionised density dX, total energy E."""
+
+    d: np.ndarray
+    u: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    p: np.ndarray
+    dX: np.ndarray
+    E: np.ndarray
+
+    NAMES = ("d", "u", "v", "w", "p", "dX", "E")
+
+    def copy(self) -> "Hydro":
+        return Hydro(*[np.array(getattr(self, k), dtype=np.float64, copy=True) for k in self.NAMES])
+
+
+GAMMA = 5.0 / 3.0
+
+
+def hydro_like(F: "Fields", rng: np.random.Generator, vmax: float = 0.1) -> Hydro:
+    """A hydro state the radiation state F is consistent with: d = n_H,
+    p = (GAMMA-1) eint, dX = x n_H, random velocities |u_i| < vmax."""
+    n = F.ncell
+    u, v, w = (rng.uniform(-vmax, vmax, n) for _ in range(3))
+    p = (GAMMA - 1.0) * F.eint
+    return Hydro(d=F.nh.copy(), u=u, v=v, w=w, p=p, dX=F.nhplus.copy(), E=np.zeros(n))
+
+
+@dataclasses.dataclass
+class Star:
+    """The WRADTEST point source (stromgren_source, src_utils.c:339-412):
+    position in box units, mass [solar masses], lifetime [yr], srcint
+    [photons/s/kg] and the lower bound of its time group [yr]."""
+
+    x: float = 0.5
+    y: float = 0.5
+    z: float = 0.5
+    mass: float = 2e2
+    lifetime: float = 1e10
+    srcint: float = 5e48 / (2e2 * 1.989e30)
+    time_bound0: float = 0.0
+
+
+def oct_corners(L: T.Level) -> np.ndarray:
+    """Lower corners of a level's octs in box units (struct OCT x, y, z)."""
+    return np.ascontiguousarray(L.pos.astype(np.float64) * 0.5 ** (L.level - 1))
+
+
 def random_fields(params: RunParams, nh: np.ndarray, rng: np.random.Generator, aexp: float = 1.0) -> Fields:
     """x ~ U[1e-6, 1], T log-U[1e2, 1e5] K, N log-U[1e-10, 1] (code units),
     F = |F| u with u uniform on the sphere and |F| ~ U[0, c N]; no sources."""

@@ -165,7 +165,38 @@ int emma_rt_coarse_step(emma_rt_ctx *ctx, double aexp, int ncoarse);
    finer levels.  emma_rt_coarse_step follows Advance_level_RAD
    (advanceamr.c:1309-1537) in this mode: one step of the coarse dt per level. */
 int emma_rt_set_mode(emma_rt_ctx *ctx, int mode);
+/* FillRad (src_utils.c:692-849) on one level: the COARSERAD block above, then
+   the per-cell coupling / source block described below (when enabled) */
 int emma_rt_fillrad(emma_rt_ctx *ctx, int level, double aexp);
+
+/* ---- coupling to the hydro state and the test source (WRADHYD / WRADTEST).
+   FillRad's per-cell block (src_utils.c:738-838) and RadSolver's injection
+   into the hydro fields (rad_utils.c:1941-1951, 1993-2004) on the device.
+   emma_rt_set_hydro stores a level's struct Wtype fields (prototypes.h:611-627;
+   d, u, v, w, p, dX, host arrays of ncell doubles).  With the coupling on
+   (emma_rt_set_coupling(ctx, 1)), emma_rt_fillrad copies n_H = d (1-YHE),
+   eint = p/(GAMMA-1), x n_H = dX into the radiation state of every cell, zeroes
+   the source of split cells and applies E2T to unsplit ones; emma_rt_radsolver
+   and emma_rt_coarse_step then write p = (GAMMA-1) eint, dX = x d (1-YHE) and
+   E = p/(GAMMA-1) + d |u|^2/2 (getE, hydro_utils.c:29-31) for every cell after
+   the level's final update, read back with emma_rt_get_hydro (any pointer may
+   be NULL).  emma_rt_coarse_step calls FillRad before every level step, as
+   Advance_level does (advanceamr.c:1035, 1459).
+   emma_rt_set_stromgren_source enables stromgren_source (src_utils.c:339-412)
+   in FillRad: unsplit cells whose centre lies within 2^-lcoarse of (x, y, z)
+   and in the positive octant get mass*SOLAR_MASS*srcint/8 photons/s spread over
+   (2^-lcoarse unit_l)^3, in code units, while tphy (years) is in the source's
+   time group [time_bound0, inf); the others get 0.  It needs the octs' lower
+   corners (struct OCT x, y, z; 3 doubles per oct) from emma_rt_set_oct_pos.
+   Calls return EMMA_RT_ERR_STATE when a level lacks what the coupling or the
+   source reads. */
+int emma_rt_set_hydro(emma_rt_ctx *ctx, int level, const double *d, const double *u, const double *v,
+                      const double *w, const double *p, const double *dX);
+int emma_rt_get_hydro(emma_rt_ctx *ctx, int level, double *p, double *dX, double *E);
+int emma_rt_set_coupling(emma_rt_ctx *ctx, int hydro);
+int emma_rt_set_oct_pos(emma_rt_ctx *ctx, int level, const double *xyz);
+int emma_rt_set_stromgren_source(emma_rt_ctx *ctx, int enable, double x, double y, double z, double mass,
+                                 double lifetime, double srcint, double time_bound0);
 
 int emma_rt_get_stats(emma_rt_ctx *ctx, emma_rt_stats *out);
 /* record HIP events around every level-step kernel (adds the per-launch

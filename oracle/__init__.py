@@ -28,6 +28,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "_build", "libemma_oracle.so")
 REF_SO = {1: os.path.join(HERE, "_ref", "libemma_ref_g1.so"), 3: os.path.join(HERE, "_ref", "libemma_ref_g3.so")}
 REF_SO_CR = {1: os.path.join(HERE, "_ref", "libemma_ref_g1_cr.so"), 3: os.path.join(HERE, "_ref", "libemma_ref_g3_cr.so")}
+REF_SO_HYD = {1: os.path.join(HERE, "_ref", "libemma_ref_g1_hyd.so"),
+              3: os.path.join(HERE, "_ref", "libemma_ref_g3_hyd.so")}
 
 RADSTEP = 0
 COARSERAD = 1
@@ -46,6 +48,24 @@ def _i(a):
 
 class _OrcFields(ctypes.Structure):
     _fields_ = [("ncell", ctypes.c_int)] + [(k, _dp) for k in Fields.NAMES]
+
+
+class _OrcHydro(ctypes.Structure):
+    _fields_ = [(k, _dp) for k in ("d", "u", "v", "w", "p", "dX", "E")]
+
+
+class _OrcStar(ctypes.Structure):
+    _fields_ = [("on", ctypes.c_int)] + [(k, ctypes.c_double) for k in
+                                        ("x", "y", "z", "mass", "lifetime", "srcint", "time_bound0")]
+
+
+def _chydro(H) -> _OrcHydro:
+    s = _OrcHydro()
+    for k in _OrcHydro._fields_:
+        a = getattr(H, k[0])
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+        setattr(s, k[0], _d(a))
+    return s
 
 
 def _cfields(F: Fields) -> _OrcFields:
@@ -103,9 +123,16 @@ class Oracle:
         L.orc_putE2coarse.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.POINTER(_OrcFields)]
         L.orc_distribE.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_fillrad_cells.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_OrcFields), _ip,
+                                        _dp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.orc_rad_to_hydro.argtypes = [ctypes.POINTER(_OrcFields), ctypes.POINTER(_OrcHydro)]
         self.params = None
         self.tree = None
         self.mode = RADSTEP
+        self.coupling = False
+        self.hydro = {}
+        self.opos = {}
+        self.star = None
 
     # ---- scalar / vector helpers
     def eddington(self, fx, fy, fz, e, c, i, j):
@@ -180,6 +207,10 @@ class Oracle:
         self.t_last = {l: np.full(f.ncell, -1.0) for l, f in fields.items()}
         self.tphy = tphy
         self.nsub = {}
+        self.coupling = False
+        self.hydro = {}
+        self.opos = {}
+        self.star = None
 
     def get(self, level, which=0) -> Fields:
         return (self.new if which else self.old)[level].copy()
@@ -241,16 +272,55 @@ class Oracle:
         fn(self.params.ngrp, t.lmin, len(childs), ctypes.cast(arr_c, ctypes.c_void_p),
            ctypes.cast(arr_f, ctypes.c_void_p), *extra)
 
+    # ---- hydro coupling (WRADHYD) and the WRADTEST source
+    def set_hydro(self, level, H):
+        self.hydro[level] = H.copy()
+
+    def get_hydro(self, level):
+        H = self.hydro[level]
+        return H.p.copy(), H.dX.copy(), H.E.copy()
+
+    def set_coupling(self, hydro=True):
+        self.coupling = bool(hydro)
+
+    def set_oct_pos(self, level, xyz):
+        self.opos[level] = np.ascontiguousarray(xyz, dtype=np.float64)
+
+    def set_stromgren_source(self, star):
+        self.star = star
+
     def fillrad(self, level, aexp=1.0):
-        """FillRad's COARSERAD block: split coarse cells gather their leaves' E, F"""
+        """FillRad: the COARSERAD block (split coarse cells gather their leaves'
+        E, F), then the per-cell hydro copy / test source when enabled"""
         if self.mode == COARSERAD and level == self.tree.lmin:
             nc = _cfields(self.new[level])
             self._levels_call(self.lib.orc_putE2coarse, ctypes.byref(nc))
+        if not (self.coupling or self.star is not None):
+            return
+        L = self.tree.level(level)
+        if L.noct == 0:
+            return
+        ch = np.ascontiguousarray(L.child, dtype=np.int32)
+        H = _chydro(self.hydro[level]) if self.coupling else None
+        st = None
+        if self.star is not None:
+            st = _OrcStar(1, *[float(getattr(self.star, k)) for k in
+                               ("x", "y", "z", "mass", "lifetime", "srcint", "time_bound0")])
+        opos = self.opos.get(level, np.zeros(3 * L.noct))
+        for F in (self.old[level], self.new[level]):  # rfield and rfieldnew
+            cf = _cfields(F)
+            self.lib.orc_fillrad_cells(ctypes.byref(self.cparams), level, self.tree.lmin, ctypes.byref(cf), _i(ch),
+                                       _d(opos), ctypes.byref(H) if H is not None else None,
+                                       ctypes.byref(st) if st is not None else None, self.tphy, aexp)
 
     def radsolver(self, level, dtnew, aexp=1.0, chemonly=0):
         n = self.advancerad(level, dtnew, aexp, chemonly)
         if n > 0:
             self.final(level, chemonly)
+            if self.coupling:
+                cf = _cfields(self.old[level])
+                ch = _chydro(self.hydro[level])
+                self.lib.orc_rad_to_hydro(ctypes.byref(cf), ctypes.byref(ch))
 
     def coarse_step(self, aexp=1.0):
         if self.mode == COARSERAD:
@@ -269,7 +339,7 @@ class OracleError(RuntimeError):
 def coarse_step(solver, tree, params, aexp=1.0):
     """One coarse step of the RADSTEP level recursion (advanceamr.c:243-1300):
     each level cleans rfieldnew, advances the finer level by two half steps,
-    then runs RadSolver with its own dt_rad."""
+    then runs FillRad and RadSolver with its own dt_rad."""
 
     def adv(l):
         dt = params.dt_rad(l, aexp)
@@ -277,6 +347,7 @@ def coarse_step(solver, tree, params, aexp=1.0):
             solver.clean_new(l)
             if l < tree.lmax and tree.level(l + 1).noct > 0:
                 adv(l + 1)
+            solver.fillrad(l, aexp)  # advanceamr.c:1035
             solver.radsolver(l, dt, aexp, 0)
 
     adv(tree.lmin)
@@ -300,16 +371,20 @@ def coarse_step_coarserad(solver, tree, params, aexp=1.0):
     adv(tree.lmin)
 
 
-def ref_available(ngrp: int, coarserad: bool = False) -> bool:
-    return os.path.exists((REF_SO_CR if coarserad else REF_SO).get(ngrp, ""))
+def ref_available(ngrp: int, coarserad: bool = False, hydro: bool = False) -> bool:
+    return os.path.exists((REF_SO_HYD if hydro else REF_SO_CR if coarserad else REF_SO).get(ngrp, ""))
 
 
 class Reference:
-    """The reference's own CPU functions, driven through ref_driver.c."""
+    """The reference's own CPU functions, driven through ref_driver.c.
+    coarserad: the COARSERAD build; hydro: the WRADHYD build (FillRad with the
+    hydro copy and the Stromgren source, RadSolver's injection into the hydro
+    fields)."""
 
-    def __init__(self, ngrp: int, coarserad: bool = False):
-        path = (REF_SO_CR if coarserad else REF_SO)[ngrp]
+    def __init__(self, ngrp: int, coarserad: bool = False, hydro: bool = False):
+        path = (REF_SO_HYD if hydro else REF_SO_CR if coarserad else REF_SO)[ngrp]
         self.mode = COARSERAD if coarserad else RADSTEP
+        self.hydro_build = hydro
         if not os.path.exists(path):
             raise FileNotFoundError(path)
         self.lib = ctypes.CDLL(path)
@@ -340,6 +415,13 @@ class Reference:
         L.refdrv_rates_n.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.refdrv_fillrad_coarse.argtypes = [ctypes.c_int, ctypes.c_double]
         assert L.refdrv_coarserad() == (1 if coarserad else 0)
+        assert L.refdrv_wradhyd() == (1 if hydro else 0)
+        if hydro:
+            L.refdrv_set_hydro.argtypes = [ctypes.c_int] + [_dp] * 6
+            L.refdrv_get_hydro.argtypes = [ctypes.c_int] + [_dp] * 3
+            L.refdrv_set_oct_pos.argtypes = [ctypes.c_int, _dp]
+            L.refdrv_set_stromgren.argtypes = [ctypes.c_double] * 7
+            L.refdrv_fillrad.argtypes = [ctypes.c_int, ctypes.c_double]
         self.stride = 4096
 
     def layout(self):
@@ -400,8 +482,31 @@ class Reference:
             coarse_step(self, self.tree, self.params, aexp)
 
     def fillrad(self, level, aexp=1.0):
-        if self.mode == COARSERAD:
+        if self.hydro_build:
+            self.lib.refdrv_fillrad(level, aexp)
+        elif self.mode == COARSERAD:
             self.lib.refdrv_fillrad_coarse(level, aexp)
+
+    # ---- WRADHYD build only
+    def set_hydro(self, level, H):
+        a = [np.ascontiguousarray(getattr(H, k), dtype=np.float64) for k in ("d", "u", "v", "w", "p", "dX")]
+        self.lib.refdrv_set_hydro(level, *[_d(x) for x in a])
+
+    def get_hydro(self, level):
+        n = self.tree.level(level).ncell
+        p, dX, E = np.empty(n), np.empty(n), np.empty(n)
+        self.lib.refdrv_get_hydro(level, _d(p), _d(dX), _d(E))
+        return p, dX, E
+
+    def set_coupling(self, hydro=True):
+        assert hydro == self.hydro_build, "the coupling is a compile-time flag of the reference (WRADHYD)"
+
+    def set_oct_pos(self, level, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self.lib.refdrv_set_oct_pos(level, _d(xyz))
+
+    def set_stromgren_source(self, star):
+        self.lib.refdrv_set_stromgren(star.x, star.y, star.z, star.mass, star.lifetime, star.srcint, star.time_bound0)
 
     def eddington(self, fx, fy, fz, e, c, i, j):
         fx, fy, fz, e = [np.ascontiguousarray(a, dtype=np.float64) for a in (fx, fy, fz, e)]

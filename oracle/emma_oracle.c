@@ -675,3 +675,76 @@ void orc_distribE(int G, int lcoarse, int nlev, const int *const *child, orc_fie
       if (nlev > 1) distrib_rec(G, lcoarse, lcoarse + 1, nlev, child[0][i] * 8 + c, child, old, C, i);
   }
 }
+
+/* ---------------------------------------------------------------- FillRad
+   src_utils.c:692-849 with init = 0 (the COARSERAD block is orc_putE2coarse).
+   Constants: param.h:31 (MYR), 37 (SOLAR_MASS). */
+#define O_MYR (3.1536e13)
+#define O_SOLAR_MASS (1.989e30)
+
+/* stromgren_source (src_utils.c:339-412) for one cell; src already cleaned
+   to 0 by putsource (src_utils.c:450-455) */
+static void orc_stromgren(const orc_params *P, const orc_star *S, int level, int lcoarse, const double *octpos,
+                          int icell, double tcur, double aexp, double *src, int ncell, int cell) {
+  const double X0 = 1. / (double)(1ULL << lcoarse);
+  const double tcur_in_yrs = tcur * P->unit_t / O_MYR * 1e6;
+  const double dxcur = pow(0.5, level);
+  const double xc = octpos[0] + (icell & 1) * dxcur + dxcur * 0.5;
+  const double yc = octpos[1] + ((icell >> 1) & 1) * dxcur + dxcur * 0.5;
+  const double zc = octpos[2] + ((icell >> 2)) * dxcur + dxcur * 0.5;
+  const int nsub = 1;
+  const double R = sqrt(pow(xc - S->x, 2) + pow(yc - S->y, 2) + pow(zc - S->z, 2));
+  if ((R <= X0) && (fmod(tcur_in_yrs, S->lifetime) < S->lifetime / nsub)) {
+    if ((xc > 0.) && (yc > 0.) && (zc > 0.)) {
+      /* one time group: [time_bound0, INFINITY) (parameters.c:459-468) */
+      if ((tcur_in_yrs >= S->time_bound0) && (tcur_in_yrs < INFINITY)) {
+        for (int g = 0; g < P->ngrp; g++) {
+          const double srcint = S->mass * O_SOLAR_MASS * S->srcint * (tcur > 0 ? 1. : 0) * nsub / 8;
+          src[g * ncell + cell] = srcint / pow(X0 * P->unit_l, 3) * P->unit_t / P->unit_N * pow(aexp, 2);
+        }
+      }
+    }
+  } else {
+    for (int g = 0; g < P->ngrp; g++) src[g * ncell + cell] = 0.;
+  }
+}
+
+int orc_fillrad_cells(const orc_params *P, int level, int lcoarse, orc_fields *F, const int *child,
+                      const double *opos, const orc_hydro *H, const orc_star *star, double tcur, double aexp) {
+  const int n = F->ncell;
+  for (int cell = 0; cell < n; cell++) {
+    if (H != NULL) {
+      F->nh[cell] = H->d[cell] * (1. - O_YHE);
+      F->eint[cell] = H->p[cell] / (O_GAMMA - 1.);
+      F->nhplus[cell] = H->dX[cell];
+    }
+    if (child[cell] >= 0) {
+      for (int g = 0; g < P->ngrp; g++) F->src[g * n + cell] = 0.;
+      continue;
+    }
+    if (star != NULL && star->on) {
+      for (int g = 0; g < P->ngrp; g++) F->src[g * n + cell] = 0.;
+      orc_stromgren(P, star, level, lcoarse, opos + 3 * (cell / 8), cell % 8, tcur, aexp, F->src, n, cell);
+    }
+    if (H != NULL) {
+      orc_cell R;
+      R.nhplus = F->nhplus[cell];
+      R.eint = F->eint[cell];
+      R.nh = F->nh[cell];
+      orc_e2t(&R, aexp, P);
+      F->temp[cell] = R.temp;
+    }
+  }
+  return ORC_OK;
+}
+
+/* rad_utils.c:1941-1951 (unsplit), 1993-2004 (split, after the restriction) */
+void orc_rad_to_hydro(const orc_fields *F, orc_hydro *H) {
+  for (int i = 0; i < F->ncell; i++) {
+    H->p[i] = (O_GAMMA - 1.) * F->eint[i];
+    H->dX[i] = F->nhplus[i] / F->nh[i] * H->d[i] * (1. - O_YHE);
+    /* getE, hydro_utils.c:29-31 */
+    H->E[i] = H->p[i] / (O_GAMMA - 1.) + 0.5 * H->d[i] * (H->u[i] * H->u[i] + H->v[i] * H->v[i] + H->w[i] * H->w[i]);
+  }
+}
+

@@ -109,6 +109,37 @@ void orc_level_final(int ngrp, int noct, const int *child, orc_fields *old, cons
 int orc_level_fluxes(int ngrp, int noct, const int *nei_oct, const int *child, const orc_fields *old,
                      const orc_fields *cold, double c, double *rflux);
 
+/* ---- coupling to the hydro state (WRADHYD) and the WRADTEST source ---- */
+
+/* struct Wtype (prototypes.h:611-627), SoA over a level's cells */
+typedef struct orc_hydro {
+  double *d, *u, *v, *w, *p, *dX, *E; /* [ncell] */
+} orc_hydro;
+
+/* stromgren_source parameters (src_utils.c:339-412); on = 0 leaves src alone */
+typedef struct orc_star {
+  int on;
+  double x, y, z;      /* unitary_stars_test->src_pos_* */
+  double mass;         /* unitary_stars_test->mass [solar masses] */
+  double lifetime;     /* unitary_stars_test->lifetime [yr] */
+  double srcint;       /* RUNPARAMS.srcint [photons/s/kg] */
+  double time_bound0;  /* atomic.time_bound[0] [yr] (one time group) */
+} orc_star;
+
+/* FillRad (src_utils.c:692-849) per cell of one level, init = 0, without the
+   COARSERAD block (orc_putE2coarse): with hydro, n_H, eint and x n_H from the
+   hydro state (738-762); split cells get src = 0 (765-772); unsplit cells the
+   test source (putsource -> stromgren_source, 446-466) when star->on, then
+   E2T with hydro (835-838).  child[cell] >= 0 marks split cells; opos holds
+   the octs' lower corners (3 per oct).  tcur = cosmo->tphy. */
+int orc_fillrad_cells(const orc_params *P, int level, int lcoarse, orc_fields *F, const int *child,
+                      const double *opos, const orc_hydro *H, const orc_star *star, double tcur, double aexp);
+
+/* RadSolver's injection into the hydro state (rad_utils.c:1941-1951,
+   1993-2004), every cell of the level after the final update:
+   p = (GAMMA-1) eint, dX = x d (1-YHE), E = getE (hydro_utils.c:29-31). */
+void orc_rad_to_hydro(const orc_fields *F, orc_hydro *H);
+
 #ifdef __cplusplus
 }
 #endif

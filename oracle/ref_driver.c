@@ -41,6 +41,10 @@ void cuCompCooling(REAL temp, REAL x, REAL nH, REAL *lambda, REAL *tcool, REAL a
 #ifdef COARSERAD
 int putE2coarse(struct CELL *cell, struct RUNPARAMS *param, int level, REAL aexp, REAL tcur);
 #endif
+#ifdef WRADHYD
+int FillRad(int level, struct RUNPARAMS *param, struct OCT **firstoct, struct CPUINFO *cpu, int init, REAL aexp,
+            REAL tloc);
+#endif
 
 #define MAXLEV 24
 
@@ -362,3 +366,77 @@ int refdrv_fillrad_coarse(int level, double aexp) {
   return -1;
 #endif
 }
+
+#ifdef WRADHYD
+/* ---- hydro coupling (WRADHYD) and the WRADTEST Strömgren source ---- */
+int refdrv_wradhyd(void) { return 1; }
+
+static REAL g_time_bound[2];
+static struct UNITARY_STARS_TEST g_star;
+
+/* per-cell hydro state of a level (struct Wtype, prototypes.h:611-627) */
+int refdrv_set_hydro(int level, const double *d, const double *u, const double *v, const double *w, const double *p,
+                     const double *dX) {
+  if (level < g_lmin || level > g_lmax) return -1;
+  int n = g_noct[level] * 8;
+  for (int i = 0; i < n; i++) {
+    struct Wtype *W = &cell_at(level, i)->field;
+    W->d = d[i];
+    W->u = u[i];
+    W->v = v[i];
+    W->w = w[i];
+    W->p = p[i];
+    W->dX = dX[i];
+    W->E = 0.;
+  }
+  return 0;
+}
+
+int refdrv_get_hydro(int level, double *p, double *dX, double *E) {
+  if (level < g_lmin || level > g_lmax) return -1;
+  int n = g_noct[level] * 8;
+  for (int i = 0; i < n; i++) {
+    const struct Wtype *W = &cell_at(level, i)->field;
+    p[i] = W->p;
+    dX[i] = W->dX;
+    E[i] = W->E;
+  }
+  return 0;
+}
+
+/* oct lower corners (struct OCT x, y, z), 3 per oct */
+int refdrv_set_oct_pos(int level, const double *xyz) {
+  if (level < g_lmin || level > g_lmax) return -1;
+  for (int i = 0; i < g_noct[level]; i++) {
+    g_lev[level][i].x = xyz[3 * i + 0];
+    g_lev[level][i].y = xyz[3 * i + 1];
+    g_lev[level][i].z = xyz[3 * i + 2];
+  }
+  return 0;
+}
+
+/* stromgren_source parameters (src_utils.c:339-412): source position, star
+   mass [solar masses], lifetime [yr], srcint [photons/s/kg] and the one time
+   group's lower bound [yr] (readAtomic, parameters.c:459-468) */
+void refdrv_set_stromgren(double x, double y, double z, double mass, double lifetime, double srcint,
+                          double time_bound0) {
+  g_star.src_pos_x = x;
+  g_star.src_pos_y = y;
+  g_star.src_pos_z = z;
+  g_star.mass = mass;
+  g_star.lifetime = lifetime;
+  g_param.unitary_stars_test = &g_star;
+  g_param.srcint = srcint;
+  g_time_bound[0] = time_bound0;
+  g_time_bound[1] = INFINITY;
+  g_param.atomic.time_bound = g_time_bound;
+}
+
+/* FillRad (src_utils.c:692-849) on one level, init = 0, tcur = cosmo->tphy */
+int refdrv_fillrad(int level, double aexp) {
+  g_param.lcoarse = g_lmin;
+  return FillRad(level, &g_param, g_firstoct, &g_cpu, 0, aexp, 0.);
+}
+#else
+int refdrv_wradhyd(void) { return 0; }
+#endif

@@ -1,0 +1,124 @@
+"""FillRad's per-cell block (hydro -> radiation copy, the Strömgren test
+source, E2T) and RadSolver's injection back into the hydro state
+(src_utils.c:692-849, rad_utils.c:1941-1951 and 1993-2004): the oracle
+against the reference's WRADHYD build run live (exact agreement), and against
+closed forms."""
+import numpy as np
+import pytest
+
+from emma_amd import synth
+from emma_amd.synth import Fields
+from oracle import Oracle, Reference, ref_available
+
+TABLES = {1: "SB99_equiv_mono_TH", 3: "Starburst99_1grpT_3grpS"}
+GAMMA = 5.0 / 3.0
+
+
+def coupled(solver, W, star, tphy, seed=7):
+    rng = np.random.default_rng(seed)
+    hyd = {l: synth.hydro_like(F, rng) for l, F in W.fields.items()}
+    solver.setup(W.params, W.tree, W.fields, tphy)
+    solver.set_coupling(True)
+    solver.set_stromgren_source(star)
+    for l, H in hyd.items():
+        solver.set_hydro(l, H)
+        solver.set_oct_pos(l, synth.oct_corners(W.tree.level(l)))
+    return hyd
+
+
+def amr(G, seed=3):
+    return synth.amr_workload(3, [40, 80, 50], TABLES[G], seed=seed, sigma=1.5, grid=16)
+
+
+@pytest.mark.parametrize("G", [1, 3])
+def test_coupled_coarse_steps_vs_reference(G):
+    if not ref_available(G, hydro=True):
+        pytest.skip("reference WRADHYD build (oracle/_ref) not present")
+    W = amr(G)
+    star = synth.Star(x=0.5, y=0.5, z=0.5)
+    o, r = Oracle(), Reference(G, hydro=True)
+    for s in (o, r):
+        coupled(s, W, star, tphy=1e-3)
+    for _ in range(2):
+        o.coarse_step()
+        r.coarse_step()
+    for l in range(W.tree.lmin, W.tree.lmax + 1):
+        for which in (0, 1):
+            A, B = o.get(l, which), r.get(l, which)
+            for k in Fields.NAMES:
+                assert np.array_equal(getattr(A, k), getattr(B, k)), (l, which, k)
+        for a, b, name in zip(o.get_hydro(l), r.get_hydro(l), ("p", "dX", "E")):
+            assert np.array_equal(a, b), (l, name)
+    # the source is on: some unsplit cells around the centre carry it
+    assert any(np.any(o.get(l).src > 0) for l in range(W.tree.lmin, W.tree.lmax + 1))
+
+
+@pytest.mark.parametrize("G", [1, 3])
+def test_fillrad_vs_reference_inactive_and_offcentre_sources(G):
+    """tcur = 0 switches the source off (src_utils.c:382); an off-centre source
+    in the positive octant lights a different set of cells."""
+    if not ref_available(G, hydro=True):
+        pytest.skip("reference WRADHYD build (oracle/_ref) not present")
+    W = amr(G, seed=5)
+    for star, tphy in ((synth.Star(), 0.0), (synth.Star(x=0.3, y=0.55, z=0.45), 2e-3)):
+        o, r = Oracle(), Reference(G, hydro=True)
+        for s in (o, r):
+            coupled(s, W, star, tphy)
+        for l in range(W.tree.lmin, W.tree.lmax + 1):
+            o.fillrad(l)
+            r.fillrad(l)
+            for which in (0, 1):
+                A, B = o.get(l, which), r.get(l, which)
+                for k in Fields.NAMES:
+                    assert np.array_equal(getattr(A, k), getattr(B, k)), (l, which, k, tphy)
+
+
+def test_fillrad_closed_forms():
+    """n_H = d, eint = p/(GAMMA-1), x n_H = dX, split cells without source,
+    the source exactly in the unsplit cells within 2^-lcoarse of it."""
+    W = amr(3)
+    star = synth.Star(x=0.5, y=0.5, z=0.5, mass=100.0, srcint=1e16)
+    o = Oracle()
+    hyd = coupled(o, W, star, tphy=1e-3)
+    P = W.params
+    for l in range(W.tree.lmin, W.tree.lmax + 1):
+        L = W.tree.level(l)
+        o.fillrad(l)
+        F, H = o.get(l), hyd[l]
+        assert np.array_equal(F.nh, H.d)
+        assert np.array_equal(F.eint, H.p / (GAMMA - 1.0))
+        assert np.array_equal(F.nhplus, H.dX)
+        dx = 0.5**l
+        centres = (L.cell_pos() + 0.5) * dx
+        R = np.sqrt(((centres - 0.5) ** 2).sum(1))
+        inside = (R <= 0.5**W.tree.lmin) & (L.child < 0)
+        X0 = 0.5**W.tree.lmin
+        val = star.mass * 1.989e30 * star.srcint / 8 / (X0 * P.unit_l) ** 3 * P.unit_t / P.unit_N
+        for g in range(P.ngrp):
+            assert np.all(F.src[g][L.child >= 0] == 0)
+            np.testing.assert_allclose(F.src[g][inside], val, rtol=1e-14)
+            assert np.all(F.src[g][~inside] == 0)
+        unsplit = L.child < 0
+        t_ref = o.e2t(P, F.nhplus[unsplit], F.eint[unsplit], F.nh[unsplit])
+        assert np.array_equal(F.temp[unsplit], t_ref)
+
+
+def test_injection_closed_forms_and_round_trip():
+    """After RadSolver: p = (GAMMA-1) eint, dX = x d, E = p/(GAMMA-1) + d|u|^2/2;
+    FillRad of that hydro state gives back eint and x n_H to rounding."""
+    W = synth.uniform_workload(3, TABLES[1], seed=9)
+    o = Oracle()
+    coupled(o, W, synth.Star(lifetime=1e10), tphy=0.0)
+    o.fillrad(3)
+    o.clean_new(3)
+    o.radsolver(3, W.params.dt_rad(3))
+    F = o.get(3)
+    p, dX, E = o.get_hydro(3)
+    H = o.hydro[3]
+    assert np.array_equal(p, (GAMMA - 1.0) * F.eint)
+    np.testing.assert_allclose(dX, F.nhplus / F.nh * H.d, rtol=1e-15)
+    np.testing.assert_allclose(E, p / (GAMMA - 1.0) + 0.5 * H.d * (H.u**2 + H.v**2 + H.w**2), rtol=1e-15)
+    o.fillrad(3)
+    G = o.get(3)
+    np.testing.assert_allclose(G.eint, F.eint, rtol=4e-16)
+    np.testing.assert_allclose(G.nhplus, F.nhplus, rtol=4e-16)

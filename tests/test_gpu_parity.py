@@ -171,3 +171,45 @@ def test_coarserad(gpu_lib, oracle, table, G, mode, precision):
         print("coarserad G=%d prec=%d level %d" % (G, precision, l), err)
         assert err["x"] < tol and err["temp"] < tol and err["e"] < tol and err["eint"] < tol, (l, err)
         assert max(err["fx"], err["fy"], err["fz"]) < tol, (l, err)
+
+
+def coupled_pair(oracle, W, precision, star, tphy, mode, steps=2, seed=7):
+    rng = np.random.default_rng(seed)
+    hyd = {l: synth.hydro_like(F, rng) for l, F in W.fields.items()}
+    g = rt.RadGPU(W.params, precision)
+    g.setup(W.params, W.tree, W.fields, tphy)
+    oracle.setup(W.params, W.tree, W.fields, tphy)
+    for s in (g, oracle):
+        s.set_coupling(True)
+        s.set_stromgren_source(star)
+        for l, H in hyd.items():
+            s.set_hydro(l, H)
+            s.set_oct_pos(l, synth.oct_corners(W.tree.level(l)))
+    from oracle import coarse_step
+
+    for _ in range(steps):
+        if mode == "radsolver":
+            coarse_step(g, W.tree, W.params)  # FillRad + RadSolver per level step from the host
+        else:
+            g.coarse_step(1.0, 1)
+        oracle.coarse_step()
+    return g
+
+
+@pytest.mark.parametrize("table,G", [("SB99_equiv_mono_TH", 1), ("Starburst99_1grpT_3grpS", 3)])
+@pytest.mark.parametrize("mode", ["radsolver", "graph"])
+@pytest.mark.parametrize("precision", [rt.FP64, rt.MIXED])
+def test_hydro_coupling_and_source(gpu_lib, oracle, table, G, mode, precision):
+    """FillRad (hydro -> radiation, Stromgren source, E2T) before every level
+    step and the injection into p, dX, E after it, on a 4-level tree."""
+    W = synth.amr_workload(3, [60, 100, 80], table, seed=13, sigma=1.5, grid=16)
+    g = coupled_pair(oracle, W, precision, synth.Star(x=0.5, y=0.5, z=0.5), 1e-3, mode)
+    tol = FP64_TOL if precision == rt.FP64 else MIXED_TOL
+    c = W.params.cloc()
+    for l in range(W.tree.lmin, W.tree.lmax + 1):
+        err = field_errors(g.get(l), oracle.get(l), c)
+        herr = [max_rel(a, b) for a, b in zip(g.get_hydro(l), oracle.get_hydro(l))]
+        print("coupled G=%d %s level %d" % (G, mode, l), err, herr)
+        assert max(err.values()) < tol, (l, err)
+        assert max(herr) < tol, (l, herr)
+
