@@ -16,8 +16,10 @@ complete device work).  roofline.achieved = algorithmic bytes of the level-step
 launches / their summed device time (HIP events around every launch on the
 library's stream; for c4 from an event-timed eager pass of one coarse step right
 after the graph-timed region).  fp64_plain: the same workload stepped by the FP64
-kernel.  cpu_baseline: the oracle (the CPU restatement of the reference path,
-one core) on a bounded sample drawn like the workload.
+kernel.  cpu_baseline: the reference's CPU path (oracle/_ref, compiled from the
+reference's own sources) or, where that build is absent, the oracle (its CPU
+restatement), on up to 16 host cores, one process each, on a bounded sample
+drawn like the workload.
 
 --calibrate-copy runs a plain device copy of known size (for scaling the
 FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc runs).
@@ -60,36 +62,67 @@ WORKLOAD_NAMES = {
 }
 
 
-def cpu_baseline(name: str, seconds: float = 10.0):
-    """Oracle (scalar C restatement, 1 core) on a level-5 sample (32^3 cells)
-    drawn like the workload; restarted from the drawn state every 4 sub-cycles."""
+def _cpu_worker(args):
+    """One core's share of the CPU baseline: RadSolver sub-cycles on a level-5
+    sample (32^3 cells) drawn like the workload, restarted from the drawn state
+    every 4 sub-cycles (without sources the sample drains to the EMIN floor,
+    where the reference aborts, chem_utils.c:382).  The reference's own CPU
+    functions when their build is present, else the oracle."""
+    name, seconds, seed, use_ref = args
     from emma_amd import synth
-    from oracle import Oracle
+    from oracle import Oracle, Reference
 
     if name in ("c3", "c4"):
-        W = synth.uniform_workload(5, "Starburst99_1grpT_3grpS", seed=3, nsrc=1)
+        W = synth.uniform_workload(5, "Starburst99_1grpT_3grpS", seed=seed, nsrc=1)
     else:
-        W = synth.uniform_workload(5, "SB99_equiv_mono_TH", seed=2)
-    o = Oracle()
+        W = synth.uniform_workload(5, "SB99_equiv_mono_TH", seed=seed)
+    s = Reference(W.params.ngrp) if use_ref else Oracle()
     dt = W.params.dt_rad(5)
-    n = 0
-    el = 0.0
-    # without sources the sample drains to the EMIN floor, where the reference aborts (chem_utils.c:382)
+    n, el = 0, 0.0
     while el < seconds:
-        o.setup(W.params, W.tree, W.fields)
+        s.setup(W.params, W.tree, W.fields)
         t0 = time.perf_counter()
         for _ in range(4):
-            o.radsolver(5, dt)
+            s.clean_new(5)
+            s.radsolver(5, dt)
         el += time.perf_counter() - t0
         n += 4
-    cells = W.tree.level(5).ncell
+    return W.tree.level(5).ncell * n, el
+
+
+def cpu_baseline(name: str, seconds: float = 10.0, cores: int = 0):
+    """The reference CPU path (oracle/_ref, compiled from the reference's own
+    sources) or, where that build is absent, the oracle, on `cores` host
+    cores at once (one process each, spawned)."""
+    import multiprocessing as mp
+
+    from oracle import ref_available
+
+    G = 1 if name == "c2" else 3
+    use_ref = ref_available(G)
+    if cores <= 0:
+        cores = max(1, min(16, os.cpu_count() or 1))
+    ctx = mp.get_context("spawn")
+    try:
+        with ctx.Pool(cores) as pool:
+            res = pool.map(_cpu_worker, [(name, seconds, 3 + k, use_ref) for k in range(cores)])
+    except Exception as exc:  # the reference build could not run here: time the oracle
+        if not use_ref:
+            raise
+        print(f"cpu_baseline: reference build failed ({exc}); timing the oracle", file=sys.stderr)
+        use_ref = False
+        with ctx.Pool(cores) as pool:
+            res = pool.map(_cpu_worker, [(name, seconds, 3 + k, use_ref) for k in range(cores)])
+    rate = sum(c / e for c, e in res)
+    nsub = sum(c for c, _ in res) // 32768
     return {
-        "value": cells * n / el,
+        "value": rate,
         "unit": "cell-updates/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"oracle RadSolver on a 32^3 periodic level (level 5, {cells} cells) drawn like the workload, "
-                  f"{n} sub-cycles in {el:.1f} s",
+        "cores": cores,
+        "kind": "reference" if use_ref else "port",
+        "sample": f"{'reference' if use_ref else 'oracle'} RadSolver (clean_new_rad + RadSolver) on a 32^3 periodic "
+                  f"level (level 5, 32768 cells) drawn like the workload, one process per core, {nsub} sub-cycles "
+                  f"in {seconds:.0f} s per core",
     }
 
 
@@ -193,6 +226,7 @@ def main():
     ap.add_argument("--precision", default=None, choices=["mixed", "fp64"])
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-cores", type=int, default=0, help="host cores for the CPU baseline (0: up to 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-plain", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -257,7 +291,7 @@ def main():
         v2 = st2["cell_updates"] / w2
         out["fp64_plain"] = {"value": v2, "unit": "cell-updates/s", "steps": nsteps, "speedup_of_value": value / v2}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds, args.cpu_cores)
     print(json.dumps(out), flush=True)
 
 
