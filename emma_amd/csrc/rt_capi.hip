@@ -35,6 +35,7 @@ namespace {
 
 struct DevLevel {
   int level = 0, noct = 0, ncell = 0, nslot = 0, cur = 0;
+  int halo_rounds = 1;  // rounds of 64 halo slots per wave the tiled kernel needs
   void* e[2] = {nullptr, nullptr};
   void* fx[2] = {nullptr, nullptr};
   void* fy[2] = {nullptr, nullptr};
@@ -339,6 +340,22 @@ extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int
         if (C.hchild[j] == -1) C.hchild[j] = -2 - (C.nslot++);
       }
     }
+    // faces leaving a cell's oct whose neighbour lies outside the cell's tile,
+    // counted per wave of 64 cells: complete levels have 64, sparse ones up
+    // to 192 (every cell's three leaving faces)
+    int maxh = 0;
+    for (int w0 = 0; w0 < L.ncell; w0 += 64) {
+      int cnt = 0;
+      for (int cell = w0; cell < std::min(L.ncell, w0 + 64); cell++) {
+        const int oct = cell >> 3, ic = cell & 7, oct0 = (cell / kTileCells) * kTileOcts;
+        for (int d = 0; d < 3; d++) {
+          const int no = nei[oct * 6 + 2 * d + ((ic >> d) & 1)];
+          if (!(no >= oct0 && no < oct0 + kTileOcts)) cnt++;
+        }
+      }
+      maxh = std::max(maxh, cnt);
+    }
+    L.halo_rounds = maxh > 64 ? 3 : 1;
   }
   // device allocation and upload
   for (int k = 0; k < nl; k++) {
@@ -525,6 +542,7 @@ static LevelArgs<R> level_args(emma_rt_ctx* c, int k, double dtnew, double aexp,
     A.keep_split_flux = 0;
     A.update_split = 0;
   }
+  A.halo_rounds = L.halo_rounds;
   A.e = (const R*)L.e[a];
   A.fx = (const R*)L.fx[a];
   A.fy = (const R*)L.fy[a];

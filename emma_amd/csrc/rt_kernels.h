@@ -17,7 +17,10 @@ struct LevelArgs {
   // (rad_utils.c:518-526,1548-1552,1808-1812): transport only on the coarse
   // level and not for chemonly calls, fluxes of split neighbours kept, split
   // cells transport-updated (their chemistry still skipped).
-  int transport, keep_split_flux, update_split, pad2;
+  int transport, keep_split_flux, update_split;
+  // mixed path: rounds of 64 halo slots per wave the level's tiles need (1
+  // on complete levels, 3 on sparse ones; see level_step_tiled)
+  int halo_rounds;
   // current state (read)
   const R* e;
   const R* fx;
@@ -75,6 +78,12 @@ struct LevelArgs {
 // address; the host sums the slots.
 constexpr int kCounterSlots = 256;
 constexpr int kCounterStride = 16;  // unsigned long longs per slot (128 B)
+
+// tiles of the mixed-precision level step: 256 cells = 32 Morton-consecutive
+// octs per work-group of four waves
+constexpr int kTileCells = 256;
+constexpr int kTileOcts = kTileCells / 8;
+constexpr int kTileWaves = kTileCells / 64;
 
 template <typename R>
 hipError_t launch_level_step(const LevelArgs<R>& a, const ChemConst& K, hipStream_t s);

@@ -407,13 +407,10 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value
 // direction passes: the neighbour's share and whether the face flux is
 // cancelled (split neighbour, RADSTEP) go to LDS slots of that wave, so an
 // in-tile and a halo face differ only in the LDS address read.  A complete
-// level has 64 halo faces per wave (16 in x and y, 32 in z); faces beyond
-// kHaloCap (sparse AMR tiles) evaluate their neighbour's share from global
-// memory.
-constexpr int kTileCells = 256;
-constexpr int kTileOcts = kTileCells / 8;
-constexpr int kTileWaves = kTileCells / 64;
-constexpr int kHaloCap = 64;
+// level has 64 halo faces per wave (16 in x and y, 32 in z), one per lane
+// (HR = 1).  Sparse AMR levels, where a wave has up to 192 halo faces (every
+// cell's three leaving faces), use HR = 3 rounds of 64; faces beyond the
+// 64*HR slots evaluate their neighbour's share from global memory.
 
 __device__ __forceinline__ int lanes_below(unsigned long long m) {
   return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -435,9 +432,10 @@ __device__ __forceinline__ float4 share_of(const LevelArgs<float>& a, bool coars
   return make_float4(X[0], X[1], X[2], X[3]);
 }
 
-template <int G>
+template <int G, int HR>
 __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4))) void level_step_tiled(
     LevelArgs<float> a_by_value, ChemConst K_by_value) {
+  constexpr int kHaloCap = 64 * HR;  // halo slots per wave
   const KernArgs<float> args = kernargs<float>();
   const LevelArgs<float>& a = args.a;
   const ChemConst& K = args.K;
@@ -509,58 +507,72 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // ---- one halo face per lane: issue the neighbour's loads ...  Straight-line
-  // code (every lane loads, lanes without a halo face from cell 0): a load
-  // under a divergent branch makes the wait before the closures below cover
-  // it as well.
-  const bool hlane = lane < nhalo && lane < kHaloCap;
-  const int it = sItem[wv][lane];         // read by every lane, used by halo lanes
-  const int hno = sItemNei[wv][lane];
-  const int hd = it & 3;
-  const int hic = (it >> 2) & 7;  // the owning cell's position in its oct
-  const int hside = (hic >> hd) & 1;
-  const bool hcoarse = hlane && hno < 0;
-  const int js = (hlane && !hcoarse) ? hno * 8 + (hic ^ (1 << hd)) : 0;
+  // ---- one halo face per lane and round: issue the neighbour's loads ...
+  // Straight-line code (every lane loads, lanes without a halo face from
+  // cell 0): a load under a divergent branch makes the wait before the
+  // closures below cover it as well.
+  bool hlane = false, hcoarse = false;
+  int hd = 0, hside = 0, hchild = -1;
   float hv[G][4];
-#pragma unroll
-  for (int g = 0; g < G; g++) {
-    hv[g][0] = a.e[g * n + js];
-    hv[g][1] = a.fx[g * n + js];
-    hv[g][2] = a.fy[g * n + js];
-    hv[g][3] = a.fz[g * n + js];
-  }
-  const int hchild = a.child[js];
-  if (__ballot(hcoarse) != 0) {  // injected coarse leaves (levels with a coarser level only)
-    const int jc = hcoarse ? -1 - hno : 0;
-    const int nc = a.ncoarse;
+  auto load_item = [&](int slot) {
+    hlane = slot < nhalo && slot < kHaloCap;
+    const int it = sItem[wv][slot < kHaloCap ? slot : 0];  // read by every lane, used by halo lanes
+    const int hno = sItemNei[wv][slot < kHaloCap ? slot : 0];
+    hd = it & 3;
+    const int hic = (it >> 2) & 7;  // the owning cell's position in its oct
+    hside = (hic >> hd) & 1;
+    hcoarse = hlane && hno < 0;
+    const int js = (hlane && !hcoarse) ? hno * 8 + (hic ^ (1 << hd)) : 0;
 #pragma unroll
     for (int g = 0; g < G; g++) {
-      const float ce = a.ce[g * nc + jc], cx = a.cfx[g * nc + jc], cy = a.cfy[g * nc + jc], cz = a.cfz[g * nc + jc];
-      if (hcoarse) {
-        hv[g][0] = ce;
-        hv[g][1] = cx;
-        hv[g][2] = cy;
-        hv[g][3] = cz;
+      hv[g][0] = a.e[g * n + js];
+      hv[g][1] = a.fx[g * n + js];
+      hv[g][2] = a.fy[g * n + js];
+      hv[g][3] = a.fz[g * n + js];
+    }
+    hchild = a.child[js];
+    if (__ballot(hcoarse) != 0) {  // injected coarse leaves (levels with a coarser level only)
+      const int jc = hcoarse ? -1 - hno : 0;
+      const int nc = a.ncoarse;
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        const float ce = a.ce[g * nc + jc], cx = a.cfx[g * nc + jc], cy = a.cfy[g * nc + jc],
+                    cz = a.cfz[g * nc + jc];
+        if (hcoarse) {
+          hv[g][0] = ce;
+          hv[g][1] = cx;
+          hv[g][2] = cy;
+          hv[g][3] = cz;
+        }
       }
     }
-  }
+  };
+  // ... the neighbour's share (a neighbour on the plus side of the face,
+  // side 1, contributes B, on the minus side A) and cancel flag
+  auto store_item = [&](int slot) {
+    if (hlane) {
+      sCancel[kTileCells + wv * kHaloCap + slot] = (unsigned char)(!hcoarse && !a.keep_split_flux && hchild >= 0);
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        const Closure<float> q = closure_fast(hv[g][1], hv[g][2], hv[g][3], hv[g][0], c);
+        float X[4];
+        lf_share_lane(hd, hside ? -1.f : 1.f, hv[g][0], hv[g][1], hv[g][2], hv[g][3], q, c, X);
+        sShare[kShares + (wv * kHaloCap + slot) * G + g] = make_float4(X[0], X[1], X[2], X[3]);
+      }
+    }
+  };
+  load_item(lane);
   // ---- ... compute this cell's closures meanwhile ...
   Closure<float> Q[G];
   if (valid) {
 #pragma unroll
     for (int g = 0; g < G; g++) Q[g] = closure_fast(Ux[g], Uy[g], Uz[g], Ue[g], c);
   }
-  // ---- ... then the neighbour's share (a neighbour on the plus side of the
-  // face, side 1, contributes B, on the minus side A) and cancel flag
-  if (hlane) {
-    sCancel[kTileCells + wv * kHaloCap + lane] = (unsigned char)(!hcoarse && !a.keep_split_flux && hchild >= 0);
+  store_item(lane);
 #pragma unroll
-    for (int g = 0; g < G; g++) {
-      const Closure<float> q = closure_fast(hv[g][1], hv[g][2], hv[g][3], hv[g][0], c);
-      float X[4];
-      lf_share_lane(hd, hside ? -1.f : 1.f, hv[g][0], hv[g][1], hv[g][2], hv[g][3], q, c, X);
-      sShare[kShares + (wv * kHaloCap + lane) * G + g] = make_float4(X[0], X[1], X[2], X[3]);
-    }
+  for (int r = 1; r < HR; r++) {
+    load_item(lane + 64 * r);
+    store_item(lane + 64 * r);
   }
 
   float De[G], Dx[G], Dy[G], Dz[G];
@@ -580,13 +592,17 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
         }
       }
       __syncthreads();
+      float Rf[G][4];  // flux through a coarse face (refluxing), 0 elsewhere
+      int rj = -1;     // the coarse leaf on that face
+#pragma unroll
+      for (int g = 0; g < G; g++) Rf[g][0] = Rf[g][1] = Rf[g][2] = Rf[g][3] = 0.f;
+      const int bit = (ic >> d) & 1;
       if (moves) {
         // Each cell has one face in d inside its oct and one leaving it; bit
         // says which is which (bit 1: the cell sits on the plus side of its
         // oct, so its minus face is the inside one).  The face on the cell's
         // minus side takes the neighbour's A plus the cell's B, the face on
         // its plus side the cell's A plus the neighbour's B.
-        const int bit = (ic >> d) & 1;
         const int jin = lc ^ (1 << d);
         const int iin = (1 - bit) * kTileCells + jin;  // neighbour share, inside face
         const bool inT = noD[d] >= oct0 && noD[d] < oct0 + kTileOcts;
@@ -633,7 +649,43 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
           Dx[g] = fmaf(Fp[1], wp, fmaf(Fm[1], wm, Dx[g]));
           Dy[g] = fmaf(Fp[2], wp, fmaf(Fm[2], wm, Dy[g]));
           Dz[g] = fmaf(Fp[3], wp, fmaf(Fm[3], wm, Dz[g]));
-          if (coarse_face) reflux<float, G>(a, j, 2 * d + bit, g, bit ? Fp : Fm, 0.5f * dts);
+          if (coarse_face) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) Rf[g][v] = bit ? Fp[v] : Fm[v];
+          }
+        }
+        if (coarse_face) rj = j;
+      }
+      // Refluxing (scatterstencilrad, rad_utils.c:1738-1762): the four cells
+      // of an oct face share its coarse neighbour, so their fluxes are summed
+      // across lanes first and one lane per face adds them to the coarse
+      // cell's accumulator (a quarter of the atomics, none contending)
+      const unsigned long long rmask = __ballot(rj >= 0);
+      if (rmask != 0ull) {
+        const int b1 = 1 << ((d + 1) % 3), b2 = 1 << ((d + 2) % 3);
+#pragma unroll
+        for (int g = 0; g < G; g++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            float x = Rf[g][v];
+            x += __shfl_xor(x, b1, 64);
+            x += __shfl_xor(x, b2, 64);
+            Rf[g][v] = x;
+          }
+        const int l0 = lane & ~(b1 | b2);
+        const unsigned long long grp =
+            (1ull << l0) | (1ull << (l0 | b1)) | (1ull << (l0 | b2)) | (1ull << (l0 | b1 | b2));
+        const unsigned long long mine = rmask & grp;
+        if (rj >= 0 && lane == __builtin_ctzll(mine)) {
+          const int cch = a.cchild[rj];
+          if (cch <= -2) {
+            float* acc = a.cracc + (size_t)(-2 - cch) * 4 * G;
+            const float w = (bit ? 1.f : -1.f) * (0.5f * dts) * 0.125f;  // pow(-1, face+1) dt/dx / 8, shares doubled
+#pragma unroll
+            for (int g = 0; g < G; g++)
+#pragma unroll
+              for (int v = 0; v < 4; v++) atomicAdd(acc + v * G + g, Rf[g][v] * w);
+          }
         }
       }
       if (d < 2) __syncthreads();  // the next direction overwrites the shares
@@ -777,10 +829,19 @@ hipError_t launch_level_step(const LevelArgs<R>& a0, const ChemConst& K, hipStre
       int nb = (a.ncell + kTileCells - 1) / kTileCells;
       nb = (nb + 7) & ~7;
       a.nblocks = nb;
-      if (a.ng == 1)
-        hipLaunchKernelGGL((level_step_tiled<1>), dim3(nb), dim3(kTileCells), 0, s, a, K);
-      else
-        hipLaunchKernelGGL((level_step_tiled<3>), dim3(nb), dim3(kTileCells), 0, s, a, K);
+      // sparse levels: three rounds of halo slots per wave
+      const bool sparse = a.halo_rounds > 1;
+      if (a.ng == 1) {
+        if (sparse)
+          hipLaunchKernelGGL((level_step_tiled<1, 3>), dim3(nb), dim3(kTileCells), 0, s, a, K);
+        else
+          hipLaunchKernelGGL((level_step_tiled<1, 1>), dim3(nb), dim3(kTileCells), 0, s, a, K);
+      } else {
+        if (sparse)
+          hipLaunchKernelGGL((level_step_tiled<3, 3>), dim3(nb), dim3(kTileCells), 0, s, a, K);
+        else
+          hipLaunchKernelGGL((level_step_tiled<3, 1>), dim3(nb), dim3(kTileCells), 0, s, a, K);
+      }
       return hipGetLastError();
     }
   }
