@@ -22,6 +22,13 @@ __device__ __forceinline__ double frcp64(double b) {
   return r;
 }
 
+// one Newton step: for quantities that set a sub-step length or feed the FP32
+// rate fits, and for absorption factors kept to FP32 accuracy downstream
+__device__ __forceinline__ double frcp64_1(double b) {
+  const double r = __builtin_amdgcn_rcp(b);
+  return fma(fma(-b, r, 1.0), r, r);
+}
+
 __device__ __forceinline__ double fdiv64(double a, double b) {
   const double r = frcp64(b);
   const double q = a * r;
@@ -73,16 +80,9 @@ __device__ __forceinline__ Rates rates_fast(double Td) {
   return r;
 }
 
-// cooling of rt_physics.h with the common factors taken out:
-// nH'^2 x [(A1 + A3)(1 + yHe)(1 - x) + (A6 + A4 (1 + yHe)^2) x] * 0.1
-__device__ __forceinline__ double cooling_fast(const Rates& r, double x, double nH) {
-#pragma clang fp contract(fast)
-  const double nh2 = nH * 1e-6;
-  const double yp = 1. + kyHe;
-  return (nh2 * nh2 * x) * ((r.A1 + r.A3) * yp * (1. - x) + (r.A6 + r.A4 * (yp * yp)) * x) * 0.1;
-}
-
-// chemrad for one unsplit cell, mixed-precision variant of chem_cell<G>
+// chemrad for one unsplit cell, mixed-precision variant of chem_cell<G>.
+// Cooling (cooling of rt_physics.h) is evaluated with its common factors
+// taken out: nH'^2 x [(A1 + A3)(1 + yHe)(1 - x) + (A6 + A4 (1 + yHe)^2) x] / 10.
 // The flux F of each group only gets multiplied by the accepted sub-steps'
 // absorption factors; instead of carrying F through the loop the function
 // returns that product (unit conversions in and out included) in fscale, and
@@ -92,14 +92,14 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fsc
                               double& nhplus, double& eint_code, double nh, double& temp, int chemonly, int& nit) {
 #pragma clang fp contract(fast)
   constexpr int GA = G > 0 ? G : kMaxGrp;
-  double egy[GA], fsc[GA], srcl[GA], et[GA], p[GA];
+  double egy[GA], fsc[GA], sf[GA], et[GA], rp[GA];
   const double notchem = (double)(!chemonly);
 #pragma unroll
   for (int g = 0; g < GA; g++) {
     if (g < ng) {
       egy[g] = e[g] * K.k_e;
       fsc[g] = 1.;
-      srcl[g] = src[g] * K.k_src;
+      sf[g] = src[g] * K.k_src * K.factgrp[g];  // source rate of the group
       fscale[g] = 1.;
     }
   }
@@ -107,6 +107,7 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fsc
   double x0 = nhplus * rnh;
   const double nH = nh * K.k_e;
   const double rnH = frcp64(nH);
+  const double n22 = (nH * 1e-6) * (nH * 1e-6) * 0.1;
   double eint = eint_code * K.k_eint;
   nit = 0;
   if (egy[0] < kEmin) return -4;
@@ -115,38 +116,41 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fsc
   double tcur = 0.;
   const double dt = K.dt;
   const double kT = 1. / (1.5 * kKboltz);
+  const double yp = 1. + kyHe;
   double tcached = -1.;
   Rates r = {};  // evaluated on the first pass (tloc != tcached)
+  double cs = 0., ct = 0.;  // cooling(x) = x (cs (1-x) + ct x) at the rates' T
   int tries = 0;
   while (tcur < dt) {
     if (++tries > kMaxTries) return -5;
-    const double tloc = eint * kT * frcp64(1. + x0) * rnH;
+    const double tloc = eint * kT * frcp64_1(1. + x0) * rnH;
     if (!(tloc == tcached)) {
       r = rates_fast(tloc);
       tcached = tloc;
+      cs = (r.A1 + r.A3) * yp * n22;
+      ct = (r.A6 + r.A4 * (yp * yp)) * n22;
     }
-    const double Cool0 = cooling_fast(r, x0, nH);
     if (fudgecool < 1e-20) return -5;
+    const double omx0 = 1. - x0;
+    const double Cool0 = x0 * fma(ct, x0, cs * omx0);
     double ai = 0.;
 #pragma unroll
     for (int g = 0; g < GA; g++)
       if (g < ng) ai += K.heat[g] * egy[g];
-    ai *= nH * (1.0 - x0);
-    const double tcool = fabs(fdiv64(eint, ai * notchem - Cool0));
+    ai *= nH * omx0;
+    const double tcool = fabs(eint * frcp64_1(ai * notchem - Cool0));
     const double dtcool = fmin(fudgecool * tcool, dt - tcur);
-    int test = 0;
+    // the trial sub-step; every rejection test of the reference leads to the
+    // same retry with fudgecool / 10, so the tests are combined
+    const double c1 = omx0 * nH * dtcool;
+    bool bad = false;
 #pragma unroll
     for (int g = 0; g < GA; g++) {
       if (g < ng) {
-        const double aig = K.alphai[g] * (1. - x0) * nH * dtcool;
-        p[g] = 1. + aig;
-        et[g] = chemonly ? egy[g] : (egy[g] + srcl[g] * dtcool * K.factgrp[g]) * frcp64(p[g]);
-        if ((et[g] < 0) || isnan(et[g])) test = 1;
+        rp[g] = frcp64_1(1. + K.alphai[g] * c1);
+        et[g] = chemonly ? egy[g] : fma(sf[g], dtcool, egy[g]) * rp[g];
+        bad = bad || (et[g] < 0) || isnan(et[g]);
       }
-    }
-    if (test) {
-      fudgecool *= 0.1;
-      continue;
     }
     ai = 0.;
 #pragma unroll
@@ -154,23 +158,18 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fsc
       if (g < ng) ai += K.alphai[g] * et[g];
     ai *= notchem;
     const double RECT = r.alphab * x0 * x0 * nH * dtcool;
-    const double xt = 1. - fdiv64(RECT + (1.0 - x0), 1. + dtcool * (r.beta * x0 * nH + ai));
-    if (((xt > 1.) || (xt < 0.)) || isnan(xt)) {
-      fudgecool *= 0.1;
-      continue;
-    }
-    const double Cool1 = cooling_fast(r, xt, nH);
+    const double xt = 1. - fdiv64(RECT + omx0, 1. + dtcool * (r.beta * x0 * nH + ai));
+    bad = bad || (xt > 1.) || (xt < 0.) || isnan(xt);
+    const double omxt = 1. - xt;
+    const double Cool1 = xt * fma(ct, xt, cs * omxt);
     ai = 0.;
 #pragma unroll
     for (int g = 0; g < GA; g++)
       if (g < ng) ai += K.heat[g] * et[g];
-    ai *= nH * (1.0 - xt);
+    ai *= nH * omxt;
     double eintt = eint + dtcool * (ai * notchem - Cool1);
-    if (eintt < 0.) {
-      fudgecool *= 0.1;
-      continue;
-    }
-    if (fabs(eintt - eint) > kFracVar * eint) {  // |eintt/eint - 1| > FRAC_VAR with eint > 0
+    bad = bad || (eintt < 0.) || (fabs(eintt - eint) > kFracVar * eint);  // |eintt/eint - 1| > FRAC_VAR
+    if (bad) {
       fudgecool *= 0.1;
       continue;
     }
@@ -179,7 +178,7 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fsc
     for (int g = 0; g < GA; g++) {
       if (g < ng) {
         egy[g] = et[g];
-        if (!chemonly) fsc[g] *= frcp64(p[g]);
+        if (!chemonly) fsc[g] *= rp[g];
       }
     }
     eint = eintt;
