@@ -244,6 +244,36 @@ def config_c3(seed: int = 3, nsub: int = 32) -> Workload:
     return uniform_workload(7, "Starburst99_1grpT_3grpS", seed, nsrc=64, nsub=nsub, name="C3-128^3-G3-mixed")
 
 
+def powerlaw_luminosities(rng, n: int, lo: float, hi: float) -> np.ndarray:
+    """n luminosities with dN/dL ~ L^-2 on [lo, hi] (inverse transform)."""
+    u = rng.random(n)
+    return 1.0 / (1.0 / lo - u * (1.0 / lo - 1.0 / hi))
+
+
+def config_c5(seed: int = 5, nsub: int = 100) -> Workload:
+    """Reionisation-like box: 256^3 coarse cells (level 8, 16.8 M cells) and two
+    refined levels with 300000 / 75000 octs (3 M cells) on a lognormal sigma=2
+    density, 4096 point sources with dN/dL ~ L^-2 on [1e49, 1e53] s^-1 in the
+    densest leaves of the coarse level, 3 groups, c~ = c/100."""
+    rng = np.random.default_rng(seed)
+    params = make_params("Starburst99_1grpT_3grpS", clight=0.01)
+    g = gaussian_field(128, rng)
+    rho = lognormal_sampler(g, 2.0)
+    tr = T.amr_tree(8, [300000, 75000], rho, rng)
+    fields = {}
+    for l in range(tr.lmin, tr.lmax + 1):
+        L = tr.level(l)
+        centres = (L.cell_pos() + 0.5) / float(1 << l)
+        fields[l] = random_fields(params, rho(l, centres), rng)
+    L = tr.level(tr.lmin)
+    leaves = np.nonzero(L.child < 0)[0]
+    dens = fields[tr.lmin].nh[leaves]
+    cells = leaves[np.argsort(-dens, kind="stable")[:4096]]
+    lum = powerlaw_luminosities(rng, cells.size, 1e49, 1e53)
+    fields[tr.lmin].src[:, cells] = params.src_code(lum, 0.5**tr.lmin)[None, :]
+    return Workload("C5-256^3+2-G3", params, tr, fields, nsub)
+
+
 def config_c4(seed: int = 4, nsub: int = 40) -> Workload:
     """Levels 7-11 with 262144/80000/25000/6000/1200 octs, lognormal sigma=1.5."""
     return amr_workload(7, [80000, 25000, 6000, 1200], "Starburst99_1grpT_3grpS", seed, sigma=1.5, nsub=nsub,

@@ -83,18 +83,28 @@ class Level:
 
 
 class _Lookup:
-    """Sorted-key lookup of oct coordinates on one level (periodic grid)."""
+    """Lookup of oct coordinates on one level (periodic grid of n^3 octs):
+    a dense index grid up to 2^24 octs, sorted keys beyond."""
+
+    DENSE_MAX = 1 << 24
 
     def __init__(self, pos: np.ndarray, n: int):
         self.n = n
         key = (pos[:, 0].astype(np.int64) * n + pos[:, 1]) * n + pos[:, 2]
-        self.order = np.argsort(key, kind="stable")
-        self.keys = key[self.order]
+        if n**3 <= self.DENSE_MAX:
+            self.grid = np.full(n**3, -1, dtype=np.int64)
+            self.grid[key] = np.arange(key.size)
+        else:
+            self.grid = None
+            self.order = np.argsort(key, kind="stable")
+            self.keys = key[self.order]
 
     def find(self, q: np.ndarray) -> np.ndarray:
         n = self.n
         q = np.mod(q, n)
         key = (q[:, 0].astype(np.int64) * n + q[:, 1]) * n + q[:, 2]
+        if self.grid is not None:
+            return self.grid[key]
         i = np.searchsorted(self.keys, key)
         i = np.minimum(i, len(self.keys) - 1)
         ok = self.keys[i] == key

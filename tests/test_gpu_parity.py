@@ -213,3 +213,48 @@ def test_hydro_coupling_and_source(gpu_lib, oracle, table, G, mode, precision):
         assert max(err.values()) < tol, (l, err)
         assert max(herr) < tol, (l, herr)
 
+
+
+def outlier_errors(A: Fields, B: Fields, c: float):
+    """per field: (max relative error, number of cells beyond MIXED_TOL)"""
+    out = {}
+    xa, xb = A.nhplus / A.nh, B.nhplus / B.nh
+    pairs = {"e": (A.e, B.e, np.abs(B.e)), "x": (xa, xb, np.abs(xb)), "eint": (A.eint, B.eint, np.abs(B.eint)),
+             "temp": (A.temp, B.temp, np.abs(B.temp))}
+    for k in ("fx", "fy", "fz"):
+        pairs[k] = (getattr(A, k), getattr(B, k), np.maximum(c * B.e, 1e-300))
+    for k, (a, b, scale) in pairs.items():
+        r = np.abs(a - b) / np.maximum(scale, 1e-300)
+        if r.ndim == 2:
+            r = r.max(0)
+        out[k] = (float(r.max()), int(np.count_nonzero(r > MIXED_TOL)))
+    return out
+
+
+@pytest.mark.parametrize("config", ["c3", "c4"])
+def test_full_size_mixed_vs_fp64(gpu_lib, config):
+    """At the benchmark sizes (C3: 128^3, 3 groups, 64 sources; C4: AMR levels
+    7-11), the mixed-precision tiled kernels against the FP64 kernels (pinned
+    to the oracle at small sizes) after two sub-cycles (C3) or one coarse step
+    of the level recursion (C4).  The chemistry sub-step controller of the
+    reference is discontinuous (test_conditioning.py: a 1e-6 perturbation of
+    the state moves single cells of C3 by 0.5 % in the FP64 oracle itself), so
+    the bound is on the cells: at most 1 in 20,000 (and at least 5 allowed per
+    level: a cell that flips also moves its neighbours in later sub-steps of
+    the level) beyond the mixed tolerance, none beyond 5 %."""
+    W = synth.config_c3(nsub=2) if config == "c3" else synth.config_c4(nsub=1)
+    runs = {}
+    for prec in (rt.FP64, rt.MIXED):
+        g = rt.RadGPU(W.params, prec)
+        g.setup(W.params, W.tree, W.fields)
+        g.coarse_step(1.0, W.nsub if config == "c3" else 1)
+        runs[prec] = {l: g.get(l) for l in range(W.tree.lmin, W.tree.lmax + 1)}
+        g.close()
+    c = W.params.cloc()
+    for l in range(W.tree.lmin, W.tree.lmax + 1):
+        err = outlier_errors(runs[rt.MIXED][l], runs[rt.FP64][l], c)
+        n = runs[rt.FP64][l].ncell
+        print(config, "level", l, n, err)
+        for k, (mx, nbad) in err.items():
+            assert mx < 0.05, (l, k, mx)
+            assert nbad <= max(5, n // 20000), (l, k, nbad)
