@@ -5,6 +5,8 @@ Workloads (seeded synthetic states; see emma_amd/synth.py):
                64 point sources, lognormal n_H; mixed precision.  A step is one
                radiative sub-cycle of the level.
   c2           64^3 periodic level, 1 group; FP64 plain.  A step is one sub-cycle.
+  c5           256^3 coarse level (level 8) + 300000/75000 refined octs, 3 groups,
+               4096 sources, c/100; timed like c4.
   c4           AMR levels 7-11 with 262144/80000/25000/6000/1200 octs, 3 groups,
                mixed precision.  A step is one coarse step of the RADSTEP level
                recursion (level l takes 2^(l-7) sub-steps, 31 launches), replayed as
@@ -52,6 +54,8 @@ def make_workload(name: str, seed: int):
         return synth.config_c2(seed=seed)
     if name == "c4":
         return synth.config_c4(seed=seed, nsub=1)
+    if name == "c5":
+        return synth.config_c5(seed=seed, nsub=1)
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -59,6 +63,7 @@ WORKLOAD_NAMES = {
     "c3": "128^3 periodic level (2,097,152 cells), 3 groups, 64 sources, lognormal n_H",
     "c2": "64^3 periodic level (262,144 cells), 1 group, lognormal n_H",
     "c4": "AMR levels 7-11 (262144/80000/25000/6000/1200 octs), 3 groups, lognormal sigma=1.5",
+    "c5": "256^3 coarse level + 2 refined levels (300000/75000 octs), 3 groups, 4096 sources, c/100, lognormal sigma=2",
 }
 
 
@@ -222,7 +227,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c4"])
+    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c4", "c5"])
     ap.add_argument("--precision", default=None, choices=["mixed", "fp64"])
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -241,11 +246,11 @@ def main():
 
     precision = args.precision or ("fp64" if args.workload == "c2" else "mixed")
     W = make_workload(args.workload, args.seed)
-    runner = run_amr if args.workload == "c4" else run_uniform
+    runner = run_amr if args.workload in ("c4", "c5") else run_uniform
     wall, st, kernel_s, bytes_per_unit, launches = runner(W, precision, args.steps, args.warmup)
     cell_updates = st["cell_updates"]
     value = cell_updates / wall
-    if args.workload == "c4":
+    if args.workload in ("c4", "c5"):
         achieved = bytes_per_unit / kernel_s / 1e9  # one eager coarse step
         mean_launch_us = kernel_s / max(launches, 1) * 1e6
     else:
@@ -270,6 +275,7 @@ def main():
             "groups": W.params.ngrp,
             "precision": precision,
             "step": ("second coarse step (31 level steps, HIP graph)" if args.workload == "c4"
+                     else "second coarse step (7 level steps, HIP graph)" if args.workload == "c5"
                      else "radiative sub-cycle"),
             "chem_substeps_per_cell_update": st["chem_substeps"] / max(cell_updates, 1),
             "mean_kernel_us": mean_launch_us,

@@ -231,18 +231,20 @@ def outlier_errors(A: Fields, B: Fields, c: float):
     return out
 
 
-@pytest.mark.parametrize("config", ["c3", "c4"])
+@pytest.mark.parametrize("config", ["c3", "c4", "c5"])
 def test_full_size_mixed_vs_fp64(gpu_lib, config):
     """At the benchmark sizes (C3: 128^3, 3 groups, 64 sources; C4: AMR levels
-    7-11), the mixed-precision tiled kernels against the FP64 kernels (pinned
-    to the oracle at small sizes) after two sub-cycles (C3) or one coarse step
-    of the level recursion (C4).  The chemistry sub-step controller of the
+    7-11; C5: 256^3 + two refined levels, 4096 sources, c/100), the
+    mixed-precision tiled kernels against the FP64 kernels (pinned to the
+    oracle at small sizes) after two sub-cycles (C3) or one coarse step of the
+    level recursion (C4, C5).  The chemistry sub-step controller of the
     reference is discontinuous (test_conditioning.py: a 1e-6 perturbation of
     the state moves single cells of C3 by 0.5 % in the FP64 oracle itself), so
     the bound is on the cells: at most 1 in 20,000 (and at least 5 allowed per
     level: a cell that flips also moves its neighbours in later sub-steps of
     the level) beyond the mixed tolerance, none beyond 5 %."""
-    W = synth.config_c3(nsub=2) if config == "c3" else synth.config_c4(nsub=1)
+    W = {"c3": lambda: synth.config_c3(nsub=2), "c4": lambda: synth.config_c4(nsub=1),
+         "c5": lambda: synth.config_c5(nsub=1)}[config]()
     runs = {}
     for prec in (rt.FP64, rt.MIXED):
         g = rt.RadGPU(W.params, prec)
