@@ -186,3 +186,49 @@ def test_stromgren_sphere(gpu_lib):
     assert abs(r_eq / r_s - 1) < 0.05
     assert abs(r_eq / 5.4 - 1) < 0.15
     assert all(a < b for a, b in zip(radii, radii[1:]))
+
+
+def test_stromgren_device_source(gpu_lib):
+    """Config C1 with the source built on the device: FillRad's
+    stromgren_source (src_utils.c:339-412) inside every level step of the
+    coarse-step graph places 200 solar masses x 5e48/(200 M_sun) photons/s,
+    spread over the 8 cells within 2^-lcoarse of the box centre, which is the
+    source of test_stromgren_sphere.  Checked: the device source equals that
+    host-placed source cell by cell, and after ~6 recombination times the
+    emission balances the case-B recombinations within 5 %."""
+    from oracle import Oracle
+
+    sig = verner_sigma(29.6)
+    p = RunParams.from_table([(29.6, sig, sig, 1.0)], 0.1, **static_units(1e3, 15.0))
+    level = 5
+    tr = T.uniform_tree(level)
+    L = tr.level(level)
+    n = L.ncell
+    F = Fields.zeros(1, n)
+    F.nh[:] = 1.0
+    F.nhplus[:] = 1.2e-3
+    F.eint[:] = p.eint_code(F.nh, 1.2e-3, 1e2)
+    F.e[:] = 1e-8 / p.unit_N * 1.01
+    synth.e2t(F, p)
+    g = rt.RadGPU(p, rt.FP64)
+    g.setup(p, tr, {level: F}, tphy=1e-3)  # the source is off at tphy = 0
+    g.set_oct_pos(level, synth.oct_corners(L))
+    g.set_stromgren_source(synth.Star(x=0.5, y=0.5, z=0.5, mass=2e2, srcint=5e48 / (2e2 * 1.989e30)))
+    g.fillrad(level)
+    src = g.get(level).src[0]
+    cp = L.cell_pos()
+    centre = np.all((cp == 15) | (cp == 16), axis=1)
+    np.testing.assert_allclose(src[centre], p.src_code(5e48 / 8, 0.5**level), rtol=1e-14)
+    assert np.all(src[~centre] == 0)
+    dt = p.dt_rad(level)
+    t, t_unit_myr = 0.0, p.unit_t / MYR
+    while t < 900.0:
+        g.coarse_step(1.0, 4000)
+        t += 4000 * dt * t_unit_myr
+    X = g.get(level)
+    x = X.nhplus / X.nh
+    nH, ndot, dv = 1e3, 5e48, (p.unit_l / 32) ** 3
+    ab = Oracle().rates(X.temp, x, np.full(n, nH))["alpha_b"]
+    rec = nH**2 * np.sum(ab * x * x) * dv
+    print(f"t={t:.0f} Myr recombinations/emission = {rec / ndot:.4f}")
+    assert abs(rec / ndot - 1) < 0.05
