@@ -562,6 +562,13 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
     }
   };
   load_item(lane);
+  // the child entries of the coarse leaves on the cell's leaving faces (their
+  // reflux slots), loaded now for the refluxing in the direction passes
+  int cchD[3] = {0, 0, 0};
+  if (a.ncoarse > 0) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) cchD[d] = a.cchild[noD[d] < 0 ? -1 - noD[d] : 0];
+  }
   // ---- ... compute this cell's closures meanwhile ...
   Closure<float> Q[G];
   if (valid) {
@@ -677,7 +684,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
             (1ull << l0) | (1ull << (l0 | b1)) | (1ull << (l0 | b2)) | (1ull << (l0 | b1 | b2));
         const unsigned long long mine = rmask & grp;
         if (rj >= 0 && lane == __builtin_ctzll(mine)) {
-          const int cch = a.cchild[rj];
+          const int cch = cchD[d];
           if (cch <= -2) {
             float* acc = a.cracc + (size_t)(-2 - cch) * 4 * G;
             const float w = (bit ? 1.f : -1.f) * (0.5f * dts) * 0.125f;  // pow(-1, face+1) dt/dx / 8, shares doubled
