@@ -309,6 +309,7 @@ extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int
   std::vector<size_t> off_n(nl), off_c(nl);
   for (int k = 0; k < nl; k++) {
     if (noct[k] < 0 || (k == 0 && noct[k] == 0)) return EMMA_RT_ERR_ARG;
+    if ((long long)noct[k] * 8 > max_level_cells(G)) return EMMA_RT_ERR_ARG;
     off_n[k] = on;
     off_c[k] = oc;
     on += (size_t)noct[k] * 6;

@@ -85,6 +85,14 @@ constexpr int kTileCells = 256;
 constexpr int kTileOcts = kTileCells / 8;
 constexpr int kTileWaves = kTileCells / 64;
 
+// Most cells in one level with ng groups: cell and group*cell indices are
+// 32-bit ints and every byte offset into a per-group array (8-byte elements at
+// most) stays below 4 GiB.
+inline long long max_level_cells(int ng) {
+  const long long by_index = 2147483647LL / (ng > 0 ? ng : 1);
+  return by_index < (1LL << 29) ? by_index : (1LL << 29);
+}
+
 template <typename R>
 hipError_t launch_level_step(const LevelArgs<R>& a, const ChemConst& K, hipStream_t s);
 

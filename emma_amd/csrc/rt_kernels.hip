@@ -41,6 +41,40 @@ __device__ __forceinline__ void load4(const LevelArgs<R>& a, const R* e, const R
   ofz = fz[g * n + j];
 }
 
+// Element i of an array whose base is the same in every lane of the wave.
+// The base goes through readfirstlane (free for a value already in SGPRs) so
+// that the optimiser cannot fold a per-lane index into it, and the byte
+// offset is a 32-bit unsigned value: the access then takes the scalar-base
+// form of the global load/store (64-bit base in SGPRs + 32-bit offset in one
+// VGPR) rather than a 64-bit address per lane, and arrays indexed by the same
+// cell share one offset register.  The level size limit (max_level_cells)
+// keeps every offset below 4 GiB.
+#define EMMA_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ EMMA_GLOBAL char* uniform_base(T* base) {
+  const unsigned long long p = (unsigned long long)base;
+  const unsigned long long lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)p);
+  const unsigned long long hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(p >> 32));
+  return (EMMA_GLOBAL char*)((hi << 32) | lo);
+}
+// the 32-bit byte offset, kept opaque so that a select or mask that formed
+// the index is not pushed through the zero extension (which would hide the
+// 32-bit offset from the instruction selector)
+template <typename T>
+__device__ __forceinline__ size_t byte_offset(int i) {
+  unsigned off = (unsigned)i * (unsigned)sizeof(T);
+  asm("" : "+v"(off));
+  return (size_t)off;
+}
+template <typename T>
+__device__ __forceinline__ T gld(const T* base, int i) {
+  return *(const EMMA_GLOBAL T*)(uniform_base(base) + byte_offset<T>(i));
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* base, int i, T v) {
+  *(EMMA_GLOBAL T*)(uniform_base(base) + byte_offset<T>(i)) = v;
+}
+
 template <typename T>
 __device__ __forceinline__ T sel(int i, T a, T b, T c) {
   return i == 0 ? a : (i == 1 ? b : c);
@@ -81,6 +115,7 @@ __device__ __forceinline__ void restrict_cell(const LevelArgs<R>& a, int cell, i
   for (int g = 0; g < ng; g++) {
     R se = R(0), sx = R(0), sy = R(0), sz = R(0), ss = R(0);
     for (int k = 0; k < 8; k++) {
+      // plain indexing: the eight children's values merge into wide loads
       const int j = g * nf + j0 + k;
       se += a.fe[j] * R(0.125);
       sx += a.ffx[j] * R(0.125);
@@ -88,11 +123,11 @@ __device__ __forceinline__ void restrict_cell(const LevelArgs<R>& a, int cell, i
       sz += a.ffz[j] * R(0.125);
       ss += a.fsrc[j] * R(0.125);
     }
-    a.eo[g * n + cell] = se;
-    a.fxo[g * n + cell] = sx;
-    a.fyo[g * n + cell] = sy;
-    a.fzo[g * n + cell] = sz;
-    a.src[g * n + cell] = ss;
+    gst(a.eo + g * n, cell, se);
+    gst(a.fxo + g * n, cell, sx);
+    gst(a.fyo + g * n, cell, sy);
+    gst(a.fzo + g * n, cell, sz);
+    gst(a.src + g * n, cell, ss);
   }
   double snp = 0., sei = 0., stp = 0., snh = 0., stf = 0., stl = 0.;
   for (int k = 0; k < 8; k++) {
@@ -106,10 +141,10 @@ __device__ __forceinline__ void restrict_cell(const LevelArgs<R>& a, int cell, i
     stp += a.ftemp[j] * 0.125;
     snh += a.fnh[j] * 0.125;
   }
-  a.nhpluso[cell] = snp;
-  a.einto[cell] = sei;
-  a.tempo[cell] = stp;
-  a.nh[cell] = snh;
+  gst(a.nhpluso, cell, snp);
+  gst(a.einto, cell, sei);
+  gst(a.tempo, cell, stp);
+  gst(a.nh, cell, snh);
   if (a.t_first != nullptr) {
     a.t_first[cell] = stf;
     a.t_last[cell] = stl;
@@ -167,15 +202,15 @@ __device__ __forceinline__ int finish_leaf(const LevelArgs<R>& a, const ChemCons
       fx[g] = Ux[g] + Dx[g];
       fy[g] = Uy[g] + Dy[g];
       fz[g] = Uz[g] + Dz[g];
-      src[g] = (double)(pre_src != nullptr ? pre_src[g] : a.src[g * n + cell]);
+      src[g] = (double)(pre_src != nullptr ? pre_src[g] : gld(a.src + g * n, cell));
     }
   }
   // rad_utils.c:1611-1650 (the second test reads e[1], i.e. fx[0] when G=1)
   if (e[0] < 0 || isnan(e[0]) || isnan(ng > 1 ? e[1 < GA ? 1 : 0] : (double)fx[0]))
     raise_status(a.status, -3, a.level, cell);
-  double nhplus = pre != nullptr ? pre[0] : a.nhplus[cell];
-  double eint = pre != nullptr ? pre[1] : a.eint[cell];
-  const double nh = pre != nullptr ? pre[2] : a.nh[cell];
+  double nhplus = pre != nullptr ? pre[0] : gld(a.nhplus, cell);
+  double eint = pre != nullptr ? pre[1] : gld(a.eint, cell);
+  const double nh = pre != nullptr ? pre[2] : gld(a.nh, cell);
   double temp = 0.;
   int nit = 0;
   int st;
@@ -215,15 +250,15 @@ __device__ __forceinline__ int finish_leaf(const LevelArgs<R>& a, const ChemCons
 #pragma unroll
   for (int g = 0; g < GA; g++) {
     if (g < ng) {
-      a.eo[g * n + cell] = (R)e[g];
-      a.fxo[g * n + cell] = fx[g];
-      a.fyo[g * n + cell] = fy[g];
-      a.fzo[g * n + cell] = fz[g];
+      gst(a.eo + g * n, cell, (R)e[g]);
+      gst(a.fxo + g * n, cell, fx[g]);
+      gst(a.fyo + g * n, cell, fy[g]);
+      gst(a.fzo + g * n, cell, fz[g]);
     }
   }
-  a.nhpluso[cell] = nhplus;
-  a.einto[cell] = eint;
-  a.tempo[cell] = temp;
+  gst(a.nhpluso, cell, nhplus);
+  gst(a.einto, cell, eint);
+  gst(a.tempo, cell, temp);
   return nit;
 }
 
@@ -238,18 +273,18 @@ __device__ __forceinline__ void transport_split_cell(const LevelArgs<R>& a, int 
 #pragma unroll
   for (int g = 0; g < GA; g++) {
     if (g < ng) {
-      a.eo[g * n + cell] = Ue[g] + De[g];
-      a.fxo[g * n + cell] = Ux[g] + Dx[g];
-      a.fyo[g * n + cell] = Uy[g] + Dy[g];
-      a.fzo[g * n + cell] = Uz[g] + Dz[g];
+      gst(a.eo + g * n, cell, Ue[g] + De[g]);
+      gst(a.fxo + g * n, cell, Ux[g] + Dx[g]);
+      gst(a.fyo + g * n, cell, Uy[g] + Dy[g]);
+      gst(a.fzo + g * n, cell, Uz[g] + Dz[g]);
     }
   }
   const double e0 = (double)(R)(Ue[0] + De[0]);
   const double e1 = ng > 1 ? (double)(R)(Ue[1 < GA ? 1 : 0] + De[1 < GA ? 1 : 0]) : (double)(R)(Ux[0] + Dx[0]);
   if (e0 < 0 || isnan(e0) || isnan(e1)) raise_status(a.status, -3, a.level, cell);
-  a.nhpluso[cell] = a.nhplus[cell];
-  a.einto[cell] = a.eint[cell];
-  a.tempo[cell] = a.temp[cell];
+  gst(a.nhpluso, cell, gld(a.nhplus, cell));
+  gst(a.einto, cell, gld(a.eint, cell));
+  gst(a.tempo, cell, gld(a.temp, cell));
 }
 
 // per-block counters: accepted chemistry sub-steps and leaf cell-updates,
@@ -409,27 +444,12 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value
 // in-tile and a halo face differ only in the LDS address read.  A complete
 // level has 64 halo faces per wave (16 in x and y, 32 in z), one per lane
 // (HR = 1).  Sparse AMR levels, where a wave has up to 192 halo faces (every
-// cell's three leaving faces), use HR = 3 rounds of 64; faces beyond the
-// 64*HR slots evaluate their neighbour's share from global memory.
+// cell's three leaving faces), use HR = 3 rounds of 64.  The host picks HR
+// from the level's topology (the most halo faces in any wave), so a wave
+// never has more halo faces than slots.
 
 __device__ __forceinline__ int lanes_below(unsigned long long m) {
   return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-}
-
-// the share (A if side = +1, B if side = -1) in direction d of cell j of
-// this level (coarse = 0) or of the coarser level (coarse = 1), group g
-template <int G>
-__device__ __forceinline__ float4 share_of(const LevelArgs<float>& a, bool coarse, int j, int g, int d, float side) {
-  const float* Ne = coarse ? a.ce : a.e;
-  const float* Nx = coarse ? a.cfx : a.fx;
-  const float* Ny = coarse ? a.cfy : a.fy;
-  const float* Nz = coarse ? a.cfz : a.fz;
-  const int nn = coarse ? a.ncoarse : a.ncell;
-  const float ve = Ne[g * nn + j], vx = Nx[g * nn + j], vy = Ny[g * nn + j], vz = Nz[g * nn + j];
-  const Closure<float> q = closure_fast(vx, vy, vz, ve, a.c);
-  float X[4];
-  lf_share_lane(d, side, ve, vx, vy, vz, q, a.c, X);
-  return make_float4(X[0], X[1], X[2], X[3]);
 }
 
 template <int G, int HR>
@@ -465,21 +485,21 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   int ch = -1;
   if (valid) {
     // in the order of use: topology (halo list), state, chemistry inputs
-    ch = a.child[cell];
+    ch = gld(a.child, cell);
 #pragma unroll
-    for (int d = 0; d < 3; d++) noD[d] = a.nei[oct * 6 + 2 * d + ((ic >> d) & 1)];
+    for (int d = 0; d < 3; d++) noD[d] = gld(a.nei, oct * 6 + 2 * d + ((ic >> d) & 1));
 #pragma unroll
     for (int g = 0; g < G; g++) {
-      Ue[g] = a.e[g * n + cell];
-      Ux[g] = a.fx[g * n + cell];
-      Uy[g] = a.fy[g * n + cell];
-      Uz[g] = a.fz[g * n + cell];
+      Ue[g] = gld(a.e + g * n, cell);
+      Ux[g] = gld(a.fx + g * n, cell);
+      Uy[g] = gld(a.fy + g * n, cell);
+      Uz[g] = gld(a.fz + g * n, cell);
     }
 #pragma unroll
-    for (int g = 0; g < G; g++) Src[g] = a.src[g * n + cell];
-    pre[0] = a.nhplus[cell];
-    pre[1] = a.eint[cell];
-    pre[2] = a.nh[cell];
+    for (int g = 0; g < G; g++) Src[g] = gld(a.src + g * n, cell);
+    pre[0] = gld(a.nhplus, cell);
+    pre[1] = gld(a.eint, cell);
+    pre[2] = gld(a.nh, cell);
   }
   sCancel[lc] = (unsigned char)(ch >= 0 && !a.keep_split_flux);
 
@@ -525,19 +545,19 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
     const int js = (hlane && !hcoarse) ? hno * 8 + (hic ^ (1 << hd)) : 0;
 #pragma unroll
     for (int g = 0; g < G; g++) {
-      hv[g][0] = a.e[g * n + js];
-      hv[g][1] = a.fx[g * n + js];
-      hv[g][2] = a.fy[g * n + js];
-      hv[g][3] = a.fz[g * n + js];
+      hv[g][0] = gld(a.e + g * n, js);
+      hv[g][1] = gld(a.fx + g * n, js);
+      hv[g][2] = gld(a.fy + g * n, js);
+      hv[g][3] = gld(a.fz + g * n, js);
     }
-    hchild = a.child[js];
+    hchild = gld(a.child, js);
     if (__ballot(hcoarse) != 0) {  // injected coarse leaves (levels with a coarser level only)
       const int jc = hcoarse ? -1 - hno : 0;
       const int nc = a.ncoarse;
 #pragma unroll
       for (int g = 0; g < G; g++) {
-        const float ce = a.ce[g * nc + jc], cx = a.cfx[g * nc + jc], cy = a.cfy[g * nc + jc],
-                    cz = a.cfz[g * nc + jc];
+        const float ce = gld(a.ce + g * nc, jc), cx = gld(a.cfx + g * nc, jc), cy = gld(a.cfy + g * nc, jc),
+                    cz = gld(a.cfz + g * nc, jc);
         if (hcoarse) {
           hv[g][0] = ce;
           hv[g][1] = cx;
@@ -567,7 +587,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   int cchD[3] = {0, 0, 0};
   if (a.ncoarse > 0) {
 #pragma unroll
-    for (int d = 0; d < 3; d++) cchD[d] = a.cchild[noD[d] < 0 ? -1 - noD[d] : 0];
+    for (int d = 0; d < 3; d++) cchD[d] = gld(a.cchild, noD[d] < 0 ? -1 - noD[d] : 0);
   }
   // ---- ... compute this cell's closures meanwhile ...
   Closure<float> Q[G];
@@ -585,6 +605,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   float De[G], Dx[G], Dy[G], Dz[G];
 #pragma unroll
   for (int g = 0; g < G; g++) De[g] = Dx[g] = Dy[g] = Dz[g] = 0.f;
+  float ucoef = 0.f;  // coefficient of c U summed over the directions
   if (transport) {
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -614,8 +635,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
         const int iin = (1 - bit) * kTileCells + jin;  // neighbour share, inside face
         const bool inT = noD[d] >= oct0 && noD[d] < oct0 + kTileOcts;
         const int jl = (noD[d] - oct0) * 8 + (ic ^ (1 << d));
-        const int slot = hbase[d] + lanes_below(hball[d]);
-        const bool overflow = !inT && slot >= kHaloCap;
+        const int slot = hbase[d] + lanes_below(hball[d]);  // < kHaloCap (HR from the topology)
         const bool coarse_face = !inT && noD[d] < 0;
         int iout, sout, cout;  // neighbour share, stride over groups, cancel flag: leaving face
         if (inT) {
@@ -623,42 +643,59 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
           sout = 2 * kTileCells;
           cout = jl;
         } else {
-          iout = overflow ? 0 : kShares + (wv * kHaloCap + slot) * G;
+          iout = kShares + (wv * kHaloCap + slot) * G;
           sout = 1;
-          cout = overflow ? 0 : kTileCells + wv * kHaloCap + slot;
+          cout = kTileCells + wv * kHaloCap + slot;
         }
         const int j = coarse_face ? -1 - noD[d] : noD[d] * 8 + (ic ^ (1 << d));
-        bool cancel_out = sCancel[cout] != 0;
-        if (overflow) cancel_out = !coarse_face && !a.keep_split_flux && a.child[j] >= 0;
+        const bool cancel_out = sCancel[cout] != 0;
         const bool cancel_in = sCancel[jin] != 0;
         const int im = bit ? iin : iout, sm = bit ? 2 * kTileCells : sout;
         const int ip = bit ? iout : iin, sp = bit ? sout : 2 * kTileCells;
-        // the shares are doubled: the face fluxes below are 2F
+        // The shares are doubled: the face fluxes are 2F_m = X_m + B and
+        // 2F_p = A + X_p, with X the neighbours' shares and A = P + cU,
+        // B = P - cU this cell's.  The update wm 2F_m + wp 2F_p then is
+        // wm X_m + wp X_p + (wm + wp) P + (wp - wm) cU.  On complete levels
+        // (HR = 1) nearly every cell has wm = -wp (both faces carry flux, or
+        // neither does): the P term drops and the cU term is summed over the
+        // directions and applied once, and only the rare cell with one
+        // cancelled face or a coarse face reads its own shares back.  Sparse
+        // levels, where coarse faces are common, keep the plain form.
         const float wm = (bit ? cancel_in : cancel_out) ? 0.f : 0.5f * dts;
         const float wp = (bit ? cancel_out : cancel_in) ? 0.f : -0.5f * dts;
+        const bool plain = HR == 1 && wm == -wp;
+        if (plain) ucoef += wp - wm;
 #pragma unroll
         for (int g = 0; g < G; g++) {
-          // this cell's shares are read back rather than held across the barrier
-          const float4 Ag = sShare[(g * 2 + 0) * kTileCells + lc];
-          const float4 Bg = sShare[(g * 2 + 1) * kTileCells + lc];
-          float4 Xm = sShare[im + g * sm];
-          float4 Xp = sShare[ip + g * sp];
-          if (overflow) {
-            const float4 Xo = share_of<G>(a, coarse_face, j, g, d, bit ? -1.f : 1.f);
-            if (bit)
-              Xp = Xo;
-            else
-              Xm = Xo;
+          const float4 Xm = sShare[im + g * sm];
+          const float4 Xp = sShare[ip + g * sp];
+          if (HR == 1) {
+            De[g] = fmaf(Xp.x, wp, fmaf(Xm.x, wm, De[g]));
+            Dx[g] = fmaf(Xp.y, wp, fmaf(Xm.y, wm, Dx[g]));
+            Dy[g] = fmaf(Xp.z, wp, fmaf(Xm.z, wm, Dy[g]));
+            Dz[g] = fmaf(Xp.w, wp, fmaf(Xm.w, wm, Dz[g]));
           }
-          const float Fm[4] = {Xm.x + Bg.x, Xm.y + Bg.y, Xm.z + Bg.z, Xm.w + Bg.w};
-          const float Fp[4] = {Ag.x + Xp.x, Ag.y + Xp.y, Ag.z + Xp.z, Ag.w + Xp.w};
-          De[g] = fmaf(Fp[0], wp, fmaf(Fm[0], wm, De[g]));
-          Dx[g] = fmaf(Fp[1], wp, fmaf(Fm[1], wm, Dx[g]));
-          Dy[g] = fmaf(Fp[2], wp, fmaf(Fm[2], wm, Dy[g]));
-          Dz[g] = fmaf(Fp[3], wp, fmaf(Fm[3], wm, Dz[g]));
-          if (coarse_face) {
+          if (!plain || coarse_face) {
+            // this cell's shares are read back rather than held across the barrier
+            const float4 Ag = sShare[(g * 2 + 0) * kTileCells + lc];
+            const float4 Bg = sShare[(g * 2 + 1) * kTileCells + lc];
+            const float Fm[4] = {Xm.x + Bg.x, Xm.y + Bg.y, Xm.z + Bg.z, Xm.w + Bg.w};
+            const float Fp[4] = {Ag.x + Xp.x, Ag.y + Xp.y, Ag.z + Xp.z, Ag.w + Xp.w};
+            if (HR != 1) {
+              De[g] = fmaf(Fp[0], wp, fmaf(Fm[0], wm, De[g]));
+              Dx[g] = fmaf(Fp[1], wp, fmaf(Fm[1], wm, Dx[g]));
+              Dy[g] = fmaf(Fp[2], wp, fmaf(Fm[2], wm, Dy[g]));
+              Dz[g] = fmaf(Fp[3], wp, fmaf(Fm[3], wm, Dz[g]));
+            } else if (!plain) {
+              De[g] = fmaf(Ag.x, wp, fmaf(Bg.x, wm, De[g]));
+              Dx[g] = fmaf(Ag.y, wp, fmaf(Bg.y, wm, Dx[g]));
+              Dy[g] = fmaf(Ag.z, wp, fmaf(Bg.z, wm, Dy[g]));
+              Dz[g] = fmaf(Ag.w, wp, fmaf(Bg.w, wm, Dz[g]));
+            }
+            if (coarse_face) {  // the flux through the coarse face, for refluxing
 #pragma unroll
-            for (int v = 0; v < 4; v++) Rf[g][v] = bit ? Fp[v] : Fm[v];
+              for (int v = 0; v < 4; v++) Rf[g][v] = bit ? Fp[v] : Fm[v];
+            }
           }
         }
         if (coarse_face) rj = j;
@@ -696,6 +733,14 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
         }
       }
       if (d < 2) __syncthreads();  // the next direction overwrites the shares
+    }
+    const float cu = ucoef * c;
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      De[g] = fmaf(cu, Ue[g], De[g]);
+      Dx[g] = fmaf(cu, Ux[g], Dx[g]);
+      Dy[g] = fmaf(cu, Uy[g], Dy[g]);
+      Dz[g] = fmaf(cu, Uz[g], Dz[g]);
     }
   }
 

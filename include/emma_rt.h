@@ -127,7 +127,9 @@ int emma_rt_advance_stencil(emma_rt_ctx *ctx, void *stencil, int nread, double d
 
 /* ---- resident-level path ---- */
 /* Device tree of levels lmin..lmax: noct[lmax-lmin+1]; nei_oct and child are
-   the levels' arrays concatenated in level order (6 and 8 ints per oct). */
+   the levels' arrays concatenated in level order (6 and 8 ints per oct).
+   A level holds at most min(2^29, (2^31-1)/ngrp) cells (EMMA_RT_ERR_ARG
+   beyond: cell indices and per-group byte offsets are 32-bit). */
 int emma_rt_tree_create(emma_rt_ctx *ctx, int lmin, int lmax, const int *noct, const int *nei_oct,
                         const int *child, int precision);
 /* host SoA (double) <-> device state of one level (rfield, prototypes.h:849) */
