@@ -35,7 +35,7 @@ namespace {
 
 struct DevLevel {
   int level = 0, noct = 0, ncell = 0, nslot = 0, cur = 0;
-  int halo_rounds = 1;  // rounds of 64 halo slots per wave the tiled kernel needs
+  int halo_per_dir = 0;  // some wave has more than 64 halo faces (tiled kernel variant)
   void* e[2] = {nullptr, nullptr};
   void* fx[2] = {nullptr, nullptr};
   void* fy[2] = {nullptr, nullptr};
@@ -356,7 +356,7 @@ extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int
       }
       maxh = std::max(maxh, cnt);
     }
-    L.halo_rounds = maxh > 64 ? 3 : 1;
+    L.halo_per_dir = maxh > 64 ? 1 : 0;
   }
   // device allocation and upload
   for (int k = 0; k < nl; k++) {
@@ -543,7 +543,7 @@ static LevelArgs<R> level_args(emma_rt_ctx* c, int k, double dtnew, double aexp,
     A.keep_split_flux = 0;
     A.update_split = 0;
   }
-  A.halo_rounds = L.halo_rounds;
+  A.halo_per_dir = L.halo_per_dir;
   A.e = (const R*)L.e[a];
   A.fx = (const R*)L.fx[a];
   A.fy = (const R*)L.fy[a];

@@ -18,9 +18,9 @@ struct LevelArgs {
   // level and not for chemonly calls, fluxes of split neighbours kept, split
   // cells transport-updated (their chemistry still skipped).
   int transport, keep_split_flux, update_split;
-  // mixed path: rounds of 64 halo slots per wave the level's tiles need (1
-  // on complete levels, 3 on sparse ones; see level_step_tiled)
-  int halo_rounds;
+  // mixed path: 1 when some wave of the level has more than 64 halo faces
+  // (sparse levels: the per-direction halo variant of level_step_tiled)
+  int halo_per_dir;
   // current state (read)
   const R* e;
   const R* fx;

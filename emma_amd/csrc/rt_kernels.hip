@@ -437,25 +437,27 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value
 // directions as three passes over one LDS share buffer.
 //
 // Neighbours outside the tile (same-level octs of other tiles, injected
-// coarse leaves) are the halo.  Each wave lists its halo faces (lane,
-// direction) and evaluates them lane-parallel in one pass before the
-// direction passes: the neighbour's share and whether the face flux is
-// cancelled (split neighbour, RADSTEP) go to LDS slots of that wave, so an
-// in-tile and a halo face differ only in the LDS address read.  A complete
-// level has 64 halo faces per wave (16 in x and y, 32 in z), one per lane
-// (HR = 1).  Sparse AMR levels, where a wave has up to 192 halo faces (every
-// cell's three leaving faces), use HR = 3 rounds of 64.  The host picks HR
-// from the level's topology (the most halo faces in any wave), so a wave
-// never has more halo faces than slots.
+// coarse leaves) are the halo.  The neighbour's share and whether the face
+// flux is cancelled (split neighbour, RADSTEP) go to LDS slots of the wave,
+// 64 per wave, so an in-tile and a halo face differ only in the LDS address
+// read.  A complete level has 64 halo faces per wave (16 in x and y, 32 in
+// z): each wave lists them (lane, direction) and evaluates them
+// lane-parallel, one per lane, in one round before the direction passes
+// (PerDir = false).  Sparse AMR levels have up to 192 (every cell's three
+// leaving faces): there each lane evaluates its own halo face of direction d
+// at the start of pass d into its own slot (PerDir = true), which needs no
+// more LDS than the complete-level kernel.  The host picks the variant from
+// the level's topology (the most halo faces in any wave).
 
 __device__ __forceinline__ int lanes_below(unsigned long long m) {
   return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
-template <int G, int HR>
+template <int G, bool PerDir>
 __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4))) void level_step_tiled(
     LevelArgs<float> a_by_value, ChemConst K_by_value) {
-  constexpr int kHaloCap = 64 * HR;  // halo slots per wave
+  constexpr int kHaloCap = 64;                 // halo slots per wave
+  constexpr int kList = PerDir ? 1 : kHaloCap;  // halo list entries per wave
   const KernArgs<float> args = kernargs<float>();
   const LevelArgs<float>& a = args.a;
   const ChemConst& K = args.K;
@@ -465,8 +467,8 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   __shared__ float4 sShare[kShares + kTileWaves * kHaloCap * G];
   // 1: the flux through the face is cancelled; [cell], then [wave][slot]
   __shared__ unsigned char sCancel[kTileCells + kTileWaves * kHaloCap];
-  __shared__ unsigned char sItem[kTileWaves][kHaloCap];  // halo slot -> lane*4 + direction
-  __shared__ int sItemNei[kTileWaves][kHaloCap];         // halo slot -> nei_oct entry of the face
+  __shared__ unsigned char sItem[kTileWaves][kList];  // halo slot -> lane*4 + direction
+  __shared__ int sItemNei[kTileWaves][kList];         // halo slot -> nei_oct entry of the face
   const int tile = xcd_block(a.nblocks);
   const int lc = (int)threadIdx.x;
   const int wv = lc >> 6, lane = lc & 63;
@@ -503,43 +505,41 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   }
   sCancel[lc] = (unsigned char)(ch >= 0 && !a.keep_split_flux);
 
-  // ---- halo faces: list them per wave
   const bool transport = a.transport != 0;
   const bool moves = valid && transport && !(ch >= 0 && !a.update_split);
-  unsigned long long hball[3];  // lanes whose face in direction d is a halo face
-  int hbase[3];                 // first halo slot of direction d
+  auto in_tile = [&](int no) { return no >= oct0 && no < oct0 + kTileOcts; };
+  // ---- complete levels: list the halo faces per wave
+  unsigned long long hball[3] = {0ull, 0ull, 0ull};  // lanes whose face in direction d is a halo face
+  int hbase[3] = {0, 0, 0};                          // first halo slot of direction d
   int nhalo = 0;
+  if constexpr (!PerDir) {
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const bool out = moves && !(noD[d] >= oct0 && noD[d] < oct0 + kTileOcts);
-    hball[d] = __ballot(out);
-    hbase[d] = nhalo;
-    nhalo += __popcll(hball[d]);
-    if (out) {
-      const int slot = hbase[d] + lanes_below(hball[d]);
-      if (slot < kHaloCap) {
+    for (int d = 0; d < 3; d++) {
+      const bool out = moves && !in_tile(noD[d]);
+      hball[d] = __ballot(out);
+      hbase[d] = nhalo;
+      nhalo += __popcll(hball[d]);
+      if (out) {
+        const int slot = hbase[d] + lanes_below(hball[d]);  // < kHaloCap (variant chosen from the topology)
         sItem[wv][slot] = (unsigned char)(lane * 4 + d);
         sItemNei[wv][slot] = noD[d];
       }
     }
+    // the list is written and read by lanes of the same wave
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  // the list is written and read by lanes of the same wave
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // ---- one halo face per lane and round: issue the neighbour's loads ...
+  // ---- one halo face per lane: issue the neighbour's loads ...
   // Straight-line code (every lane loads, lanes without a halo face from
   // cell 0): a load under a divergent branch makes the wait before the
   // closures below cover it as well.
   bool hlane = false, hcoarse = false;
   int hd = 0, hside = 0, hchild = -1;
   float hv[G][4];
-  auto load_item = [&](int slot) {
-    hlane = slot < nhalo && slot < kHaloCap;
-    const int it = sItem[wv][slot < kHaloCap ? slot : 0];  // read by every lane, used by halo lanes
-    const int hno = sItemNei[wv][slot < kHaloCap ? slot : 0];
-    hd = it & 3;
-    const int hic = (it >> 2) & 7;  // the owning cell's position in its oct
+  // the neighbour across the face in direction hd that leaves the oct of
+  // the cell at position hic, nei_oct entry hno
+  auto load_neighbour = [&](int hno, int hic) {
     hside = (hic >> hd) & 1;
     hcoarse = hlane && hno < 0;
     const int js = (hlane && !hcoarse) ? hno * 8 + (hic ^ (1 << hd)) : 0;
@@ -567,6 +567,19 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
       }
     }
   };
+  // complete levels: list entry `slot` (read by every lane, used by halo lanes)
+  auto load_item = [&](int slot) {
+    hlane = slot < nhalo;
+    const int it = sItem[wv][PerDir ? 0 : slot];
+    hd = it & 3;
+    load_neighbour(sItemNei[wv][PerDir ? 0 : slot], (it >> 2) & 7);
+  };
+  // sparse levels: this lane's own face in direction d
+  auto load_face = [&](int d) {
+    hlane = moves && !in_tile(noD[d]);
+    hd = d;
+    load_neighbour(noD[d], ic);
+  };
   // ... the neighbour's share (a neighbour on the plus side of the face,
   // side 1, contributes B, on the minus side A) and cancel flag
   auto store_item = [&](int slot) {
@@ -581,7 +594,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
       }
     }
   };
-  load_item(lane);
+  if constexpr (!PerDir) load_item(lane);
   // the child entries of the coarse leaves on the cell's leaving faces (their
   // reflux slots), loaded now for the refluxing in the direction passes
   int cchD[3] = {0, 0, 0};
@@ -595,12 +608,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
     for (int g = 0; g < G; g++) Q[g] = closure_fast(Ux[g], Uy[g], Uz[g], Ue[g], c);
   }
-  store_item(lane);
-#pragma unroll
-  for (int r = 1; r < HR; r++) {
-    load_item(lane + 64 * r);
-    store_item(lane + 64 * r);
-  }
+  if constexpr (!PerDir) store_item(lane);
 
   float De[G], Dx[G], Dy[G], Dz[G];
 #pragma unroll
@@ -609,6 +617,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   if (transport) {
 #pragma unroll
     for (int d = 0; d < 3; d++) {
+      if constexpr (PerDir) load_face(d);  // latency behind this cell's shares
       if (valid) {
 #pragma unroll
         for (int g = 0; g < G; g++) {
@@ -619,6 +628,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
           sShare[(g * 2 + 1) * kTileCells + lc] = make_float4(X[0], X[1], X[2], X[3]);
         }
       }
+      if constexpr (PerDir) store_item(lane);  // own slot: written and read by this lane only
       __syncthreads();
       float Rf[G][4];  // flux through a coarse face (refluxing), 0 elsewhere
       int rj = -1;     // the coarse leaf on that face
@@ -633,9 +643,9 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
         // its plus side the cell's A plus the neighbour's B.
         const int jin = lc ^ (1 << d);
         const int iin = (1 - bit) * kTileCells + jin;  // neighbour share, inside face
-        const bool inT = noD[d] >= oct0 && noD[d] < oct0 + kTileOcts;
+        const bool inT = in_tile(noD[d]);
         const int jl = (noD[d] - oct0) * 8 + (ic ^ (1 << d));
-        const int slot = hbase[d] + lanes_below(hball[d]);  // < kHaloCap (HR from the topology)
+        const int slot = PerDir ? lane : hbase[d] + lanes_below(hball[d]);
         const bool coarse_face = !inT && noD[d] < 0;
         int iout, sout, cout;  // neighbour share, stride over groups, cancel flag: leaving face
         if (inT) {
@@ -656,20 +666,20 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
         // 2F_p = A + X_p, with X the neighbours' shares and A = P + cU,
         // B = P - cU this cell's.  The update wm 2F_m + wp 2F_p then is
         // wm X_m + wp X_p + (wm + wp) P + (wp - wm) cU.  On complete levels
-        // (HR = 1) nearly every cell has wm = -wp (both faces carry flux, or
+        // nearly every cell has wm = -wp (both faces carry flux, or
         // neither does): the P term drops and the cU term is summed over the
         // directions and applied once, and only the rare cell with one
         // cancelled face or a coarse face reads its own shares back.  Sparse
         // levels, where coarse faces are common, keep the plain form.
         const float wm = (bit ? cancel_in : cancel_out) ? 0.f : 0.5f * dts;
         const float wp = (bit ? cancel_out : cancel_in) ? 0.f : -0.5f * dts;
-        const bool plain = HR == 1 && wm == -wp;
+        const bool plain = !PerDir && wm == -wp;
         if (plain) ucoef += wp - wm;
 #pragma unroll
         for (int g = 0; g < G; g++) {
           const float4 Xm = sShare[im + g * sm];
           const float4 Xp = sShare[ip + g * sp];
-          if (HR == 1) {
+          if (!PerDir) {
             De[g] = fmaf(Xp.x, wp, fmaf(Xm.x, wm, De[g]));
             Dx[g] = fmaf(Xp.y, wp, fmaf(Xm.y, wm, Dx[g]));
             Dy[g] = fmaf(Xp.z, wp, fmaf(Xm.z, wm, Dy[g]));
@@ -681,7 +691,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
             const float4 Bg = sShare[(g * 2 + 1) * kTileCells + lc];
             const float Fm[4] = {Xm.x + Bg.x, Xm.y + Bg.y, Xm.z + Bg.z, Xm.w + Bg.w};
             const float Fp[4] = {Ag.x + Xp.x, Ag.y + Xp.y, Ag.z + Xp.z, Ag.w + Xp.w};
-            if (HR != 1) {
+            if (PerDir) {
               De[g] = fmaf(Fp[0], wp, fmaf(Fm[0], wm, De[g]));
               Dx[g] = fmaf(Fp[1], wp, fmaf(Fm[1], wm, Dx[g]));
               Dy[g] = fmaf(Fp[2], wp, fmaf(Fm[2], wm, Dy[g]));
@@ -881,18 +891,18 @@ hipError_t launch_level_step(const LevelArgs<R>& a0, const ChemConst& K, hipStre
       int nb = (a.ncell + kTileCells - 1) / kTileCells;
       nb = (nb + 7) & ~7;
       a.nblocks = nb;
-      // sparse levels: three rounds of halo slots per wave
-      const bool sparse = a.halo_rounds > 1;
+      // sparse levels: per-direction halo faces
+      const bool sparse = a.halo_per_dir != 0;
       if (a.ng == 1) {
         if (sparse)
-          hipLaunchKernelGGL((level_step_tiled<1, 3>), dim3(nb), dim3(kTileCells), 0, s, a, K);
+          hipLaunchKernelGGL((level_step_tiled<1, true>), dim3(nb), dim3(kTileCells), 0, s, a, K);
         else
-          hipLaunchKernelGGL((level_step_tiled<1, 1>), dim3(nb), dim3(kTileCells), 0, s, a, K);
+          hipLaunchKernelGGL((level_step_tiled<1, false>), dim3(nb), dim3(kTileCells), 0, s, a, K);
       } else {
         if (sparse)
-          hipLaunchKernelGGL((level_step_tiled<3, 3>), dim3(nb), dim3(kTileCells), 0, s, a, K);
+          hipLaunchKernelGGL((level_step_tiled<3, true>), dim3(nb), dim3(kTileCells), 0, s, a, K);
         else
-          hipLaunchKernelGGL((level_step_tiled<3, 1>), dim3(nb), dim3(kTileCells), 0, s, a, K);
+          hipLaunchKernelGGL((level_step_tiled<3, false>), dim3(nb), dim3(kTileCells), 0, s, a, K);
       }
       return hipGetLastError();
     }

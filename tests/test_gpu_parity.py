@@ -148,6 +148,27 @@ def test_status_negative_photons(gpu_lib):
 
 
 @pytest.mark.parametrize("table,G", [("SB99_equiv_mono_TH", 1), ("Starburst99_1grpT_3grpS", 3)])
+def test_level_size_limit(gpu_lib, table, G):
+    """A level of more than min(2^29, (2^31-1)/ngrp) cells is refused before
+    any table is read or any memory allocated (32-bit cell indices and byte
+    offsets); the context stays usable."""
+    import ctypes
+    W = synth.uniform_workload(3, table, seed=1)
+    g = rt.RadGPU(W.params, rt.MIXED)
+    ip = ctypes.POINTER(ctypes.c_int)
+    dummy = np.zeros(8, dtype=np.int32)
+    limit = min(2**29, (2**31 - 1) // G)
+    for noct in (limit // 8 + 1, 2**28):
+        n = np.array([noct], dtype=np.int32)
+        st = g.lib.emma_rt_tree_create(g.h, 3, 3, n.ctypes.data_as(ip), dummy.ctypes.data_as(ip),
+                                       dummy.ctypes.data_as(ip), rt.MIXED)
+        assert st == rt.ERR_ARG, (noct, st)
+    g.setup(W.params, W.tree, W.fields)
+    g.radsolver(3, W.params.dt_rad(3))
+    g.close()
+
+
+@pytest.mark.parametrize("table,G", [("SB99_equiv_mono_TH", 1), ("Starburst99_1grpT_3grpS", 3)])
 @pytest.mark.parametrize("mode", ["host", "graph"])
 @pytest.mark.parametrize("precision", [rt.FP64, rt.MIXED])
 def test_coarserad(gpu_lib, oracle, table, G, mode, precision):
