@@ -936,7 +936,7 @@ extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
   if (st) return st;
   st = begin_call(c);
   if (st) return st;
-  if (c->timing) {  // eager launches with per-kernel events
+  if (c->timing == 1) {  // eager launches with per-kernel events
     for (int i = 0; i < ncoarse; i++) {
       st = recurse(c, 0, aexp);
       if (st) return st;
@@ -954,7 +954,10 @@ extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rs = EMMA_RT_OK;
+    const int timing = c->timing;
+    c->timing = 0;  // no event records inside the graph
     for (int i = 0; i < ncoarse && rs == EMMA_RT_OK; i++) rs = recurse(c, 0, aexp);
+    c->timing = timing;
     hipError_t ce = hipStreamEndCapture(c->stream, &g);
     if (rs) return rs;
     HIPCHK(ce);
@@ -969,7 +972,15 @@ extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
     it = c->graphs.emplace(key, std::make_pair(ex, endpar)).first;
     c->stats.level_steps = 0;
   }
-  HIPCHK(hipGraphLaunch(it->second.first, c->stream));
+  if (c->timing == 2) {  // one event pair around the replay
+    hipEvent_t e0 = next_event(c), e1 = next_event(c);
+    if (!e0 || !e1) return EMMA_RT_ERR_HIP;
+    HIPCHK(hipEventRecord(e0, c->stream));
+    HIPCHK(hipGraphLaunch(it->second.first, c->stream));
+    HIPCHK(hipEventRecord(e1, c->stream));
+  } else {
+    HIPCHK(hipGraphLaunch(it->second.first, c->stream));
+  }
   for (size_t k = 0; k < c->lev.size(); k++) c->lev[k].cur = it->second.second[k];
   long long steps = 0;
   for (int k = 0; k < (int)c->lev.size(); k++)
@@ -979,7 +990,7 @@ extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
 }
 
 extern "C" int emma_rt_enable_timing(emma_rt_ctx* c, int enable) {
-  if (c == nullptr) return EMMA_RT_ERR_ARG;
+  if (c == nullptr || enable < 0 || enable > 2) return EMMA_RT_ERR_ARG;
   c->timing = enable ? 1 : 0;
   return EMMA_RT_OK;
 }

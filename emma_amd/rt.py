@@ -47,6 +47,10 @@ MIXED = 1
 RADSTEP = 0
 COARSERAD = 1
 
+TIMING_OFF = 0
+TIMING_LAUNCH = 1
+TIMING_GRAPH = 2
+
 EXPORTS = [
     "emma_rt_countdevices", "emma_rt_initlocaldevice", "emma_rt_checkdevice", "emma_rt_strerror",
     "emma_rt_create", "emma_rt_set_params", "emma_rt_destroy",
@@ -233,8 +237,12 @@ class RadGPU:
     def set_tphy(self, tphy: float):
         self._check(self.lib.emma_rt_set_tphy(self.h, float(tphy)))
 
-    def enable_timing(self, on: bool = True):
-        self._check(self.lib.emma_rt_enable_timing(self.h, int(on)))
+    def enable_timing(self, mode: int = TIMING_LAUNCH):
+        """TIMING_OFF, TIMING_LAUNCH (events around each level-step kernel,
+        coarse steps launched eagerly) or TIMING_GRAPH (coarse steps replay
+        their graph between one pair of events); True/False map to
+        TIMING_LAUNCH/TIMING_OFF"""
+        self._check(self.lib.emma_rt_enable_timing(self.h, int(mode)))
 
     def set_mode(self, mode: int):
         """RADSTEP (per-level transport) or COARSERAD (coarse-level transport)"""

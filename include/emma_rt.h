@@ -83,7 +83,7 @@ typedef struct emma_rt_stats {
   long long cell_updates;   /* leaf cells advanced (one per leaf per level step) */
   long long chem_substeps;  /* accepted chemistry sub-steps, summed over cells */
   long long level_steps;    /* level-step kernel launches */
-  double kernel_ms;         /* summed device time of the level-step kernels (HIP events) */
+  double kernel_ms;         /* device time (HIP events), see emma_rt_enable_timing */
   double photons;           /* FP64 sum of N*dx^3 over leaf cells after the call (photon budget) */
   int status;               /* first error raised on the device, EMMA_RT_OK if none */
   int bad_level;            /* level and cell of that error */
@@ -201,8 +201,15 @@ int emma_rt_set_stromgren_source(emma_rt_ctx *ctx, int enable, double x, double 
                                  double lifetime, double srcint, double time_bound0);
 
 int emma_rt_get_stats(emma_rt_ctx *ctx, emma_rt_stats *out);
-/* record HIP events around every level-step kernel (adds the per-launch
-   device times to emma_rt_stats.kernel_ms) */
+/* device timing into emma_rt_stats.kernel_ms: EMMA_RT_TIMING_LAUNCH puts
+   HIP events around every level-step kernel (coarse steps then launch
+   eagerly, not from the captured graph); EMMA_RT_TIMING_GRAPH does the same
+   for single level steps, and coarse steps replay their graph between one
+   pair of events (the replay's device time, launch gaps inside the graph
+   included) */
+#define EMMA_RT_TIMING_OFF 0
+#define EMMA_RT_TIMING_LAUNCH 1
+#define EMMA_RT_TIMING_GRAPH 2
 int emma_rt_enable_timing(emma_rt_ctx *ctx, int enable);
 
 #ifdef __cplusplus

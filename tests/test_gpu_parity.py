@@ -281,3 +281,26 @@ def test_full_size_mixed_vs_fp64(gpu_lib, config):
         for k, (mx, nbad) in err.items():
             assert mx < 0.05, (l, k, mx)
             assert nbad <= max(5, n // 20000), (l, k, nbad)
+
+
+def test_timing_modes(gpu_lib):
+    """TIMING_GRAPH brackets a coarse step's graph replay with one event pair,
+    TIMING_LAUNCH every level-step kernel of an eager coarse step; both give
+    the same state, a positive device time and the same step counts."""
+    W = synth.uniform_workload(4, "SB99_equiv_mono_TH", seed=2)
+    out = {}
+    for mode in (rt.TIMING_LAUNCH, rt.TIMING_GRAPH):
+        g = rt.RadGPU(W.params, rt.MIXED)
+        g.setup(W.params, W.tree, W.fields)
+        g.coarse_step(1.0, 4)  # capture (graph mode) or eager
+        g.set(4, 0, W.fields[4])
+        g.enable_timing(mode)
+        g.coarse_step(1.0, 4)
+        st = g.stats()
+        assert st["kernel_ms"] > 0 and st["level_steps"] == 4, (mode, st)
+        out[mode] = g.get(4)
+        with pytest.raises(rt.EmmaRTError):
+            g.enable_timing(3)
+        g.close()
+    for k in Fields.NAMES:
+        assert np.array_equal(getattr(out[rt.TIMING_LAUNCH], k), getattr(out[rt.TIMING_GRAPH], k)), k
