@@ -35,7 +35,12 @@ __device__ __forceinline__ double fdiv64(double a, double b) {
   return fma(fma(-b, q, a), r, q);
 }
 
-// Levermore closure (rad_utils.c:448-475) with fast FP32 reciprocals
+// Levermore closure (rad_utils.c:448-475) with fast FP32 reciprocals.
+// With s = sqrt(4 - 3 f^2) (f^2 <= 4/3), chi = (3 + 4 f^2) / (5 + 2 s) equals
+// (5 - 2 s) / 3 (multiply by 5 - 2 s: the denominator becomes
+// 25 - 4 (4 - 3 f^2) = 3 (3 + 4 f^2)), so (1 - chi)/2 = (s - 1)/3 and
+// (3 chi - 1)/2 = 2 - s need no division.  Beyond f^2 = 4/3 the reference
+// clamps the root to 0 and chi = (3 + 4 f^2) / 5.
 __device__ __forceinline__ Closure<float> closure_fast(float fx, float fy, float fz, float ee, float c) {
 #pragma clang fp contract(fast)
   // branch-free: the isotropic case (E <= 0 or F = 0: n = 0, f = 0) by selects
@@ -49,10 +54,10 @@ __device__ __forceinline__ Closure<float> closure_fast(float fx, float fy, float
   q.n[2] = fz * ri;
   const float ff = aniso ? f2 * ri * frcp32(c * ee) : 0.f;
   const float ff2 = ff * ff;
-  const float arg = fmaxf(4.f - 3.f * ff2, 0.f);
-  const float chi = (3.f + 4.f * ff2) * frcp32(5.f + 2.f * __builtin_amdgcn_sqrtf(arg));
-  q.diag = 0.5f * (1.f - chi) * c2e;
-  q.arg = 0.5f * (3.f * chi - 1.f) * c2e;
+  const float s = __builtin_amdgcn_sqrtf(fmaxf(4.f - 3.f * ff2, 0.f));
+  const bool clamped = ff2 > 4.f / 3.f;
+  q.diag = c2e * (clamped ? 0.2f * (1.f - 2.f * ff2) : (s - 1.f) * (1.f / 3.f));
+  q.arg = c2e * (clamped ? 0.2f * (2.f + 6.f * ff2) : 2.f - s);
   return q;
 }
 
