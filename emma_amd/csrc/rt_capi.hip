@@ -563,6 +563,7 @@ static LevelArgs<R> level_args(emma_rt_ctx* c, int k, double dtnew, double aexp,
   A.child = L.child;
   A.nei = L.nei;
   A.racc = (R*)L.racc;
+  A.nslot = L.nslot;
   if (k > 0) {
     DevLevel& C = c->lev[k - 1];
     A.ncoarse = C.ncell;
@@ -572,6 +573,7 @@ static LevelArgs<R> level_args(emma_rt_ctx* c, int k, double dtnew, double aexp,
     A.cfz = (const R*)C.fz[C.cur];
     A.cchild = C.child;
     A.cracc = (R*)C.racc;
+    A.cnslot = C.nslot;
   }
   if (k + 1 < (int)c->lev.size()) {
     DevLevel& F = c->lev[k + 1];

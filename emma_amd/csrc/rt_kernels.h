@@ -12,7 +12,8 @@ namespace emma {
 template <typename R>
 struct LevelArgs {
   int ncell, ncoarse, nfine, ng;
-  int level, chemonly, nblocks, pad;
+  int level, chemonly, nblocks;
+  int nslot;   // reflux slots of this level (stride of racc)
   // RADSTEP: transport=1, keep_split_flux=0, update_split=0.  COARSERAD
   // (rad_utils.c:518-526,1548-1552,1808-1812): transport only on the coarse
   // level and not for chemonly calls, fluxes of split neighbours kept, split
@@ -21,6 +22,7 @@ struct LevelArgs {
   // mixed path: 1 when some wave of the level has more than 64 halo faces
   // (sparse levels: the per-direction halo variant of level_step_tiled)
   int halo_per_dir;
+  int cnslot;  // reflux slots of the coarser level (stride of cracc)
   // current state (read)
   const R* e;
   const R* fx;
@@ -43,7 +45,7 @@ struct LevelArgs {
   // topology: child[cell] >= 0 split, -1 leaf, <= -2 leaf with reflux slot -2-child
   const int* child;
   const int* nei;
-  R* racc;  // this level's reflux slots [nslot][4][GA]
+  R* racc;  // this level's reflux slots, component-major: [4][GA][nslot]
   // coarser level: current state (injection), child table, reflux slots
   const R* ce;
   const R* cfx;

@@ -152,17 +152,20 @@ __device__ __forceinline__ void restrict_cell(const LevelArgs<R>& a, int cell, i
 }
 
 // reflux of one face flux into the coarse leaf j (scatterstencilrad,
-// rad_utils.c:1738-1762): the coarse cell gains -/+ F dt/dx / 8
+// rad_utils.c:1738-1762): the coarse cell gains -/+ F dt/dx / 8.  The slots
+// are component-major ([4][GA][nslot]): the atomics of neighbouring faces
+// (consecutive slots) share cache lines, those of one face do not.
 template <typename R, int GA>
 __device__ __forceinline__ void reflux(const LevelArgs<R>& a, int j, int f, int g, const R F[4], R dts) {
   const int cch = a.cchild[j];
   if (cch <= -2) {
-    R* acc = a.cracc + (size_t)(-2 - cch) * 4 * GA;
+    R* acc = a.cracc + (-2 - cch);
+    const size_t st = (size_t)a.cnslot;
     const R w = (f & 1) ? R(1) : R(-1);  // pow(-1, face+1)
-    atomicAdd(acc + 0 * GA + g, F[0] * dts * w * R(0.125));
-    atomicAdd(acc + 1 * GA + g, F[1] * dts * w * R(0.125));
-    atomicAdd(acc + 2 * GA + g, F[2] * dts * w * R(0.125));
-    atomicAdd(acc + 3 * GA + g, F[3] * dts * w * R(0.125));
+    atomicAdd(acc + (0 * GA + g) * st, F[0] * dts * w * R(0.125));
+    atomicAdd(acc + (1 * GA + g) * st, F[1] * dts * w * R(0.125));
+    atomicAdd(acc + (2 * GA + g) * st, F[2] * dts * w * R(0.125));
+    atomicAdd(acc + (3 * GA + g) * st, F[3] * dts * w * R(0.125));
   }
 }
 
@@ -178,18 +181,19 @@ __device__ __forceinline__ int finish_leaf(const LevelArgs<R>& a, const ChemCons
   constexpr int GA = G > 0 ? G : kMaxGrp;
   const int n = a.ncell;
   if (ch <= -2) {
-    R* acc = a.racc + (size_t)(-2 - ch) * 4 * GA;
+    R* acc = a.racc + (-2 - ch);
+    const size_t st = (size_t)a.nslot;
 #pragma unroll
     for (int g = 0; g < GA; g++) {
       if (g < ng) {
-        Ue[g] += acc[0 * GA + g];
-        Ux[g] += acc[1 * GA + g];
-        Uy[g] += acc[2 * GA + g];
-        Uz[g] += acc[3 * GA + g];
-        acc[0 * GA + g] = R(0);
-        acc[1 * GA + g] = R(0);
-        acc[2 * GA + g] = R(0);
-        acc[3 * GA + g] = R(0);
+        Ue[g] += acc[(0 * GA + g) * st];
+        Ux[g] += acc[(1 * GA + g) * st];
+        Uy[g] += acc[(2 * GA + g) * st];
+        Uz[g] += acc[(3 * GA + g) * st];
+        acc[(0 * GA + g) * st] = R(0);
+        acc[(1 * GA + g) * st] = R(0);
+        acc[(2 * GA + g) * st] = R(0);
+        acc[(3 * GA + g) * st] = R(0);
       }
     }
   }
@@ -712,8 +716,10 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
       }
       // Refluxing (scatterstencilrad, rad_utils.c:1738-1762): the four cells
       // of an oct face share its coarse neighbour, so their fluxes are summed
-      // across lanes first and one lane per face adds them to the coarse
-      // cell's accumulator (a quarter of the atomics, none contending)
+      // across lanes first and each of the four lanes then adds one flux
+      // component (E, Fx, Fy, Fz) of every group to the coarse cell's
+      // accumulator: a quarter of the atomics, none contending, issued by
+      // all lanes of the face
       const unsigned long long rmask = __ballot(rj >= 0);
       if (rmask != 0ull) {
         const int b1 = 1 << ((d + 1) % 3), b2 = 1 << ((d + 2) % 3);
@@ -730,15 +736,23 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
         const unsigned long long grp =
             (1ull << l0) | (1ull << (l0 | b1)) | (1ull << (l0 | b2)) | (1ull << (l0 | b1 | b2));
         const unsigned long long mine = rmask & grp;
-        if (rj >= 0 && lane == __builtin_ctzll(mine)) {
-          const int cch = cchD[d];
-          if (cch <= -2) {
-            float* acc = a.cracc + (size_t)(-2 - cch) * 4 * G;
-            const float w = (bit ? 1.f : -1.f) * (0.5f * dts) * 0.125f;  // pow(-1, face+1) dt/dx / 8, shares doubled
+        const int cch = cchD[d];
+        if (rj >= 0 && cch <= -2) {
+          float* acc = a.cracc + (-2 - cch);
+          const size_t st = (size_t)a.cnslot;
+          const float w = (bit ? 1.f : -1.f) * (0.5f * dts) * 0.125f;  // pow(-1, face+1) dt/dx / 8, shares doubled
+          if (mine == grp) {  // the face's four cells all carry flux (always, with 2:1 balance)
+            const int k = __popcll(mine & ((1ull << lane) - 1ull));  // component of this lane
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+              const float x = k == 0 ? Rf[g][0] : (k == 1 ? Rf[g][1] : (k == 2 ? Rf[g][2] : Rf[g][3]));
+              atomicAdd(acc + (k * G + g) * st, x * w);
+            }
+          } else if (lane == __builtin_ctzll(mine)) {
 #pragma unroll
             for (int g = 0; g < G; g++)
 #pragma unroll
-              for (int v = 0; v < 4; v++) atomicAdd(acc + v * G + g, Rf[g][v] * w);
+              for (int v = 0; v < 4; v++) atomicAdd(acc + (v * G + g) * st, Rf[g][v] * w);
           }
         }
       }
