@@ -453,6 +453,18 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value
 // more LDS than the complete-level kernel.  The host picks the variant from
 // the level's topology (the most halo faces in any wave).
 
+// the value of lane (lane ^ m), m = 1, 2 or 4 (wave-uniform), through DPP
+// lane moves within rows of 16 (VALU, no LDS): quad permutations for 1 and
+// 2, a half-row mirror (lane ^ 7) followed by a quad reversal (lane ^ 3) for
+// 4.  Every lane of the wave must be active.
+__device__ __forceinline__ float lane_xor(float x, int m) {
+  const int v = __float_as_int(x);
+  if (m == 1) return __int_as_float(__builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));  // quad_perm 1,0,3,2
+  if (m == 2) return __int_as_float(__builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));  // quad_perm 2,3,0,1
+  const int t = __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true);                       // row_half_mirror
+  return __int_as_float(__builtin_amdgcn_mov_dpp(t, 0x1B, 0xF, 0xF, true));              // quad_perm 3,2,1,0
+}
+
 __device__ __forceinline__ int lanes_below(unsigned long long m) {
   return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
@@ -728,8 +740,8 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
           for (int v = 0; v < 4; v++) {
             float x = Rf[g][v];
-            x += __shfl_xor(x, b1, 64);
-            x += __shfl_xor(x, b2, 64);
+            x += lane_xor(x, b1);
+            x += lane_xor(x, b2);
             Rf[g][v] = x;
           }
         const int l0 = lane & ~(b1 | b2);
