@@ -304,3 +304,57 @@ def test_timing_modes(gpu_lib):
         g.close()
     for k in Fields.NAMES:
         assert np.array_equal(getattr(out[rt.TIMING_LAUNCH], k), getattr(out[rt.TIMING_GRAPH], k)), k
+
+
+def refined_block_workload(table, seed):
+    """Level 3 complete, its 4^3 cells at the origin refined into a block of
+    64 level-4 octs (two 32-oct tiles): every wave of level 4 has at most 64
+    halo faces, so the level runs the complete-level kernel variant, and the
+    waves on the block's surface have coarse faces (injection, refluxing)."""
+    from emma_amd import tree as T
+    from emma_amd.params import make_params
+    rng = np.random.default_rng(seed)
+    L3 = T.uniform_level(3)
+    split = np.nonzero(np.all(L3.cell_pos() < 4, axis=1))[0]
+    L4 = T.refine(L3, split)
+    tr = T.build_tree([L3, L4])
+    params = make_params(table, clight=0.1)
+    fields = {}
+    for l in (3, 4):
+        L = tr.level(l)
+        fields[l] = synth.random_fields(params, np.exp(rng.normal(0.0, 1.0, L.ncell)), rng)
+    synth.place_sources(params, fields[4], tr.level(4), 3, np.array([1e49, 1e50, 3e50]), rng)
+    return synth.Workload("refined-block", params, tr, fields, 1)
+
+
+def halo_faces_per_wave(L):
+    """leaving faces per wave of 64 cells whose neighbour lies outside the
+    cell's 32-oct tile, and whether any of them is a coarse face"""
+    nei = L.nei_oct.reshape(-1, 6)
+    cell = np.arange(L.ncell)
+    oct, ic, oct0 = cell >> 3, cell & 7, (cell // 256) * 32
+    cnt = np.zeros(L.ncell, int)
+    coarse = np.zeros(L.ncell, bool)
+    for d in range(3):
+        no = nei[oct, 2 * d + ((ic >> d) & 1)]
+        cnt += ~((no >= oct0) & (no < oct0 + 32))
+        coarse |= no < 0
+    return cnt.reshape(-1, 64).sum(1), coarse.reshape(-1, 64).any(1)
+
+
+@pytest.mark.parametrize("table,G", [("SB99_equiv_mono_TH", 1), ("Starburst99_1grpT_3grpS", 3)])
+@pytest.mark.parametrize("precision", [rt.FP64, rt.MIXED])
+def test_refined_block_complete_level_kernel(gpu_lib, oracle, table, G, precision):
+    W = refined_block_workload(table, seed=31 + G)
+    per_wave, has_coarse = halo_faces_per_wave(W.tree.level(4))
+    assert per_wave.max() <= 64 and has_coarse.any()
+    g = run_pair(oracle, W, precision, steps=3, mode="graph")
+    c = W.params.cloc()
+    for l in (3, 4):
+        err = field_errors(g.get(l), oracle.get(l), c)
+        print("refined block G=%d prec=%d level %d" % (G, precision, l), err)
+        if precision == rt.FP64:
+            assert max(err.values()) < FP64_TOL, (l, err)
+        else:
+            assert err["x"] < MIXED_TOL and err["temp"] < MIXED_TOL and err["e"] < MIXED_TOL, (l, err)
+            assert max(err["fx"], err["fy"], err["fz"]) < MIXED_TOL, (l, err)
