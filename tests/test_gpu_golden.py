@@ -232,3 +232,28 @@ def test_stromgren_device_source(gpu_lib):
     rec = nH**2 * np.sum(ab * x * x) * dv
     print(f"t={t:.0f} Myr recombinations/emission = {rec / ndot:.4f}")
     assert abs(rec / ndot - 1) < 0.05
+
+
+@pytest.mark.parametrize("G", [1, 3])
+def test_amr_golden_mixed_xion_times(gpu_lib, G):
+    """The mixed path keeps the reionisation-time bookkeeping (rad_utils.c:
+    1901-1913): x_HII within the mixed bar of the reference after three coarse
+    steps, and t_first / t_last equal to the reference's except in cells whose
+    x_HII sits within that bar of the 0.5 threshold at the end."""
+    z = gio.load(f"amr_g{G}.npz")
+    tr = gio.amr_tree(z)
+    p = gio.params(G)
+    g = rt.RadGPU(p, rt.MIXED)
+    g.enable_xion_times(True)
+    g.setup(p, tr, {l: gio.fields(z, f"in{l}_") for l in range(tr.lmin, tr.lmax + 1)})
+    for tphy in z["tphy"]:
+        g.set_tphy(float(tphy))
+        g.coarse_step(1.0, 1)
+    for l in range(tr.lmin, tr.lmax + 1):
+        F, R = g.get(l), gio.fields(z, f"out{l}_")
+        x, xr = F.nhplus / F.nh, R.nhplus / R.nh
+        assert np.max(np.abs(x - xr) / np.abs(xr)) < 1e-4, l
+        tf, tl = g.xion_times(l)
+        near = np.abs(xr - 0.5) < 1e-4
+        assert np.array_equal(tf[~near], z[f"tfirst{l}"][~near]), l
+        assert np.array_equal(tl[~near], z[f"tlast{l}"][~near]), l
