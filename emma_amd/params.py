@@ -44,6 +44,18 @@ ATOMIC_TABLES = {
         (32.5864317739, 6.62279605918e-23, 6.06571839078e-23, 0.230075891068),
         (55.6621660507, 1.16142701989e-23, 1.15856478863e-23, 1.76194762129e-07),
     ],
+    # Starburst99_3grpT_3grpS.dat: three time groups of three frequency groups
+    "Starburst99_3grpT_3grpS": [
+        (18.0960837205, 3.33247782214e-22, 3.1029498972e-22, 0.711623692435),
+        (32.7734105767, 6.54066737569e-23, 5.97338219337e-23, 0.288376214572),
+        (58.0179019951, 1.03517562247e-23, 1.02588318683e-23, 9.29939343807e-08),
+        (17.5908242307, 3.56894560696e-22, 3.34329166724e-22, 0.923901199349),
+        (31.0925705306, 7.27896047138e-23, 6.80343540065e-23, 0.0760984046781),
+        (54.6277040944, 1.21686713315e-23, 1.21682856377e-23, 3.95972432576e-07),
+        (15.1064199164, 4.9465484405e-22, 4.85759793145e-22, 0.999990281992),
+        (25.8970145725, 1.08028137438e-22, 1.07501523182e-22, 9.71800798434e-06),
+        (77.9786617129, 5.72753801219e-24, 4.92806933216e-24, 2.87618856869e-51),
+    ],
 }
 
 

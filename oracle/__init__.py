@@ -26,7 +26,8 @@ from emma_amd.synth import Fields
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "_build", "libemma_oracle.so")
-REF_SO = {1: os.path.join(HERE, "_ref", "libemma_ref_g1.so"), 3: os.path.join(HERE, "_ref", "libemma_ref_g3.so")}
+REF_SO = {1: os.path.join(HERE, "_ref", "libemma_ref_g1.so"), 3: os.path.join(HERE, "_ref", "libemma_ref_g3.so"),
+          9: os.path.join(HERE, "_ref", "libemma_ref_g9.so")}
 REF_SO_CR = {1: os.path.join(HERE, "_ref", "libemma_ref_g1_cr.so"), 3: os.path.join(HERE, "_ref", "libemma_ref_g3_cr.so")}
 REF_SO_HYD = {1: os.path.join(HERE, "_ref", "libemma_ref_g1_hyd.so"),
               3: os.path.join(HERE, "_ref", "libemma_ref_g3_hyd.so")}

@@ -358,3 +358,23 @@ def test_refined_block_complete_level_kernel(gpu_lib, oracle, table, G, precisio
         else:
             assert err["x"] < MIXED_TOL and err["temp"] < MIXED_TOL and err["e"] < MIXED_TOL, (l, err)
             assert max(err["fx"], err["fy"], err["fz"]) < MIXED_TOL, (l, err)
+
+
+@pytest.mark.parametrize("precision", [rt.FP64, rt.MIXED])
+@pytest.mark.parametrize("mode", ["radsolver", "graph"])
+def test_nine_groups(gpu_lib, oracle, precision, mode):
+    """Nine photon groups (Starburst99_3grpT_3grpS: three time groups of three
+    frequency groups) run the kernels' generic group count; the oracle agrees
+    with the reference's NGRP=9 build bit for bit (test_oracle_ref.py)."""
+    W = synth.amr_workload(3, [60, 100, 80], "Starburst99_3grpT_3grpS", seed=13, sigma=1.5, nsrc=3, grid=16)
+    assert W.params.ngrp == 9
+    g = run_pair(oracle, W, precision, steps=2, mode=mode)
+    c = W.params.cloc()
+    for l in range(W.tree.lmin, W.tree.lmax + 1):
+        err = field_errors(g.get(l), oracle.get(l), c)
+        print("nine groups prec=%d %s level %d" % (precision, mode, l), err)
+        if precision == rt.FP64:
+            assert max(err.values()) < FP64_TOL, (l, err)
+        else:
+            assert err["x"] < MIXED_TOL and err["temp"] < MIXED_TOL and err["e"] < MIXED_TOL, (l, err)
+            assert max(err["fx"], err["fy"], err["fz"]) < MIXED_TOL, (l, err)

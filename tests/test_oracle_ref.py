@@ -9,7 +9,7 @@ from emma_amd.params import make_params
 from emma_amd.synth import Fields
 from oracle import Oracle, Reference, ref_available
 
-TABLES = {1: "SB99_equiv_mono_TH", 3: "Starburst99_1grpT_3grpS"}
+TABLES = {1: "SB99_equiv_mono_TH", 3: "Starburst99_1grpT_3grpS", 9: "Starburst99_3grpT_3grpS"}
 
 
 def need(G):
@@ -26,7 +26,7 @@ def same_state(o, r, tree):
                 assert np.array_equal(a, b), (l, which, k, np.count_nonzero(a != b))
 
 
-@pytest.mark.parametrize("G", [1, 3])
+@pytest.mark.parametrize("G", [1, 3, 9])
 @pytest.mark.parametrize("seed", [1, 2])
 def test_amr_coarse_steps(G, seed):
     need(G)
