@@ -255,7 +255,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the radiation step on one GPU's shard "
+        print("bench.py measures one GPU: the library runs the radiation step on one GPU's shard "
               "(the multi-GPU halo exchange is not part of it)", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
