@@ -27,6 +27,11 @@ reference's own sources) or, where that build is absent, the oracle (its CPU
 restatement), on up to 16 host cores, one process each, on a bounded sample
 drawn like the workload.
 
+--stencil-path measures the drop-in path instead (advanceradGPU's contract: host
+struct RGRID batches in, rfieldnew and fluxes back) on the 64^3, 1-group level in
+batches of 8192 octs, --steps sweeps over the level after --warmup; its line
+carries the PCIe-inclusive rate under its own metric name.
+
 --calibrate-copy runs a plain device copy of known size (for scaling the
 FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc runs).
 """
@@ -225,6 +230,58 @@ def run_amr(W, precision, steps, warmup):
     return wall, tot, kernel_ms * 1e-3, per_coarse * steps, tot["level_steps"]
 
 
+def stencil_path(steps: int, warmup: int, stride: int = 8192) -> dict:
+    """The drop-in path on the 64^3 level: every oct gathered on the host into
+    struct RGRID batches of `stride` octs (untimed), then per sweep over the
+    level one emma_rt_advance_stencil call per batch (host -> device copy,
+    sweeps + update + chemistry, device -> host copy of the New part), timed
+    by the wall clock.  The batch handed over lives in page-locked memory
+    (emma_rt_create_pinned_stencil, as EMMA's create_pinned_stencil_rad)."""
+    import numpy as np
+    from emma_amd import rgrid, rt, synth
+
+    W = synth.config_c2()
+    lvl = W.tree.lmin
+    L, F = W.tree.level(lvl), W.fields[lvl]
+    batches = [rgrid.gather_complete(L, F, np.arange(o, min(L.noct, o + stride)), W.params.ngrp)
+               for o in range(0, L.noct, stride)]
+    g = rt.RadGPU(W.params, rt.FP64)
+    g.create_radstencil(stride, 8)
+    pinned = g.pinned_stencil(stride, 8)
+    dt = W.params.dt_rad(lvl)
+
+    def sweep(timed):
+        t = 0.0
+        for b in batches:
+            w = pinned.array[: b.shape[0]]
+            w[...] = b  # every sweep starts from the gathered state
+            t0 = time.perf_counter()
+            g.advance_stencil(w, w.shape[0], 0.5**lvl, dt)
+            t += time.perf_counter() - t0
+        return t
+
+    for _ in range(warmup):
+        sweep(False)
+    wall = sum(sweep(True) for _ in range(steps))
+    pinned.free()
+    cells = L.ncell * steps
+    return {
+        "metric": "drop-in stencil path cell-updates/s (host RGRID batches, PCIe-inclusive)",
+        "value": cells / wall,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": wall * 1e3 / steps,
+        "higher_is_better": True,
+        "dtype": "fp64",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD_NAMES["c2"] + ", struct RGRID batches of %d octs" % stride,
+                   "bytes_per_oct_to_device": int(rgrid.rgrid_dtype(W.params.ngrp).itemsize),
+                   "bytes_per_oct_to_host": int(rgrid.rgrid_dtype(W.params.ngrp)["New"].itemsize)},
+    }
+
+
 def copy_calibration(nbytes: int = 1 << 30, reps: int = 10):
     import torch
 
@@ -253,11 +310,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-plain", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--stencil-path", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the library runs the radiation step on one GPU's shard "
               "(the multi-GPU halo exchange is not part of it)", file=sys.stderr)
         sys.exit(2)
+    if args.stencil_path:
+        print(json.dumps(stencil_path(args.steps, args.warmup)))
+        return
     if args.calibrate_copy:
         copy_calibration()
         return

@@ -1055,6 +1055,21 @@ extern "C" int emma_rt_destroy_radstencil(emma_rt_ctx* c) {
   return EMMA_RT_OK;
 }
 
+extern "C" int emma_rt_create_pinned_stencil(emma_rt_ctx* c, int stride, int real_bytes, void** out) {
+  if (c == nullptr || out == nullptr || stride <= 0 || (real_bytes != 4 && real_bytes != 8)) return EMMA_RT_ERR_ARG;
+  RgridLayout L;
+  if (!rgrid_layout(c->p.ngrp, real_bytes, &L)) return EMMA_RT_ERR_ARG;
+  *out = nullptr;
+  HIPCHK(hipHostMalloc(out, (size_t)stride * L.v[19], hipHostMallocDefault));
+  memset(*out, 0, (size_t)stride * L.v[19]);
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_destroy_pinned_stencil(void* stencil) {
+  if (stencil != nullptr) HIPCHK(hipHostFree(stencil));
+  return EMMA_RT_OK;
+}
+
 extern "C" int emma_rt_advance_stencil(emma_rt_ctx* c, void* stencil, int nread, double dxcur, double dtnew,
                                        double aexp, int chemonly) {
   if (c == nullptr || stencil == nullptr || nread < 0) return EMMA_RT_ERR_ARG;
@@ -1071,6 +1086,9 @@ extern "C" int emma_rt_advance_stencil(emma_rt_ctx* c, void* stencil, int nread,
   const double cl = cloc_of(c->p, aexp);
   HIPCHK(launch_stencil(c->d_stencil, L, c->p.ngrp, c->stencil_rbytes, nread, cl, dtnew / dxcur, chemonly, K,
                         c->d_status, c->d_counters, c->stream));
-  HIPCHK(hipMemcpyAsync(stencil, c->d_stencil, bytes, hipMemcpyDeviceToHost, c->stream));
+  // only New (rfieldnew, rflux) changes: copy back that part of each record
+  const size_t rsize = (size_t)L.v[19], noff = (size_t)L.v[21], nsize = (size_t)L.v[18];
+  HIPCHK(hipMemcpy2DAsync((char*)stencil + noff, rsize, (const char*)c->d_stencil + noff, rsize, nsize, (size_t)nread,
+                          hipMemcpyDeviceToHost, c->stream));
   return end_call(c);
 }

@@ -48,3 +48,33 @@ def empty(nread: int, G: int, rb: int = 8) -> np.ndarray:
 def flux_view(st: np.ndarray, G: int) -> np.ndarray:
     """rflux as [nread, 8 cells, 6 faces, G, 5 variables]."""
     return st["New"]["rflux"].reshape(st.shape[0], 8, 6, G, 5)
+
+
+def gather_complete(L, F, octs: np.ndarray, G: int, rb: int = 8) -> np.ndarray:
+    """The host gather (gatherstencilrad, rad_utils.c:1482-1534) for octs of
+    a complete periodic level: slot 6 holds the oct's own cells, slots 0..5
+    the cells of the same-level neighbour oct across each face
+    (recursive_neighbor_gather_oct_rad, rad_utils.c:1413-1477, with every
+    neighbour present), ``split`` marks split cells, and ``New`` starts as
+    the centre cells' state (rfieldnew = rfield after clean_new_rad) with zero
+    fluxes.  L is the level (``nei_oct``, ``child``), F its fields."""
+    octs = np.asarray(octs, dtype=np.int64)
+    n = octs.shape[0]
+    nei = L.nei_oct.reshape(-1, 6)
+    if (nei[octs] < 0).any():
+        raise ValueError("gather_complete: an oct has a coarse neighbour")
+    slot_oct = np.empty((n, 7), dtype=np.int64)
+    slot_oct[:, :6] = nei[octs]
+    slot_oct[:, 6] = octs
+    cells = slot_oct[:, :, None] * 8 + np.arange(8)[None, None, :]  # [n, 7, 8]
+    st = empty(n, G, rb)
+    R = st["oct"]["rfield"]
+    for k in ("e", "fx", "fy", "fz", "src"):
+        R[k] = np.moveaxis(getattr(F, k)[:, cells], 0, -1)
+    for k in ("nhplus", "eint", "nh", "temp"):
+        R[k] = getattr(F, k)[cells]
+    st["oct"]["split"] = (L.child[cells] >= 0).astype(np.int8)
+    centre = R[:, 6, :]
+    st["New"]["rfield"] = centre
+    st["New"]["rfieldnew"] = centre
+    return st

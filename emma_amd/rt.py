@@ -55,7 +55,7 @@ EXPORTS = [
     "emma_rt_countdevices", "emma_rt_initlocaldevice", "emma_rt_checkdevice", "emma_rt_strerror",
     "emma_rt_create", "emma_rt_set_params", "emma_rt_destroy",
     "emma_rt_rgrid_layout", "emma_rt_check_layout", "emma_rt_create_radstencil", "emma_rt_destroy_radstencil",
-    "emma_rt_advance_stencil",
+    "emma_rt_advance_stencil", "emma_rt_create_pinned_stencil", "emma_rt_destroy_pinned_stencil",
     "emma_rt_tree_create", "emma_rt_set_level", "emma_rt_get_level", "emma_rt_enable_xion_times",
     "emma_rt_get_xion_times", "emma_rt_set_tphy", "emma_rt_radsolver", "emma_rt_coarse_step",
     "emma_rt_get_stats", "emma_rt_enable_timing", "emma_rt_set_mode", "emma_rt_fillrad",
@@ -115,6 +115,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.emma_rt_destroy_radstencil.argtypes = [ctypes.c_void_p]
     L.emma_rt_advance_stencil.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    L.emma_rt_create_pinned_stencil.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                ctypes.POINTER(ctypes.c_void_p)]
+    L.emma_rt_destroy_pinned_stencil.argtypes = [ctypes.c_void_p]
     L.emma_rt_tree_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, ctypes.c_int]
     L.emma_rt_set_level.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_dp] * 9
     L.emma_rt_get_level.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_dp] * 9
@@ -156,6 +159,24 @@ def device_count() -> int:
 
 def _d(a):
     return a.ctypes.data_as(_dp)
+
+
+class PinnedStencil:
+    """struct RGRID records in page-locked host memory, viewed as a numpy
+    array (``array``); ``free`` releases the memory (destroy_pinned_stencil_rad)."""
+
+    def __init__(self, lib, ptr, stride: int, ngrp: int, real_bytes: int):
+        from . import rgrid
+        self.lib, self.ptr = lib, ptr
+        dt = rgrid.rgrid_dtype(ngrp, real_bytes)
+        buf = (ctypes.c_uint8 * (stride * dt.itemsize)).from_address(ptr.value)
+        self.array = np.frombuffer(buf, dtype=dt, count=stride)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            self.lib.emma_rt_destroy_pinned_stencil(self.ptr)
+            self.ptr = None
 
 
 class RadGPU:
@@ -303,6 +324,12 @@ class RadGPU:
     # ---- stencil path
     def create_radstencil(self, stride: int, real_bytes: int = 8):
         self._check(self.lib.emma_rt_create_radstencil(self.h, stride, real_bytes))
+
+    def pinned_stencil(self, stride: int, real_bytes: int = 8) -> "PinnedStencil":
+        """page-locked host buffer of `stride` RGRIDs (create_pinned_stencil_rad)"""
+        p = ctypes.c_void_p()
+        self._check(self.lib.emma_rt_create_pinned_stencil(self.h, stride, real_bytes, ctypes.byref(p)))
+        return PinnedStencil(self.lib, p, stride, self.params.ngrp, real_bytes)
 
     def advance_stencil(self, stencil: np.ndarray, nread: int, dxcur: float, dtnew: float, aexp: float = 1.0,
                         chemonly: int = 0):

@@ -118,10 +118,17 @@ int emma_rt_check_layout(int ngrp, int real_bytes, const long *layout22);
    RGRIDs of REAL = double (real_bytes 8) or float (4). */
 int emma_rt_create_radstencil(emma_rt_ctx *ctx, int stride, int real_bytes);
 int emma_rt_destroy_radstencil(emma_rt_ctx *ctx);
+/* create_pinned_stencil_rad / destroy_pinned_stencil_rad (rad_utils_gpu.cu:
+   39-43, 51-53): page-locked host memory for `stride` RGRIDs of the context's
+   ngrp (zeroed), so the copies of emma_rt_advance_stencil run at DMA speed. */
+int emma_rt_create_pinned_stencil(emma_rt_ctx *ctx, int stride, int real_bytes, void **out);
+int emma_rt_destroy_pinned_stencil(void *stencil);
 /* rad_utils_gpu.cu:1054-1086: copy `nread` RGRIDs host->device, run
    rad_sweepX/Y/Z + updatefieldrad + chemrad semantics (CPU semantics of
-   rad_utils.c/chem_utils.c), copy back.  `stencil` is host memory in the
-   reference's RGRID layout; only New.cell[*].rfieldnew and rflux change. */
+   rad_utils.c/chem_utils.c), copy back the New part of each RGRID.  `stencil`
+   is host memory in the reference's RGRID layout (page-locked memory from
+   emma_rt_create_pinned_stencil is fastest); only New.cell[*].rfieldnew and
+   rflux change. */
 int emma_rt_advance_stencil(emma_rt_ctx *ctx, void *stencil, int nread, double dxcur, double dtnew, double aexp,
                             int chemonly);
 

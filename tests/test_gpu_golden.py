@@ -257,3 +257,58 @@ def test_amr_golden_mixed_xion_times(gpu_lib, G):
         near = np.abs(xr - 0.5) < 1e-4
         assert np.array_equal(tf[~near], z[f"tfirst{l}"][~near]), l
         assert np.array_equal(tl[~near], z[f"tlast{l}"][~near]), l
+
+
+@pytest.mark.parametrize("G", [1, 3])
+def test_stencil_path_matches_resident_path(gpu_lib, G):
+    """The drop-in path on host-gathered RGRID batches of a complete periodic
+    level (gatherstencilrad's layout, rgrid.gather_complete) gives the same
+    next state in New.rfieldnew as the resident path's level step, both in
+    the reference's FP64 operation order."""
+    W = synth.uniform_workload(4, "SB99_equiv_mono_TH" if G == 1 else "Starburst99_1grpT_3grpS", seed=40 + G, nsrc=4)
+    L, F0 = W.tree.level(4), W.fields[4]
+    dt = W.params.dt_rad(4)
+    g = rt.RadGPU(W.params, rt.FP64)
+    g.setup(W.params, W.tree, W.fields)
+    g.radsolver(4, dt)
+    F1 = g.get(4)
+    s = rt.RadGPU(W.params, rt.FP64)
+    stride = 128
+    s.create_radstencil(stride, 8)
+    new = []
+    for o0 in range(0, L.noct, stride):
+        st = rgrid.gather_complete(L, F0, np.arange(o0, min(L.noct, o0 + stride)), G)
+        s.advance_stencil(st, st.shape[0], 0.5**4, dt)
+        new.append(st["New"]["rfieldnew"])
+    R = np.concatenate(new)  # [noct, 8] in cell order
+    c = W.params.cloc()
+    for k in ("e", "fx", "fy", "fz"):
+        a = R[k].reshape(-1, G).T
+        b = getattr(F1, k)
+        sc = np.maximum(np.abs(F1.e), 1e-300) * (1.0 if k == "e" else c)
+        assert np.max(np.abs(a - b) / sc) < 1e-12, k
+    for k in ("nhplus", "eint", "temp"):
+        a = R[k].reshape(-1)
+        assert np.max(np.abs(a - getattr(F1, k)) / np.abs(getattr(F1, k))) < 1e-12, k
+
+
+def test_stencil_pinned_buffer(gpu_lib):
+    """A page-locked RGRID buffer (emma_rt_create_pinned_stencil) gives the
+    same results as pageable memory; the oct slots are left as handed over."""
+    G = 3
+    z = gio.load(f"stencil_g{G}.npz")
+    dt_ = rgrid.rgrid_dtype(G)
+    inp = z["inp"].view(dt_)
+    n = int(z["nread"])
+    g = rt.RadGPU(gio.params(G), rt.FP64)
+    g.create_radstencil(64, 8)
+    a = inp.copy()
+    g.advance_stencil(a, n, float(z["dx"]), float(z["dt"]))
+    pin = g.pinned_stencil(64, 8)
+    b = pin.array[:n]
+    b[...] = inp[:n]
+    g.advance_stencil(b, n, float(z["dx"]), float(z["dt"]))
+    for k in ("rfieldnew", "rflux"):  # the fields (struct padding is not copied by numpy)
+        assert np.array_equal(a["New"][k], b["New"][k]), k
+    assert np.array_equal(b["oct"], inp[:n]["oct"])
+    pin.free()
