@@ -260,25 +260,26 @@ def test_amr_golden_mixed_xion_times(gpu_lib, G):
 
 
 @pytest.mark.parametrize("G", [1, 3])
-def test_stencil_path_matches_resident_path(gpu_lib, G):
+@pytest.mark.parametrize("lvl,stride", [(4, 128), (5, 2048)])
+def test_stencil_path_matches_resident_path(gpu_lib, G, lvl, stride):
     """The drop-in path on host-gathered RGRID batches of a complete periodic
     level (gatherstencilrad's layout, rgrid.gather_complete) gives the same
     next state in New.rfieldnew as the resident path's level step, both in
-    the reference's FP64 operation order."""
-    W = synth.uniform_workload(4, "SB99_equiv_mono_TH" if G == 1 else "Starburst99_1grpT_3grpS", seed=40 + G, nsrc=4)
-    L, F0 = W.tree.level(4), W.fields[4]
-    dt = W.params.dt_rad(4)
+    the reference's FP64 operation order, for small and large batches."""
+    W = synth.uniform_workload(lvl, "SB99_equiv_mono_TH" if G == 1 else "Starburst99_1grpT_3grpS", seed=40 + G,
+                               nsrc=4)
+    L, F0 = W.tree.level(lvl), W.fields[lvl]
+    dt = W.params.dt_rad(lvl)
     g = rt.RadGPU(W.params, rt.FP64)
     g.setup(W.params, W.tree, W.fields)
-    g.radsolver(4, dt)
-    F1 = g.get(4)
+    g.radsolver(lvl, dt)
+    F1 = g.get(lvl)
     s = rt.RadGPU(W.params, rt.FP64)
-    stride = 128
     s.create_radstencil(stride, 8)
     new = []
     for o0 in range(0, L.noct, stride):
         st = rgrid.gather_complete(L, F0, np.arange(o0, min(L.noct, o0 + stride)), G)
-        s.advance_stencil(st, st.shape[0], 0.5**4, dt)
+        s.advance_stencil(st, st.shape[0], 0.5**lvl, dt)
         new.append(st["New"]["rfieldnew"])
     R = np.concatenate(new)  # [noct, 8] in cell order
     c = W.params.cloc()
