@@ -1078,17 +1078,31 @@ extern "C" int emma_rt_advance_stencil(emma_rt_ctx* c, void* stencil, int nread,
   if (nread == 0) return EMMA_RT_OK;
   RgridLayout L;
   rgrid_layout(c->p.ngrp, c->stencil_rbytes, &L);
-  const size_t bytes = (size_t)nread * L.v[19];
   int st = begin_call(c);
   if (st) return st;
-  HIPCHK(hipMemcpyAsync(c->d_stencil, stencil, bytes, hipMemcpyHostToDevice, c->stream));
+  // The kernel reads the 7 oct slots and each centre cell's rfieldnew and
+  // writes rfieldnew and rflux (every entry); deltaR, rfield and the fluxes
+  // of the host's buffer are neither read nor changed.  So only those parts
+  // cross PCIe: the oct slots (contiguous at the start of each record), then
+  // per centre cell its rfieldnew, and back per centre cell rfieldnew + rflux
+  // (contiguous at the end of CELLFLUX_R).
+  const size_t rsize = (size_t)L.v[19], ooff = (size_t)L.v[20], osize = 7 * (size_t)L.v[17];
+  const size_t noff = (size_t)L.v[21], fsize = (size_t)L.v[12], rfn = (size_t)L.v[15], rtype = (size_t)L.v[0];
+  const size_t back = fsize - rfn;  // rfieldnew + rflux
+  char* h = (char*)stencil;
+  char* d = (char*)c->d_stencil;
+  HIPCHK(hipMemcpy2DAsync(d + ooff, rsize, h + ooff, rsize, osize, (size_t)nread, hipMemcpyHostToDevice, c->stream));
+  for (int ic = 0; ic < 8; ic++) {
+    const size_t o = noff + ic * fsize + rfn;
+    HIPCHK(hipMemcpy2DAsync(d + o, rsize, h + o, rsize, rtype, (size_t)nread, hipMemcpyHostToDevice, c->stream));
+  }
   const ChemConst K = make_chem(c->p, dtnew, aexp);
   const double cl = cloc_of(c->p, aexp);
   HIPCHK(launch_stencil(c->d_stencil, L, c->p.ngrp, c->stencil_rbytes, nread, cl, dtnew / dxcur, chemonly, K,
                         c->d_status, c->d_counters, c->stream));
-  // only New (rfieldnew, rflux) changes: copy back that part of each record
-  const size_t rsize = (size_t)L.v[19], noff = (size_t)L.v[21], nsize = (size_t)L.v[18];
-  HIPCHK(hipMemcpy2DAsync((char*)stencil + noff, rsize, (const char*)c->d_stencil + noff, rsize, nsize, (size_t)nread,
-                          hipMemcpyDeviceToHost, c->stream));
+  for (int ic = 0; ic < 8; ic++) {
+    const size_t o = noff + ic * fsize + rfn;
+    HIPCHK(hipMemcpy2DAsync(h + o, rsize, d + o, rsize, back, (size_t)nread, hipMemcpyDeviceToHost, c->stream));
+  }
   return end_call(c);
 }

@@ -125,10 +125,11 @@ int emma_rt_create_pinned_stencil(emma_rt_ctx *ctx, int stride, int real_bytes, 
 int emma_rt_destroy_pinned_stencil(void *stencil);
 /* rad_utils_gpu.cu:1054-1086: copy `nread` RGRIDs host->device, run
    rad_sweepX/Y/Z + updatefieldrad + chemrad semantics (CPU semantics of
-   rad_utils.c/chem_utils.c), copy back the New part of each RGRID.  `stencil`
-   is host memory in the reference's RGRID layout (page-locked memory from
-   emma_rt_create_pinned_stencil is fastest); only New.cell[*].rfieldnew and
-   rflux change. */
+   rad_utils.c/chem_utils.c), copy back.  Only the parts the kernel reads (oct
+   slots, New.cell[*].rfieldnew) go to the device and only New.cell[*].rfieldnew
+   and rflux, the parts that change, come back.  `stencil` is host memory in
+   the reference's RGRID layout (page-locked memory from
+   emma_rt_create_pinned_stencil is fastest). */
 int emma_rt_advance_stencil(emma_rt_ctx *ctx, void *stencil, int nread, double dxcur, double dtnew, double aexp,
                             int chemonly);
 
