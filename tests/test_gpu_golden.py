@@ -313,3 +313,34 @@ def test_stencil_pinned_buffer(gpu_lib):
         assert np.array_equal(a["New"][k], b["New"][k]), k
     assert np.array_equal(b["oct"], inp[:n]["oct"])
     pin.free()
+
+
+@pytest.mark.parametrize("G", [1, 3])
+def test_stencil_single_precision_rgrid(gpu_lib, G):
+    """A struct RGRID of REAL = float (the reference's SINGLEPRECISION layout):
+    the golden batch rounded to FP32 gives the FP64 golden results within the
+    mixed-precision bar (fluxes and photon density carry FP32 rounding; the
+    chemistry runs in FP64)."""
+    z = gio.load(f"stencil_g{G}.npz")
+    d8, d4 = rgrid.rgrid_dtype(G, 8), rgrid.rgrid_dtype(G, 4)
+    inp, ref = z["inp"].view(d8), z["out"].view(d8)
+    n = int(z["nread"])
+    st = np.zeros(n, dtype=d4)
+    for k in inp["oct"]["rfield"].dtype.names:
+        st["oct"]["rfield"][k] = inp["oct"]["rfield"][k]
+        st["New"]["rfieldnew"][k] = inp["New"]["rfieldnew"][k]
+    st["oct"]["split"] = inp["oct"]["split"]
+    p = gio.params(G)
+    g = rt.RadGPU(p, rt.FP64)
+    g.create_radstencil(64, 4)
+    g.advance_stencil(st, n, float(z["dx"]), float(z["dt"]))
+    A, B = st["New"]["rfieldnew"], ref["New"]["rfieldnew"]
+    for k in ("e", "nhplus", "eint", "temp"):
+        a, b = A[k].astype(np.float64), B[k]
+        assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)) < 1e-4, k
+    for k in ("fx", "fy", "fz"):
+        a, b = A[k].astype(np.float64), B[k]
+        assert np.max(np.abs(a - b) / np.maximum(p.cloc() * np.abs(B["e"]), 1e-300)) < 1e-4, k
+    fl, rf = rgrid.flux_view(st, G).astype(np.float64), rgrid.flux_view(ref, G)
+    scale = np.maximum(np.abs(rf[..., 0:1]).max(), 1e-300)
+    assert np.max(np.abs(fl[..., :4] - rf[..., :4])) / scale < 1e-5
