@@ -89,45 +89,80 @@ __device__ __forceinline__ Rates rates_fast(double Td) {
 // Cooling (cooling of rt_physics.h) is evaluated with its common factors
 // taken out: nH'^2 x [(A1 + A3)(1 + yHe)(1 - x) + (A6 + A4 (1 + yHe)^2) x] / 10.
 // The flux F of each group only gets multiplied by the accepted sub-steps'
-// absorption factors; instead of carrying F through the loop the function
+// absorption factors; instead of carrying F through the loop the chemistry
 // returns that product (unit conversions in and out included) in fscale, and
 // the caller forms F * fscale.
+//
+// The sub-step loop is split into begin (unit conversions, EMIN test), run
+// (the loop, optionally for a limited number of tries, its whole state in
+// ChemFast so that another thread can continue it bit for bit) and end (the
+// conversions back).
 template <int G>
-__device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fscale, const double* src,
-                              double& nhplus, double& eint_code, double nh, double& temp, int chemonly, int& nit) {
+struct ChemFast {
+  static constexpr int GA = G > 0 ? G : kMaxGrp;
+  double egy[GA];  // photon densities (SI)
+  double fsc[GA];  // product of the accepted absorption factors of the flux
+  double sf[GA];   // source rates
+  double x0, eint, tcur, fudgecool, aexp, nh;
+  int nit, tries;
+};
+
+template <int G>
+__device__ __forceinline__ int chem_fast_begin(const ChemConst& K, int ng, const double* e, const double* src,
+                                               double nhplus, double eint_code, double nh, ChemFast<G>& s) {
 #pragma clang fp contract(fast)
-  constexpr int GA = G > 0 ? G : kMaxGrp;
-  double egy[GA], fsc[GA], sf[GA], et[GA], rp[GA];
-  const double notchem = (double)(!chemonly);
+  constexpr int GA = ChemFast<G>::GA;
 #pragma unroll
   for (int g = 0; g < GA; g++) {
     if (g < ng) {
-      egy[g] = e[g] * K.k_e;
-      fsc[g] = 1.;
-      sf[g] = src[g] * K.k_src * K.factgrp[g];  // source rate of the group
-      fscale[g] = 1.;
+      s.egy[g] = e[g] * K.k_e;
+      s.fsc[g] = 1.;
+      s.sf[g] = src[g] * K.k_src * K.factgrp[g];  // source rate of the group
     }
   }
   const double rnh = frcp64(nh);
-  double x0 = nhplus * rnh;
-  const double nH = nh * K.k_e;
+  s.x0 = nhplus * rnh;
+  s.nh = nh;
+  s.eint = eint_code * K.k_eint;
+  s.nit = 0;
+  s.tries = 0;
+  s.aexp = K.aexp;
+  s.fudgecool = K.fudgecool;
+  s.tcur = 0.;
+  if (s.egy[0] < kEmin) return -4;
+  return 0;
+}
+
+// the sub-step loop until the cell's dt is covered (0), for at most `budget`
+// more tries when budget >= 0 (1: the cell still runs, its state in s), or
+// until the controller fails (-5)
+template <int G>
+__device__ __forceinline__ int chem_fast_run(const ChemConst& K, int ng, int chemonly, ChemFast<G>& s, int budget) {
+#pragma clang fp contract(fast)
+  constexpr int GA = ChemFast<G>::GA;
+  double et[GA], rp[GA];
+  const double notchem = (double)(!chemonly);
+  const double nH = s.nh * K.k_e;
   const double rnH = frcp64(nH);
   const double n22 = (nH * 1e-6) * (nH * 1e-6) * 0.1;
-  double eint = eint_code * K.k_eint;
-  nit = 0;
-  if (egy[0] < kEmin) return -4;
-  double aexp = K.aexp;
-  double fudgecool = K.fudgecool;
-  double tcur = 0.;
   const double dt = K.dt;
   const double kT = 1. / (1.5 * kKboltz);
   const double yp = 1. + kyHe;
   double tcached = -1.;
   Rates r = {};  // evaluated on the first pass (tloc != tcached)
   double cs = 0., ct = 0.;  // cooling(x) = x (cs (1-x) + ct x) at the rates' T
-  int tries = 0;
+  double x0 = s.x0, eint = s.eint, tcur = s.tcur, fudgecool = s.fudgecool, aexp = s.aexp;
+  int tries = s.tries, nit = s.nit;
+  int rc = 0;
   while (tcur < dt) {
-    if (++tries > kMaxTries) return -5;
+    if (budget >= 0 && budget-- == 0) {
+      rc = 1;
+      break;
+    }
+    if (++tries > kMaxTries) {
+      rc = -5;
+      break;
+    }
     const double tloc = eint * kT * frcp64_1(1. + x0) * rnH;
     if (!(tloc == tcached)) {
       r = rates_fast(tloc);
@@ -135,13 +170,16 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fsc
       cs = (r.A1 + r.A3) * yp * n22;
       ct = (r.A6 + r.A4 * (yp * yp)) * n22;
     }
-    if (fudgecool < 1e-20) return -5;
+    if (fudgecool < 1e-20) {
+      rc = -5;
+      break;
+    }
     const double omx0 = 1. - x0;
     const double Cool0 = x0 * fma(ct, x0, cs * omx0);
     double ai = 0.;
 #pragma unroll
     for (int g = 0; g < GA; g++)
-      if (g < ng) ai += K.heat[g] * egy[g];
+      if (g < ng) ai += K.heat[g] * s.egy[g];
     ai *= nH * omx0;
     const double tcool = fabs(eint * frcp64_1(ai * notchem - Cool0));
     const double dtcool = fmin(fudgecool * tcool, dt - tcur);
@@ -153,7 +191,7 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fsc
     for (int g = 0; g < GA; g++) {
       if (g < ng) {
         rp[g] = frcp64_1(1. + K.alphai[g] * c1);
-        et[g] = chemonly ? egy[g] : fma(sf[g], dtcool, egy[g]) * rp[g];
+        et[g] = chemonly ? s.egy[g] : fma(s.sf[g], dtcool, s.egy[g]) * rp[g];
         bad = bad || (et[g] < 0) || isnan(et[g]);
       }
     }
@@ -182,8 +220,8 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fsc
 #pragma unroll
     for (int g = 0; g < GA; g++) {
       if (g < ng) {
-        egy[g] = et[g];
-        if (!chemonly) fsc[g] *= rp[g];
+        s.egy[g] = et[g];
+        if (!chemonly) s.fsc[g] *= rp[g];
       }
     }
     eint = eintt;
@@ -196,8 +234,8 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fsc
 #pragma unroll
       for (int g = 0; g < GA; g++) {
         if (g < ng) {
-          egy[g] *= r3;
-          fsc[g] *= r4;
+          s.egy[g] *= r3;
+          s.fsc[g] *= r4;
         }
       }
       eint *= r5;
@@ -208,9 +246,26 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fsc
     nit++;
     if ((nit == K.ncvgcool) && (K.ncvgcool != 0)) break;
   }
+  s.x0 = x0;
+  s.eint = eint;
+  s.tcur = tcur;
+  s.fudgecool = fudgecool;
+  s.aexp = aexp;
+  s.tries = tries;
+  s.nit = nit;
+  return rc;
+}
+
+template <int G>
+__device__ __forceinline__ void chem_fast_end(const ChemConst& K, int ng, int chemonly, const ChemFast<G>& s,
+                                              double* e, double* fscale, double& nhplus, double& eint_code,
+                                              double& temp) {
+#pragma clang fp contract(fast)
+  constexpr int GA = ChemFast<G>::GA;
+  const double kT = 1. / (1.5 * kKboltz);
   double ki_e = K.ki_e, ki_f = K.ki_f, ki_eint = K.ki_eint, a3f = K.pa3;
   if (!K.static_cosmo) {
-    const double ra = aexp / K.aexp;
+    const double ra = s.aexp / K.aexp;
     a3f = K.pa3 * ra * ra * ra;
     ki_e *= ra * ra * ra;
     ki_f *= ra * ra * ra * ra;
@@ -220,15 +275,32 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fsc
 #pragma unroll
     for (int g = 0; g < GA; g++) {
       if (g < ng) {
-        e[g] = fmax(egy[g] * ki_e, K.emin_grp[g]);
-        fscale[g] = fsc[g] * (K.k_f * ki_f);
+        e[g] = fmax(s.egy[g] * ki_e, K.emin_grp[g]);
+        fscale[g] = s.fsc[g] * (K.k_f * ki_f);
       }
     }
   }
-  nhplus = x0 * nh;
-  eint_code = eint * ki_eint;
+  nhplus = s.x0 * s.nh;
+  eint_code = s.eint * ki_eint;
   // E2T: eint_code back in physical units is eint itself
-  temp = eint * kT * frcp64((nh * K.unit_N / a3f) * (1. + x0));
+  temp = s.eint * kT * frcp64((s.nh * K.unit_N / a3f) * (1. + s.x0));
+}
+
+template <int G>
+__device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fscale, const double* src,
+                              double& nhplus, double& eint_code, double nh, double& temp, int chemonly, int& nit) {
+  constexpr int GA = ChemFast<G>::GA;
+#pragma unroll
+  for (int g = 0; g < GA; g++)
+    if (g < ng) fscale[g] = 1.;
+  ChemFast<G> s;
+  int st = chem_fast_begin<G>(K, ng, e, src, nhplus, eint_code, nh, s);
+  nit = 0;
+  if (st != 0) return st;
+  st = chem_fast_run<G>(K, ng, chemonly, s, -1);
+  nit = s.nit;
+  if (st != 0) return st;
+  chem_fast_end<G>(K, ng, chemonly, s, e, fscale, nhplus, eint_code, temp);
   return 0;
 }
 
