@@ -34,7 +34,9 @@ static hipError_t& last_hip_error() {
 namespace {
 
 struct DevLevel {
-  int level = 0, noct = 0, ncell = 0, nslot = 0, cur = 0;
+  int level = 0, noct = 0, ncell = 0, cur = 0;
+  int npair = 0;  // reflux pairs: (leaf, face) with a finer oct behind the face
+  int nslot = 0;  // accumulators per flux component and group: npair x sub-slots
   int halo_per_dir = 0;  // some wave has more than 64 halo faces (tiled kernel variant)
   void* e[2] = {nullptr, nullptr};
   void* fx[2] = {nullptr, nullptr};
@@ -47,10 +49,12 @@ struct DevLevel {
   double* nh = nullptr;
   int* child = nullptr;
   int* nei = nullptr;
+  int* rpair = nullptr;  // [noct][6] reflux pair of each face in the coarser level, -1 if none
   void* racc = nullptr;
   double* t_first = nullptr;
   double* t_last = nullptr;
-  std::vector<int> hchild;  // host copy with reflux slots encoded
+  std::vector<int> hchild;  // host copy with the reflux pairs encoded
+  std::vector<int> hrpair;
   // hydro state (struct Wtype) for the WRADHYD coupling, and the octs' lower
   // corners for the test source; allocated on first use
   double* hyd[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // d u v w p dX E
@@ -121,6 +125,7 @@ static void free_tree(emma_rt_ctx* c) {
     (void)hipFree(L.child);
     (void)hipFree(L.nei);
     (void)hipFree(L.racc);
+    (void)hipFree(L.rpair);
     (void)hipFree(L.t_first);
     (void)hipFree(L.t_last);
     for (int k = 0; k < 7; k++) (void)hipFree(L.hyd[k]);
@@ -329,6 +334,7 @@ extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int
     }
     const int* nei = nei_oct + off_n[k];
     const int ncoarse = k > 0 ? c->lev[k - 1].ncell : 0;
+    std::vector<unsigned char> rmask(ncoarse, 0);  // faces of the coarse leaves with an oct of level k behind
     for (int i = 0; i < L.noct * 6; i++) {
       int no = nei[i];
       if (no >= L.noct) return EMMA_RT_ERR_NEIGHBOR;
@@ -336,9 +342,33 @@ extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int
         if (k == 0) return EMMA_RT_ERR_NEIGHBOR;  // the coarsest level is complete
         int j = -1 - no;
         if (j >= ncoarse) return EMMA_RT_ERR_NEIGHBOR;
-        DevLevel& C = c->lev[k - 1];
-        if (C.hchild[j] >= 0) return EMMA_RT_ERR_NEIGHBOR;  // injected cells are leaves
-        if (C.hchild[j] == -1) C.hchild[j] = -2 - (C.nslot++);
+        if (c->lev[k - 1].hchild[j] >= 0) return EMMA_RT_ERR_NEIGHBOR;  // injected cells are leaves
+        const int fc = (i % 6) ^ 1;  // the coarse leaf's face towards the oct
+        if (rmask[j] & (1 << fc)) return EMMA_RT_ERR_NEIGHBOR;  // two octs behind one face
+        rmask[j] |= (unsigned char)(1 << fc);
+      }
+    }
+    if (k > 0) {
+      // reflux pairs of level k-1, leaf by leaf, faces in order; child holds
+      // -2 - (first pair << 3 | pairs) for those leaves
+      DevLevel& C = c->lev[k - 1];
+      std::vector<int> first(ncoarse, 0);
+      long long np = 0;
+      for (int j = 0; j < ncoarse; j++) {
+        if (rmask[j] == 0) continue;
+        first[j] = (int)np;
+        const int m = __builtin_popcount(rmask[j]);
+        C.hchild[j] = -2 - (int)((np << 3) | m);
+        np += m;
+        if (np >= (1LL << 28)) return EMMA_RT_ERR_ARG;
+      }
+      C.npair = (int)np;
+      C.nslot = C.npair * (c->rbytes == 4 && (G == 1 || G == 3) ? kRefluxTiled : kRefluxCell);
+      L.hrpair.assign((size_t)L.noct * 6, -1);
+      for (int i = 0; i < L.noct * 6; i++) {
+        if (nei[i] >= 0) continue;
+        const int j = -1 - nei[i], fc = (i % 6) ^ 1;
+        L.hrpair[i] = first[j] + __builtin_popcount(rmask[j] & ((1u << fc) - 1u));
       }
     }
     // faces leaving a cell's oct whose neighbour lies outside the cell's tile,
@@ -390,6 +420,10 @@ extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int
     HIPCHK(hipMemset(L.racc, 0, std::max<size_t>(accn, 1) * c->rbytes));
     HIPCHK(hipMemcpy(L.child, L.hchild.data(), L.ncell * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L.nei, nei_oct + off_n[k], (size_t)L.noct * 6 * sizeof(int), hipMemcpyHostToDevice));
+    if (k > 0) {
+      HIPCHK(dalloc(&L.rpair, (size_t)L.noct * 6));
+      HIPCHK(hipMemcpy(L.rpair, L.hrpair.data(), (size_t)L.noct * 6 * sizeof(int), hipMemcpyHostToDevice));
+    }
   }
   c->has_tree = true;
   if (c->xion) {
@@ -562,6 +596,7 @@ static LevelArgs<R> level_args(emma_rt_ctx* c, int k, double dtnew, double aexp,
   A.tempo = L.temp[b];
   A.child = L.child;
   A.nei = L.nei;
+  A.rpair = L.rpair;
   A.racc = (R*)L.racc;
   A.nslot = L.nslot;
   if (k > 0) {
@@ -571,7 +606,6 @@ static LevelArgs<R> level_args(emma_rt_ctx* c, int k, double dtnew, double aexp,
     A.cfx = (const R*)C.fx[C.cur];
     A.cfy = (const R*)C.fy[C.cur];
     A.cfz = (const R*)C.fz[C.cur];
-    A.cchild = C.child;
     A.cracc = (R*)C.racc;
     A.cnslot = C.nslot;
   }

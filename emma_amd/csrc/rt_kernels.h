@@ -13,7 +13,7 @@ template <typename R>
 struct LevelArgs {
   int ncell, ncoarse, nfine, ng;
   int level, chemonly, nblocks;
-  int nslot;   // reflux slots of this level (stride of racc)
+  int nslot;   // stride of racc (reflux pairs x kReflux sub-slots)
   // RADSTEP: transport=1, keep_split_flux=0, update_split=0.  COARSERAD
   // (rad_utils.c:518-526,1548-1552,1808-1812): transport only on the coarse
   // level and not for chemonly calls, fluxes of split neighbours kept, split
@@ -42,16 +42,23 @@ struct LevelArgs {
   double* nhpluso;
   double* einto;
   double* tempo;
-  // topology: child[cell] >= 0 split, -1 leaf, <= -2 leaf with reflux slot -2-child
+  // topology: child[cell] >= 0 split, -1 leaf, <= -2 leaf with finer octs
+  // behind some of its faces: -2-child = (first reflux pair << 3) | pairs.
+  // A reflux pair is one (coarse leaf, face) with a finer oct behind it; a
+  // leaf's pairs are consecutive, in face order.  Each pair has one
+  // accumulator per flux component and group for the tiled mixed kernel
+  // (its four face cells are summed across lanes first), four (one per face
+  // cell) for the per-cell kernel: every accumulator has one writer per
+  // launch, so refluxing needs no atomics and sums in a fixed order.
   const int* child;
   const int* nei;
-  R* racc;  // this level's reflux slots, component-major: [4][GA][nslot]
-  // coarser level: current state (injection), child table, reflux slots
+  const int* rpair;  // [noct][6]: the coarser level's reflux pair of each oct face, -1 if none
+  R* racc;           // this level's reflux accumulators, component-major: [4][GA][nslot]
+  // coarser level: current state (injection), child table, reflux accumulators
   const R* ce;
   const R* cfx;
   const R* cfy;
   const R* cfz;
-  const int* cchild;
   R* cracc;
   // finer level: current state (restriction)
   const R* fe;
@@ -83,6 +90,10 @@ constexpr int kCounterStride = 16;  // unsigned long longs per slot (128 B)
 
 // tiles of the mixed-precision level step: 256 cells = 32 Morton-consecutive
 // octs per work-group of four waves
+// reflux accumulators per (coarse leaf, face) pair: one per face cell for
+// the per-cell kernel, one for the tiled kernel (face summed across lanes)
+constexpr int kRefluxCell = 4;
+constexpr int kRefluxTiled = 1;
 constexpr int kTileCells = 256;
 constexpr int kTileOcts = kTileCells / 8;
 constexpr int kTileWaves = kTileCells / 64;

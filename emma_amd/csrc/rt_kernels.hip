@@ -151,22 +151,21 @@ __device__ __forceinline__ void restrict_cell(const LevelArgs<R>& a, int cell, i
   }
 }
 
-// reflux of one face flux into the coarse leaf j (scatterstencilrad,
-// rad_utils.c:1738-1762): the coarse cell gains -/+ F dt/dx / 8.  The slots
-// are component-major ([4][GA][nslot]): the atomics of neighbouring faces
-// (consecutive slots) share cache lines, those of one face do not.
+// reflux of one face flux into the coarse leaf behind it (scatterstencilrad,
+// rad_utils.c:1738-1762): the coarse cell gains -/+ F dt/dx / 8.  The cell's
+// own accumulator of the reflux pair p (sub-slot q: its place in the oct
+// face), component-major ([4][GA][nslot]).  No other lane adds to it in this
+// launch, so the sum is the same every run; the add is an atomic only so
+// that the lane need not wait for the old value.
 template <typename R, int GA>
-__device__ __forceinline__ void reflux(const LevelArgs<R>& a, int j, int f, int g, const R F[4], R dts) {
-  const int cch = a.cchild[j];
-  if (cch <= -2) {
-    R* acc = a.cracc + (-2 - cch);
-    const size_t st = (size_t)a.cnslot;
-    const R w = (f & 1) ? R(1) : R(-1);  // pow(-1, face+1)
-    atomicAdd(acc + (0 * GA + g) * st, F[0] * dts * w * R(0.125));
-    atomicAdd(acc + (1 * GA + g) * st, F[1] * dts * w * R(0.125));
-    atomicAdd(acc + (2 * GA + g) * st, F[2] * dts * w * R(0.125));
-    atomicAdd(acc + (3 * GA + g) * st, F[3] * dts * w * R(0.125));
-  }
+__device__ __forceinline__ void reflux(const LevelArgs<R>& a, int p, int q, int f, int g, const R F[4], R dts) {
+  R* acc = a.cracc + (size_t)p * kRefluxCell + q;
+  const size_t st = (size_t)a.cnslot;
+  const R w = (f & 1) ? R(1) : R(-1);  // pow(-1, face+1)
+  atomicAdd(acc + (0 * GA + g) * st, F[0] * dts * w * R(0.125));
+  atomicAdd(acc + (1 * GA + g) * st, F[1] * dts * w * R(0.125));
+  atomicAdd(acc + (2 * GA + g) * st, F[2] * dts * w * R(0.125));
+  atomicAdd(acc + (3 * GA + g) * st, F[3] * dts * w * R(0.125));
 }
 
 // leaf cell after its fluxes: base state (+ refluxes from the finer level),
@@ -174,26 +173,45 @@ __device__ __forceinline__ void reflux(const LevelArgs<R>& a, int j, int f, int 
 // accepted chemistry sub-steps.
 // The chemistry inputs (src, x n_H, eint, n_H) come from memory, or from
 // `pre` when the caller loaded them early (pre_src[GA], pre[3]).
-template <typename R, int G, bool Fast>
+template <typename R, int G, bool Fast, int Sub>
 __device__ __forceinline__ int finish_leaf(const LevelArgs<R>& a, const ChemConst& K, int cell, int ch, int ng,
                                            R* Ue, R* Ux, R* Uy, R* Uz, const R* De, const R* Dx, const R* Dy,
                                            const R* Dz, const R* pre_src = nullptr, const double* pre = nullptr) {
   constexpr int GA = G > 0 ? G : kMaxGrp;
   const int n = a.ncell;
-  if (ch <= -2) {
-    R* acc = a.racc + (-2 - ch);
+  if (ch <= -2) {  // the refluxes of the finer octs behind its faces, in pair order
+    const int code = -2 - ch;
+    R* acc = a.racc + (size_t)(code >> 3) * Sub;
+    const int m = (code & 7) * Sub;
     const size_t st = (size_t)a.nslot;
+    // all loads before the clearing stores.  A leaf has at most six pairs:
+    // the tiled kernel's loop unrolls, and its loads, clamped to the leaf's
+    // last accumulator, go out together instead of one pair after another
+    constexpr int kMaxSub = 6 * Sub;
 #pragma unroll
     for (int g = 0; g < GA; g++) {
       if (g < ng) {
-        Ue[g] += acc[(0 * GA + g) * st];
-        Ux[g] += acc[(1 * GA + g) * st];
-        Uy[g] += acc[(2 * GA + g) * st];
-        Uz[g] += acc[(3 * GA + g) * st];
-        acc[(0 * GA + g) * st] = R(0);
-        acc[(1 * GA + g) * st] = R(0);
-        acc[(2 * GA + g) * st] = R(0);
-        acc[(3 * GA + g) * st] = R(0);
+        R s[4] = {R(0), R(0), R(0), R(0)};
+#pragma unroll(Sub == 1 ? kMaxSub : 1)
+        for (int i = 0; i < (Sub == 1 ? kMaxSub : m); i++) {
+          const int ii = min(i, m - 1);
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const R x = acc[(v * GA + g) * st + ii];
+            s[v] += i < m ? x : R(0);
+          }
+        }
+#pragma unroll(Sub == 1 ? kMaxSub : 1)
+        for (int i = 0; i < kMaxSub; i++) {
+          if (i < m) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[(v * GA + g) * st + i] = R(0);
+          }
+        }
+        Ue[g] += s[0];
+        Ux[g] += s[1];
+        Uy[g] += s[2];
+        Uz[g] += s[3];
       }
     }
   }
@@ -372,7 +390,7 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value
         const R* Ny = a.fy;
         const R* Nz = a.fz;
         int nn = n, j, split;
-        int coarse_face = 0;
+        int coarse_face = 0, pr = -1;
         if (((ic >> d) & 1) != s) {  // neighbour inside the oct
           j = oct * 8 + nc;
           split = a.child[j] >= 0;
@@ -390,6 +408,7 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value
             Nz = a.cfz;
             nn = a.ncoarse;
             coarse_face = 1;
+            pr = a.rpair[oct * 6 + f];
           }
         }
         const R ffact = (split && !a.keep_split_flux) ? R(0) : R(1);
@@ -412,7 +431,11 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value
             Dx[g] += F[1] * dts * one;
             Dy[g] += F[2] * dts * one;
             Dz[g] += F[3] * dts * one;
-            if (coarse_face) reflux<R, GA>(a, j, f, g, F, dts);
+            if (coarse_face) {
+              // the cell's place in the oct face: its two coordinates across d
+              const int q = ((ic >> ((d + 1) % 3)) & 1) | (((ic >> ((d + 2) % 3)) & 1) << 1);
+              reflux<R, GA>(a, pr, q, f, g, F, dts);
+            }
           }
         }
       }
@@ -420,7 +443,8 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value
         transport_split_cell<R, GA>(a, cell, ng, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz);
       } else {
         leaf = 1;
-        nit_acc = (unsigned)finish_leaf<R, G, Fast>(a, K, cell, ch, ng, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz);
+        nit_acc =
+            (unsigned)finish_leaf<R, G, Fast, kRefluxCell>(a, K, cell, ch, ng, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz);
       }
     }
   }
@@ -611,12 +635,13 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
     }
   };
   if constexpr (!PerDir) load_item(lane);
-  // the child entries of the coarse leaves on the cell's leaving faces (their
-  // reflux slots), loaded now for the refluxing in the direction passes
-  int cchD[3] = {0, 0, 0};
-  if (a.ncoarse > 0) {
+  // the reflux pairs of the cell's leaving faces that have a coarse leaf
+  // behind them, loaded now for the refluxing in the direction passes
+  int pairD[3] = {-1, -1, -1};
+  if (valid && a.ncoarse > 0) {
 #pragma unroll
-    for (int d = 0; d < 3; d++) cchD[d] = gld(a.cchild, noD[d] < 0 ? -1 - noD[d] : 0);
+    for (int d = 0; d < 3; d++)
+      if (noD[d] < 0) pairD[d] = gld(a.rpair, oct * 6 + 2 * d + ((ic >> d) & 1));
   }
   // ---- ... compute this cell's closures meanwhile ...
   Closure<float> Q[G];
@@ -728,10 +753,10 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
       }
       // Refluxing (scatterstencilrad, rad_utils.c:1738-1762): the four cells
       // of an oct face share its coarse neighbour, so their fluxes are summed
-      // across lanes first and each of the four lanes then adds one flux
-      // component (E, Fx, Fy, Fz) of every group to the coarse cell's
-      // accumulator: a quarter of the atomics, none contending, issued by
-      // all lanes of the face
+      // across lanes first (the same order every run) and each of the four
+      // lanes then adds one flux component (E, Fx, Fy, Fz) of every group to
+      // the face's reflux pair, whose accumulators no other lane writes (an
+      // atomic add without return: the lane does not wait for the old value)
       const unsigned long long rmask = __ballot(rj >= 0);
       if (rmask != 0ull) {
         const int b1 = 1 << ((d + 1) % 3), b2 = 1 << ((d + 2) % 3);
@@ -748,9 +773,9 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
         const unsigned long long grp =
             (1ull << l0) | (1ull << (l0 | b1)) | (1ull << (l0 | b2)) | (1ull << (l0 | b1 | b2));
         const unsigned long long mine = rmask & grp;
-        const int cch = cchD[d];
-        if (rj >= 0 && cch <= -2) {
-          float* acc = a.cracc + (-2 - cch);
+        const int pr = pairD[d];
+        if (rj >= 0 && pr >= 0) {
+          float* acc = a.cracc + (size_t)pr * kRefluxTiled;
           const size_t st = (size_t)a.cnslot;
           const float w = (bit ? 1.f : -1.f) * (0.5f * dts) * 0.125f;  // pow(-1, face+1) dt/dx / 8, shares doubled
           if (mine == grp) {  // the face's four cells all carry flux (always, with 2:1 balance)
@@ -789,7 +814,8 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
       transport_split_cell<float, G>(a, cell, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz);
     } else {
       leaf = 1;
-      nit_acc = (unsigned)finish_leaf<float, G, true>(a, K, cell, ch, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz, Src, pre);
+      nit_acc = (unsigned)finish_leaf<float, G, true, kRefluxTiled>(a, K, cell, ch, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz,
+                                                                    Src, pre);
     }
   }
   if (a.counters != nullptr) block_counters<kTileWaves>(a.counters, nit_acc, leaf);

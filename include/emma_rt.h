@@ -136,8 +136,12 @@ int emma_rt_advance_stencil(emma_rt_ctx *ctx, void *stencil, int nread, double d
 /* ---- resident-level path ---- */
 /* Device tree of levels lmin..lmax: noct[lmax-lmin+1]; nei_oct and child are
    the levels' arrays concatenated in level order (6 and 8 ints per oct).
-   A level holds at most min(2^29, (2^31-1)/ngrp) cells (EMMA_RT_ERR_ARG
-   beyond: cell indices and per-group byte offsets are 32-bit). */
+   A level holds at most min(2^29, (2^31-1)/ngrp) cells, and fewer than 2^28
+   (leaf, face) pairs with a finer oct behind the face (EMMA_RT_ERR_ARG
+   beyond: cell indices and per-group byte offsets are 32-bit, and a leaf's
+   child entry encodes its first pair).  Each such face has its own reflux
+   accumulators, written by one thread: AMR results are the same bit for bit
+   from run to run. */
 int emma_rt_tree_create(emma_rt_ctx *ctx, int lmin, int lmax, const int *noct, const int *nei_oct,
                         const int *child, int precision);
 /* host SoA (double) <-> device state of one level (rfield, prototypes.h:849) */
