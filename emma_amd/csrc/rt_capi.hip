@@ -31,6 +31,9 @@ static hipError_t& last_hip_error() {
   return e;
 }
 
+static_assert(emma::kNeiTransmissive == EMMA_RT_NEI_TRANSMISSIVE && emma::kNeiReflective == EMMA_RT_NEI_REFLECTIVE,
+              "domain-face codes of the kernels and the C-ABI");
+
 namespace {
 
 struct DevLevel {
@@ -338,6 +341,7 @@ extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int
     for (int i = 0; i < L.noct * 6; i++) {
       int no = nei[i];
       if (no >= L.noct) return EMMA_RT_ERR_NEIGHBOR;
+      if (no == EMMA_RT_NEI_TRANSMISSIVE || no == EMMA_RT_NEI_REFLECTIVE) continue;  // domain face
       if (no < 0) {
         if (k == 0) return EMMA_RT_ERR_NEIGHBOR;  // the coarsest level is complete
         int j = -1 - no;
@@ -366,7 +370,7 @@ extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int
       C.nslot = C.npair * (c->rbytes == 4 && (G == 1 || G == 3) ? kRefluxTiled : kRefluxCell);
       L.hrpair.assign((size_t)L.noct * 6, -1);
       for (int i = 0; i < L.noct * 6; i++) {
-        if (nei[i] >= 0) continue;
+        if (nei[i] >= 0 || nei[i] <= EMMA_RT_NEI_REFLECTIVE) continue;
         const int j = -1 - nei[i], fc = (i % 6) ^ 1;
         L.hrpair[i] = first[j] + __builtin_popcount(rmask[j] & ((1u << fc) - 1u));
       }

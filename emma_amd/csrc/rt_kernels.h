@@ -90,6 +90,18 @@ constexpr int kCounterStride = 16;  // unsigned long longs per slot (128 B)
 
 // tiles of the mixed-precision level step: 256 cells = 32 Morton-consecutive
 // octs per work-group of four waves
+// nei_oct entries of domain-boundary faces (EMMA_RT_NEI_TRANSMISSIVE /
+// EMMA_RT_NEI_REFLECTIVE): the ghost across the face is a copy of a cell of
+// the same oct, its normal flux negated when reflective; no refluxing
+constexpr int kNeiTransmissive = -2147483647 - 1;
+constexpr int kNeiReflective = -2147483647;
+
+// the cell of the boundary oct whose copy cell `ic` sees across the domain
+// face f: itself, except on +y, where the reference's face table
+// (rad_utils.c:1124-1139: face[4] = 7, face[5] = 6) makes cells 6 and 7 see
+// each other
+__host__ __device__ __forceinline__ int trans_source(int f, int ic) { return (f == 3 && ic >= 6) ? ic ^ 1 : ic; }
+
 // reflux accumulators per (coarse leaf, face) pair: one per face cell for
 // the per-cell kernel, one for the tiled kernel (face summed across lanes)
 constexpr int kRefluxCell = 4;

@@ -390,7 +390,7 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value
         const R* Ny = a.fy;
         const R* Nz = a.fz;
         int nn = n, j, split;
-        int coarse_face = 0, pr = -1;
+        int coarse_face = 0, pr = -1, reflect = 0;
         if (((ic >> d) & 1) != s) {  // neighbour inside the oct
           j = oct * 8 + nc;
           split = a.child[j] >= 0;
@@ -399,6 +399,10 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value
           if (no >= 0) {
             j = no * 8 + nc;
             split = a.child[j] >= 0;
+          } else if (no <= kNeiReflective) {  // domain face (rad_utils.c:1106-1213, 1376-1407)
+            j = oct * 8 + trans_source(f, ic);
+            split = a.child[j] >= 0;
+            reflect = no == kNeiReflective;
           } else {  // coarse leaf injected into the stencil
             j = -1 - no;
             split = 0;
@@ -418,6 +422,11 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value
           if (g < ng) {
             R ve, vx, vy, vz;
             load4(a, Ne, Nx, Ny, Nz, nn, j, g, ve, vx, vy, vz);
+            if (reflect) {
+              if (d == 0) vx = -vx;
+              if (d == 1) vy = -vy;
+              if (d == 2) vz = -vz;
+            }
             Closure<R> qn;
             if constexpr (Fast)
               qn = closure_fast(vx, vy, vz, ve, c);
@@ -578,11 +587,16 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   int hd = 0, hside = 0, hchild = -1;
   float hv[G][4];
   // the neighbour across the face in direction hd that leaves the oct of
-  // the cell at position hic, nei_oct entry hno
-  auto load_neighbour = [&](int hno, int hic) {
+  // cell hcell, nei_oct entry hno: a same-level cell, an injected coarse
+  // leaf, or on a domain face a copy of a cell of the same oct (its normal
+  // flux negated when reflective)
+  auto load_neighbour = [&](int hno, int hcell) {
+    const int hic = hcell & 7;
     hside = (hic >> hd) & 1;
-    hcoarse = hlane && hno < 0;
-    const int js = (hlane && !hcoarse) ? hno * 8 + (hic ^ (1 << hd)) : 0;
+    const bool hbound = hlane && hno <= kNeiReflective;
+    hcoarse = hlane && hno < 0 && !hbound;
+    const int js = hbound ? (hcell & ~7) + trans_source(2 * hd + hside, hic)
+                          : ((hlane && !hcoarse) ? hno * 8 + (hic ^ (1 << hd)) : 0);
 #pragma unroll
     for (int g = 0; g < G; g++) {
       hv[g][0] = gld(a.e + g * n, js);
@@ -591,6 +605,17 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
       hv[g][3] = gld(a.fz + g * n, js);
     }
     hchild = gld(a.child, js);
+    {  // reflective face: negate the normal component (hd differs between lanes: no indexing by it)
+      const bool refl = hbound && hno == kNeiReflective;
+      const float sx = (refl && hd == 0) ? -1.f : 1.f, sy = (refl && hd == 1) ? -1.f : 1.f,
+                  sz = (refl && hd == 2) ? -1.f : 1.f;
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        hv[g][1] *= sx;
+        hv[g][2] *= sy;
+        hv[g][3] *= sz;
+      }
+    }
     if (__ballot(hcoarse) != 0) {  // injected coarse leaves (levels with a coarser level only)
       const int jc = hcoarse ? -1 - hno : 0;
       const int nc = a.ncoarse;
@@ -612,13 +637,13 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
     hlane = slot < nhalo;
     const int it = sItem[wv][PerDir ? 0 : slot];
     hd = it & 3;
-    load_neighbour(sItemNei[wv][PerDir ? 0 : slot], (it >> 2) & 7);
+    load_neighbour(sItemNei[wv][PerDir ? 0 : slot], (cell & ~63) + (it >> 2));
   };
   // sparse levels: this lane's own face in direction d
   auto load_face = [&](int d) {
     hlane = moves && !in_tile(noD[d]);
     hd = d;
-    load_neighbour(noD[d], ic);
+    load_neighbour(noD[d], cell);
   };
   // ... the neighbour's share (a neighbour on the plus side of the face,
   // side 1, contributes B, on the minus side A) and cancel flag
@@ -685,9 +710,10 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
         const int jin = lc ^ (1 << d);
         const int iin = (1 - bit) * kTileCells + jin;  // neighbour share, inside face
         const bool inT = in_tile(noD[d]);
-        const int jl = (noD[d] - oct0) * 8 + (ic ^ (1 << d));
+        // (unsigned: a domain-face entry is near INT_MIN; jl and j are then unused)
+        const int jl = (int)(((unsigned)noD[d] - (unsigned)oct0) * 8u + (unsigned)(ic ^ (1 << d)));
         const int slot = PerDir ? lane : hbase[d] + lanes_below(hball[d]);
-        const bool coarse_face = !inT && noD[d] < 0;
+        const bool coarse_face = !inT && noD[d] < 0 && noD[d] > kNeiReflective;
         int iout, sout, cout;  // neighbour share, stride over groups, cancel flag: leaving face
         if (inT) {
           iout = bit * kTileCells + jl;
@@ -698,7 +724,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
           sout = 1;
           cout = kTileCells + wv * kHaloCap + slot;
         }
-        const int j = coarse_face ? -1 - noD[d] : noD[d] * 8 + (ic ^ (1 << d));
+        const int j = coarse_face ? -1 - noD[d] : (int)((unsigned)noD[d] * 8u + (unsigned)(ic ^ (1 << d)));
         const bool cancel_out = sCancel[cout] != 0;
         const bool cancel_in = sCancel[jin] != 0;
         const int im = bit ? iin : iout, sm = bit ? 2 * kTileCells : sout;

@@ -24,6 +24,12 @@ import dataclasses
 
 import numpy as np
 
+# nei_oct entries of domain-boundary faces (WRADTEST TRANS*/REF* flags,
+# rad_utils.c:1065-1409): the ghost across the face is the cell's own copy,
+# with the normal flux negated when reflective; no refluxing through it
+NEI_TRANSMISSIVE = -(2**31)
+NEI_REFLECTIVE = -(2**31) + 1
+
 # face f = 2d + s: axis d, s=0 minus side, s=1 plus side
 FACE_DIR = np.array([[-1, 0, 0], [1, 0, 0], [0, -1, 0], [0, 1, 0], [0, 0, -1], [0, 0, 1]], dtype=np.int64)
 CELL_OFF = np.array([[i & 1, (i >> 1) & 1, i >> 2] for i in range(8)], dtype=np.int64)
@@ -142,7 +148,7 @@ class Tree:
             L = self.levels[l]
             assert L.nei_oct.shape == (L.noct, 6)
             if l == self.lmin:
-                assert (L.nei_oct >= 0).all(), "the coarsest level must be complete"
+                assert ((L.nei_oct >= 0) | (L.nei_oct <= NEI_REFLECTIVE)).all(), "the coarsest level must be complete"
             if l < self.lmax:
                 F = self.levels[l + 1]
                 split = np.nonzero(L.child >= 0)[0]
@@ -252,3 +258,27 @@ def amr_tree(lmin: int, counts: list, score_fn, rng: np.random.Generator) -> Tre
         pick = np.argpartition(-score, want - 1)[:want]
         levels.append(refine(L, np.sort(pick)))
     return build_tree(levels)
+
+
+def with_boundaries(tree: Tree, trans=(0, 1, 2, 3, 4, 5), reflect=()) -> Tree:
+    """The tree with non-periodic domain faces, as EMMA's WRADTEST build flags
+    TRANSXM..TRANSZP and REFXM/REFYM/REFZM set them up (rad_utils.c:1065-1409):
+    an oct whose lower corner lies on the domain face f in `trans`
+    (x == 0 for f = 0, x + 2 dx == 1 for f = 1, ...) sees its own cells
+    across that face, with the flux component normal to f negated for f in
+    `reflect` (the reference defines reflection on the minus faces 0, 2, 4
+    only), and refluxes nothing through it (rad_utils.c:1700-1734).  The
+    faces' nei_oct entries become NEI_TRANSMISSIVE / NEI_REFLECTIVE;
+    ``neicell`` keeps the periodic neighbours (the reference decides from the
+    octs' positions)."""
+    assert set(reflect) <= set(trans) and set(trans) <= set(range(6))
+    levels = {}
+    for l, L in tree.levels.items():
+        nei = L.nei_oct.copy()
+        last = (1 << (l - 1)) - 1  # the last oct coordinate on level l's grid
+        for f in trans:
+            d, s = f >> 1, f & 1
+            on = L.pos[:, d] == (last if s else 0)
+            nei[on, f] = NEI_REFLECTIVE if f in reflect else NEI_TRANSMISSIVE
+        levels[l] = dataclasses.replace(L, nei_oct=nei)
+    return Tree(tree.lmin, tree.lmax, levels)

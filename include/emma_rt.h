@@ -134,8 +134,22 @@ int emma_rt_advance_stencil(emma_rt_ctx *ctx, void *stencil, int nread, double d
                             int chemonly);
 
 /* ---- resident-level path ---- */
+/* nei_oct entries of faces on a non-periodic domain face, the WRADTEST
+   TRANSXM..TRANSZP and REFXM/REFYM/REFZM build flags of the reference
+   (rad_utils.c:1065-1409): the cells of the boundary oct see a copy of the
+   oct's own cells across the face (each cell itself, except on +y where the
+   reference's face table makes cells 6 and 7 see each other), with the flux
+   component normal to the face negated for REFLECTIVE; nothing is refluxed
+   through the face (rad_utils.c:1700-1734).  The reference decides from the
+   octs' positions (x == 0 for the minus x face, x + 2 dx == 1 for the plus
+   x face, ...); the caller marks those faces. */
+#define EMMA_RT_NEI_TRANSMISSIVE (-2147483647 - 1)
+#define EMMA_RT_NEI_REFLECTIVE (-2147483647)
+
 /* Device tree of levels lmin..lmax: noct[lmax-lmin+1]; nei_oct and child are
    the levels' arrays concatenated in level order (6 and 8 ints per oct).
+   nei_oct: the same-level oct across the face, -1-c for the coarser level's
+   leaf cell c, or one of the EMMA_RT_NEI_* domain-face codes.
    A level holds at most min(2^29, (2^31-1)/ngrp) cells, and fewer than 2^28
    (leaf, face) pairs with a finer oct behind the face (EMMA_RT_ERR_ARG
    beyond: cell indices and per-group byte offsets are 32-bit, and a leaf's

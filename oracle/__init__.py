@@ -31,6 +31,10 @@ REF_SO = {1: os.path.join(HERE, "_ref", "libemma_ref_g1.so"), 3: os.path.join(HE
 REF_SO_CR = {1: os.path.join(HERE, "_ref", "libemma_ref_g1_cr.so"), 3: os.path.join(HERE, "_ref", "libemma_ref_g3_cr.so")}
 REF_SO_HYD = {1: os.path.join(HERE, "_ref", "libemma_ref_g1_hyd.so"),
               3: os.path.join(HERE, "_ref", "libemma_ref_g3_hyd.so")}
+# non-periodic domain faces: TRANS* on all six faces (G=1); plus REF* on the
+# minus faces (G=3)
+REF_SO_TRANS = {1: os.path.join(HERE, "_ref", "libemma_ref_g1_tr.so"),
+                3: os.path.join(HERE, "_ref", "libemma_ref_g3_trref.so")}
 
 RADSTEP = 0
 COARSERAD = 1
@@ -372,18 +376,23 @@ def coarse_step_coarserad(solver, tree, params, aexp=1.0):
     adv(tree.lmin)
 
 
-def ref_available(ngrp: int, coarserad: bool = False, hydro: bool = False) -> bool:
-    return os.path.exists((REF_SO_HYD if hydro else REF_SO_CR if coarserad else REF_SO).get(ngrp, ""))
+def _ref_table(coarserad: bool, hydro: bool, boundary: bool) -> dict:
+    return REF_SO_TRANS if boundary else REF_SO_HYD if hydro else REF_SO_CR if coarserad else REF_SO
+
+
+def ref_available(ngrp: int, coarserad: bool = False, hydro: bool = False, boundary: bool = False) -> bool:
+    return os.path.exists(_ref_table(coarserad, hydro, boundary).get(ngrp, ""))
 
 
 class Reference:
     """The reference's own CPU functions, driven through ref_driver.c.
     coarserad: the COARSERAD build; hydro: the WRADHYD build (FillRad with the
     hydro copy and the Stromgren source, RadSolver's injection into the hydro
-    fields)."""
+    fields); boundary: the builds with non-periodic domain faces (REF_SO_TRANS),
+    which decide from the octs' positions (set_oct_pos)."""
 
-    def __init__(self, ngrp: int, coarserad: bool = False, hydro: bool = False):
-        path = (REF_SO_HYD if hydro else REF_SO_CR if coarserad else REF_SO)[ngrp]
+    def __init__(self, ngrp: int, coarserad: bool = False, hydro: bool = False, boundary: bool = False):
+        path = _ref_table(coarserad, hydro, boundary)[ngrp]
         self.mode = COARSERAD if coarserad else RADSTEP
         self.hydro_build = hydro
         if not os.path.exists(path):
@@ -415,12 +424,12 @@ class Reference:
         L.refdrv_set_tphy.argtypes = [ctypes.c_double]
         L.refdrv_rates_n.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.refdrv_fillrad_coarse.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.refdrv_set_oct_pos.argtypes = [ctypes.c_int, _dp]
         assert L.refdrv_coarserad() == (1 if coarserad else 0)
         assert L.refdrv_wradhyd() == (1 if hydro else 0)
         if hydro:
             L.refdrv_set_hydro.argtypes = [ctypes.c_int] + [_dp] * 6
             L.refdrv_get_hydro.argtypes = [ctypes.c_int] + [_dp] * 3
-            L.refdrv_set_oct_pos.argtypes = [ctypes.c_int, _dp]
             L.refdrv_set_stromgren.argtypes = [ctypes.c_double] * 7
             L.refdrv_fillrad.argtypes = [ctypes.c_int, ctypes.c_double]
         self.stride = 4096

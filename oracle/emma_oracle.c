@@ -337,6 +337,10 @@ static void read_slot(int G, const orc_fields *f, int idx, int split, slot_state
   S->split = split;
 }
 
+/* the cell of the boundary oct whose copy the cell `icell` sees across the
+   domain face f (the reference's TRANS* face tables, rad_utils.c:1106-1213) */
+int orc_trans_source(int f, int icell) { return (f == 3 && icell >= 6) ? icell ^ 1 : icell; }
+
 static int cell_gather(int G, const int *nei_oct, const int *child, const orc_fields *old, const orc_fields *cold,
                        int oct, int icell, slot_state *C, slot_state N[6]) {
   int vnei[6], vcell[6];
@@ -352,6 +356,21 @@ static int cell_gather(int G, const int *nei_oct, const int *child, const orc_fi
       if (no >= 0) {
         int j = no * 8 + vcell[f];
         read_slot(G, old, j, child[j] >= 0, &N[f]);
+      } else if (no <= ORC_NEI_REFLECTIVE) {
+        /* domain boundary (rad_utils.c:1106-1213, 1376-1407): the ghost oct
+           copies the boundary oct through the reference's face tables, which
+           give each cell its own copy except on +y, where cells 6 and 7 see
+           each other (face[4] = 7, face[5] = 6, rad_utils.c:1124-1139); the
+           normal flux is negated for REF* */
+        const int m = orc_trans_source(f, icell);
+        read_slot(G, old, oct * 8 + m, child[oct * 8 + m] >= 0, &N[f]);
+        if (no == ORC_NEI_REFLECTIVE) {
+          for (int g = 0; g < G; g++) {
+            if (f >> 1 == 0) N[f].fx[g] *= -1.;
+            if (f >> 1 == 1) N[f].fy[g] *= -1.;
+            if (f >> 1 == 2) N[f].fz[g] *= -1.;
+          }
+        }
       } else {
         if (cold == NULL) return ORC_ERR_NEIGHBOR;
         read_slot(G, cold, -1 - no, 0, &N[f]);
@@ -493,7 +512,7 @@ int orc_level_advance_mode(const orc_params *P, int mode, int is_coarse, int noc
         for (int inei = 0; inei < 6; inei++) {
           if (vnei[inei] == 6) continue;
           int no = nei_oct[oct * 6 + vnei[inei]];
-          if (no >= 0) continue;
+          if (no >= 0 || no <= ORC_NEI_REFLECTIVE) continue; /* TRANS* faces: rad_utils.c:1700-1734 */
           int ci = -1 - no, nc = cnew->ncell;
           double one = pow(-1., inei + 1);
           const double *Ff = F + inei * 5 * G;

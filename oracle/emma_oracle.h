@@ -37,6 +37,15 @@ extern "C" {
 #define ORC_ERR_NOCONV (-5)     /* chem_utils.c:453-466 */
 #define ORC_ERR_NEIGHBOR (-6)   /* rad_utils.c:1441-1444 */
 
+/* nei_oct entries of the faces on the domain boundary under the WRADTEST
+   TRANS* flags (rad_utils.c:1065-1409): the neighbour across the face is the
+   cell itself (the ghost oct mirrors the boundary oct), and for REF* the
+   flux component normal to the face changes sign; refluxing skips the face
+   (rad_utils.c:1700-1734).  The same values as the library's. */
+#define ORC_NEI_TRANSMISSIVE (-2147483647 - 1)
+#define ORC_NEI_REFLECTIVE (-2147483647)
+int orc_trans_source(int f, int icell);
+
 typedef struct orc_params {
   int ngrp;
   int ncvgcool;      /* RUNPARAMS.ncvgcool (0 = no cap) */

@@ -367,6 +367,18 @@ int refdrv_fillrad_coarse(int level, double aexp) {
 #endif
 }
 
+/* oct lower corners (struct OCT x, y, z), 3 per oct: read by stromgren_source
+   and by the TRANS* domain faces of the WRADTEST gather */
+int refdrv_set_oct_pos(int level, const double *xyz) {
+  if (level < g_lmin || level > g_lmax) return -1;
+  for (int i = 0; i < g_noct[level]; i++) {
+    g_lev[level][i].x = xyz[3 * i + 0];
+    g_lev[level][i].y = xyz[3 * i + 1];
+    g_lev[level][i].z = xyz[3 * i + 2];
+  }
+  return 0;
+}
+
 #ifdef WRADHYD
 /* ---- hydro coupling (WRADHYD) and the WRADTEST Strömgren source ---- */
 int refdrv_wradhyd(void) { return 1; }
@@ -400,17 +412,6 @@ int refdrv_get_hydro(int level, double *p, double *dX, double *E) {
     p[i] = W->p;
     dX[i] = W->dX;
     E[i] = W->E;
-  }
-  return 0;
-}
-
-/* oct lower corners (struct OCT x, y, z), 3 per oct */
-int refdrv_set_oct_pos(int level, const double *xyz) {
-  if (level < g_lmin || level > g_lmax) return -1;
-  for (int i = 0; i < g_noct[level]; i++) {
-    g_lev[level][i].x = xyz[3 * i + 0];
-    g_lev[level][i].y = xyz[3 * i + 1];
-    g_lev[level][i].z = xyz[3 * i + 2];
   }
   return 0;
 }

@@ -405,3 +405,34 @@ def test_amr_bit_reproducible(gpu_lib, table, G, precision):
     for l in runs[0]:
         for k in Fields.NAMES:
             assert np.array_equal(getattr(runs[0][l], k), getattr(runs[1][l], k)), (l, k)
+
+
+@pytest.mark.parametrize("table,G,reflect", [("SB99_equiv_mono_TH", 1, ()), ("Starburst99_1grpT_3grpS", 3, (0, 2, 4)),
+                                             ("Starburst99_3grpT_3grpS", 9, (0, 2, 4))])
+@pytest.mark.parametrize("precision", [rt.FP64, rt.MIXED])
+@pytest.mark.parametrize("amr", [False, True])
+def test_domain_boundaries(gpu_lib, oracle, table, G, reflect, precision, amr):
+    """Non-periodic domain faces (EMMA_RT_NEI_TRANSMISSIVE / _REFLECTIVE, the
+    reference's TRANS*/REF* gather; the oracle agrees with the reference's
+    TRANS builds bit for bit, test_oracle_ref.py): a complete 16^3 level (the
+    tiled kernel's listed halo) and a 4-level tree (its per-direction halo,
+    refined octs on the domain faces, no refluxing through them); 9 groups run
+    the per-cell kernel.  Two coarse steps, graph mode."""
+    from emma_amd import tree as T
+
+    if amr:
+        W = synth.amr_workload(3, [60, 100, 80], table, seed=29, sigma=1.5, nsrc=4, grid=16)
+    else:
+        W = synth.uniform_workload(4, table, seed=29, nsrc=6)
+    tb = T.with_boundaries(W.tree, trans=range(6), reflect=reflect)
+    W = synth.Workload(W.name, W.params, tb, W.fields, W.nsub)
+    g = run_pair(oracle, W, precision, steps=2, mode="graph")
+    c = W.params.cloc()
+    for l in range(W.tree.lmin, W.tree.lmax + 1):
+        err = field_errors(g.get(l), oracle.get(l), c)
+        print("boundaries G=%d prec=%d amr=%d level %d" % (G, precision, amr, l), err)
+        if precision == rt.FP64:
+            assert max(err.values()) < FP64_TOL, (l, err)
+        else:
+            assert err["x"] < MIXED_TOL and err["temp"] < MIXED_TOL and err["e"] < MIXED_TOL, (l, err)
+            assert max(err["fx"], err["fy"], err["fz"]) < MIXED_TOL, (l, err)

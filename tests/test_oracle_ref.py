@@ -136,3 +136,41 @@ def test_coarserad_steps(G):
         o.coarse_step()
         r.coarse_step()
     same_state(o, r, W.tree)
+
+
+@pytest.mark.parametrize("G,reflect", [(1, ()), (3, (0, 2, 4))])
+@pytest.mark.parametrize("amr", [False, True])
+def test_domain_boundaries(G, reflect, amr):
+    """Non-periodic domain faces (WRADTEST with TRANSXM..TRANSZP, and REFXM,
+    REFYM, REFZM for G=3; rad_utils.c:1065-1409, 1700-1734): the reference
+    decides from the octs' positions, the oracle from the boundary codes that
+    tree.with_boundaries writes into nei_oct.  Sources and a lognormal
+    density make the boundary fluxes nonzero; the AMR tree has refined octs
+    on the domain faces (transmissive fine faces, no refluxing across)."""
+    from emma_amd import tree as T
+
+    if not ref_available(G, boundary=True):
+        pytest.skip("reference build (oracle/_ref) not present")
+    if amr:
+        W = synth.amr_workload(3, [60, 100], TABLES[G], seed=23, sigma=1.5, nsrc=4, grid=16)
+    else:
+        W = synth.uniform_workload(3, TABLES[G], seed=23, nsrc=4)
+    tb = T.with_boundaries(W.tree, trans=range(6), reflect=reflect)
+    tb.validate()
+    nb = sum(int(np.count_nonzero(tb.level(l).nei_oct <= T.NEI_REFLECTIVE)) for l in tb.levels)
+    assert nb > 0
+    o, r = Oracle(), Reference(G, boundary=True)
+    o.setup(W.params, tb, W.fields)
+    r.setup(W.params, W.tree, W.fields)
+    for l in range(W.tree.lmin, W.tree.lmax + 1):
+        r.set_oct_pos(l, synth.oct_corners(W.tree.level(l)))
+    for _ in range(2):
+        o.coarse_step()
+        r.coarse_step()
+    same_state(o, r, W.tree)
+    # the faces make a difference: the periodic oracle differs
+    p = Oracle()
+    p.setup(W.params, W.tree, W.fields)
+    for _ in range(2):
+        p.coarse_step()
+    assert not np.array_equal(p.get(W.tree.lmax, 0).e, o.get(W.tree.lmax, 0).e)
