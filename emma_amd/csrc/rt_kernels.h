@@ -13,7 +13,7 @@ template <typename R>
 struct LevelArgs {
   int ncell, ncoarse, nfine, ng;
   int level, chemonly, nblocks;
-  int nslot;   // stride of racc (reflux pairs x kReflux sub-slots)
+  int nslot;   // accumulators in racc (reflux pairs x kReflux sub-slots)
   // RADSTEP: transport=1, keep_split_flux=0, update_split=0.  COARSERAD
   // (rad_utils.c:518-526,1548-1552,1808-1812): transport only on the coarse
   // level and not for chemonly calls, fluxes of split neighbours kept, split
@@ -22,7 +22,7 @@ struct LevelArgs {
   // mixed path: 1 when some wave of the level has more than 64 halo faces
   // (sparse levels: the per-direction halo variant of level_step_tiled)
   int halo_per_dir;
-  int cnslot;  // reflux slots of the coarser level (stride of cracc)
+  int cnslot;  // accumulators in cracc
   // current state (read)
   const R* e;
   const R* fx;
@@ -53,7 +53,7 @@ struct LevelArgs {
   const int* child;
   const int* nei;
   const int* rpair;  // [noct][6]: the coarser level's reflux pair of each oct face, -1 if none
-  R* racc;           // this level's reflux accumulators, component-major: [4][GA][nslot]
+  R* racc;           // this level's reflux accumulators: [nslot][4][GA], an accumulator's values contiguous
   // coarser level: current state (injection), child table, reflux accumulators
   const R* ce;
   const R* cfx;

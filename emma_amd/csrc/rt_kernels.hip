@@ -154,18 +154,17 @@ __device__ __forceinline__ void restrict_cell(const LevelArgs<R>& a, int cell, i
 // reflux of one face flux into the coarse leaf behind it (scatterstencilrad,
 // rad_utils.c:1738-1762): the coarse cell gains -/+ F dt/dx / 8.  The cell's
 // own accumulator of the reflux pair p (sub-slot q: its place in the oct
-// face), component-major ([4][GA][nslot]).  No other lane adds to it in this
+// face), its 4 x GA values contiguous.  No other lane adds to it in this
 // launch, so the sum is the same every run; the add is an atomic only so
 // that the lane need not wait for the old value.
 template <typename R, int GA>
 __device__ __forceinline__ void reflux(const LevelArgs<R>& a, int p, int q, int f, int g, const R F[4], R dts) {
-  R* acc = a.cracc + (size_t)p * kRefluxCell + q;
-  const size_t st = (size_t)a.cnslot;
+  R* acc = a.cracc + ((size_t)p * kRefluxCell + q) * 4 * GA;
   const R w = (f & 1) ? R(1) : R(-1);  // pow(-1, face+1)
-  atomicAdd(acc + (0 * GA + g) * st, F[0] * dts * w * R(0.125));
-  atomicAdd(acc + (1 * GA + g) * st, F[1] * dts * w * R(0.125));
-  atomicAdd(acc + (2 * GA + g) * st, F[2] * dts * w * R(0.125));
-  atomicAdd(acc + (3 * GA + g) * st, F[3] * dts * w * R(0.125));
+  atomicAdd(acc + 0 * GA + g, F[0] * dts * w * R(0.125));
+  atomicAdd(acc + 1 * GA + g, F[1] * dts * w * R(0.125));
+  atomicAdd(acc + 2 * GA + g, F[2] * dts * w * R(0.125));
+  atomicAdd(acc + 3 * GA + g, F[3] * dts * w * R(0.125));
 }
 
 // leaf cell after its fluxes: base state (+ refluxes from the finer level),
@@ -181,37 +180,48 @@ __device__ __forceinline__ int finish_leaf(const LevelArgs<R>& a, const ChemCons
   const int n = a.ncell;
   if (ch <= -2) {  // the refluxes of the finer octs behind its faces, in pair order
     const int code = -2 - ch;
-    R* acc = a.racc + (size_t)(code >> 3) * Sub;
+    R* acc = a.racc + (size_t)(code >> 3) * Sub * 4 * GA;
     const int m = (code & 7) * Sub;
-    const size_t st = (size_t)a.nslot;
     // all loads before the clearing stores.  A leaf has at most six pairs:
-    // the tiled kernel's loop unrolls, and its loads, clamped to the leaf's
-    // last accumulator, go out together instead of one pair after another
+    // the tiled kernel's loop unrolls, and its loads (an accumulator's 4G
+    // values are contiguous), clamped to the leaf's last accumulator, go out
+    // together instead of one pair after another
     constexpr int kMaxSub = 6 * Sub;
+    R s[GA][4];
+#pragma unroll
+    for (int g = 0; g < GA; g++) s[g][0] = s[g][1] = s[g][2] = s[g][3] = R(0);
+#pragma unroll(Sub == 1 ? kMaxSub : 1)
+    for (int i = 0; i < (Sub == 1 ? kMaxSub : m); i++) {
+      const R* p = acc + (size_t)min(i, m - 1) * 4 * GA;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+#pragma unroll
+        for (int g = 0; g < GA; g++) {
+          if (g < ng) {
+            const R x = p[v * GA + g];
+            s[g][v] += i < m ? x : R(0);
+          }
+        }
+      }
+    }
+#pragma unroll(Sub == 1 ? kMaxSub : 1)
+    for (int i = 0; i < kMaxSub; i++) {
+      if (i < m) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+#pragma unroll
+          for (int g = 0; g < GA; g++)
+            if (g < ng) acc[(size_t)i * 4 * GA + v * GA + g] = R(0);
+        }
+      }
+    }
 #pragma unroll
     for (int g = 0; g < GA; g++) {
       if (g < ng) {
-        R s[4] = {R(0), R(0), R(0), R(0)};
-#pragma unroll(Sub == 1 ? kMaxSub : 1)
-        for (int i = 0; i < (Sub == 1 ? kMaxSub : m); i++) {
-          const int ii = min(i, m - 1);
-#pragma unroll
-          for (int v = 0; v < 4; v++) {
-            const R x = acc[(v * GA + g) * st + ii];
-            s[v] += i < m ? x : R(0);
-          }
-        }
-#pragma unroll(Sub == 1 ? kMaxSub : 1)
-        for (int i = 0; i < kMaxSub; i++) {
-          if (i < m) {
-#pragma unroll
-            for (int v = 0; v < 4; v++) acc[(v * GA + g) * st + i] = R(0);
-          }
-        }
-        Ue[g] += s[0];
-        Ux[g] += s[1];
-        Uy[g] += s[2];
-        Uz[g] += s[3];
+        Ue[g] += s[g][0];
+        Ux[g] += s[g][1];
+        Uy[g] += s[g][2];
+        Uz[g] += s[g][3];
       }
     }
   }
@@ -801,21 +811,20 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
         const unsigned long long mine = rmask & grp;
         const int pr = pairD[d];
         if (rj >= 0 && pr >= 0) {
-          float* acc = a.cracc + (size_t)pr * kRefluxTiled;
-          const size_t st = (size_t)a.cnslot;
+          float* acc = a.cracc + (size_t)pr * kRefluxTiled * 4 * G;
           const float w = (bit ? 1.f : -1.f) * (0.5f * dts) * 0.125f;  // pow(-1, face+1) dt/dx / 8, shares doubled
           if (mine == grp) {  // the face's four cells all carry flux (always, with 2:1 balance)
             const int k = __popcll(mine & ((1ull << lane) - 1ull));  // component of this lane
 #pragma unroll
             for (int g = 0; g < G; g++) {
               const float x = k == 0 ? Rf[g][0] : (k == 1 ? Rf[g][1] : (k == 2 ? Rf[g][2] : Rf[g][3]));
-              atomicAdd(acc + (k * G + g) * st, x * w);
+              atomicAdd(acc + k * G + g, x * w);
             }
           } else if (lane == __builtin_ctzll(mine)) {
 #pragma unroll
             for (int g = 0; g < G; g++)
 #pragma unroll
-              for (int v = 0; v < 4; v++) atomicAdd(acc + (v * G + g) * st, Rf[g][v] * w);
+              for (int v = 0; v < 4; v++) atomicAdd(acc + v * G + g, Rf[g][v] * w);
           }
         }
       }
