@@ -114,6 +114,7 @@ __device__ __forceinline__ void restrict_cell(const LevelArgs<R>& a, int cell, i
   const int n = a.ncell, nf = a.nfine, j0 = ch * 8;
   for (int g = 0; g < ng; g++) {
     R se = R(0), sx = R(0), sy = R(0), sz = R(0), ss = R(0);
+#pragma unroll
     for (int k = 0; k < 8; k++) {
       // plain indexing: the eight children's values merge into wide loads
       const int j = g * nf + j0 + k;
@@ -129,13 +130,11 @@ __device__ __forceinline__ void restrict_cell(const LevelArgs<R>& a, int cell, i
     gst(a.fzo + g * n, cell, sz);
     gst(a.src + g * n, cell, ss);
   }
-  double snp = 0., sei = 0., stp = 0., snh = 0., stf = 0., stl = 0.;
+  // (no branch inside the loop: the 32 loads go out together)
+  double snp = 0., sei = 0., stp = 0., snh = 0.;
+#pragma unroll
   for (int k = 0; k < 8; k++) {
     const int j = j0 + k;
-    if (a.t_first != nullptr) {
-      stf += a.ft_first[j] * 0.125;
-      stl += a.ft_last[j] * 0.125;
-    }
     snp += a.fnhplus[j] * 0.125;
     sei += a.feint[j] * 0.125;
     stp += a.ftemp[j] * 0.125;
@@ -146,6 +145,12 @@ __device__ __forceinline__ void restrict_cell(const LevelArgs<R>& a, int cell, i
   gst(a.tempo, cell, stp);
   gst(a.nh, cell, snh);
   if (a.t_first != nullptr) {
+    double stf = 0., stl = 0.;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      stf += a.ft_first[j0 + k] * 0.125;
+      stl += a.ft_last[j0 + k] * 0.125;
+    }
     a.t_first[cell] = stf;
     a.t_last[cell] = stl;
   }
