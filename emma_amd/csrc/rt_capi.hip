@@ -146,6 +146,15 @@ static void drop_graphs(emma_rt_ctx* c) {
 }
 
 // chemrad's per-call constants (chem_utils.c:266-331), evaluated on the host
+// the photon densities' floor per group (chem_utils.c:382, 772) and, for FP32
+// storage, the smallest float at or above it
+static double emin_floor(const emma_rt_params& p, int g) { return kEmin * p.factgrp[g] / p.unit_N; }
+static float emin_floor_up(double f) {
+  float up = (float)f;
+  if ((double)up < f) up = nextafterf(up, INFINITY);
+  return up;
+}
+
 static ChemConst make_chem(const emma_rt_params& p, double dtnew, double aexp) {
   ChemConst K;
   memset(&K, 0, sizeof(K));
@@ -184,7 +193,8 @@ static ChemConst make_chem(const emma_rt_params& p, double dtnew, double aexp) {
     K.heat[g] = p.alphae[g] * p.hnu[g] - p.alphai[g] * kHnu0;
     K.alphai[g] = p.alphai[g];
     K.factgrp[g] = p.factgrp[g];
-    K.emin_grp[g] = kEmin * p.factgrp[g] / p.unit_N;
+    K.emin_grp[g] = emin_floor(p, g);
+    K.emin_grp_up[g] = emin_floor_up(K.emin_grp[g]);
   }
   return K;
 }
@@ -442,8 +452,11 @@ static DevLevel* level_of(emma_rt_ctx* c, int level) {
   return &c->lev[level - c->lmin];
 }
 
+// floor (photon densities, FP32 storage): a value at or above the group's
+// floor must not round below it (ncell values per group)
 template <typename R>
-static int upload_r(void* dst, const double* src, size_t n, hipStream_t s) {
+static int upload_r(void* dst, const double* src, size_t n, hipStream_t s, const emma_rt_params* floor = nullptr,
+                    size_t ncell = 0) {
   if (sizeof(R) == sizeof(double)) {
     HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -451,6 +464,13 @@ static int upload_r(void* dst, const double* src, size_t n, hipStream_t s) {
   }
   std::vector<R> tmp(n);
   for (size_t i = 0; i < n; i++) tmp[i] = (R)src[i];
+  if (floor != nullptr) {
+    for (size_t i = 0; i < n; i++) {
+      const int g = (int)(i / ncell);
+      const double f = emin_floor(*floor, g);
+      if (src[i] >= f && (double)tmp[i] < f) tmp[i] = (R)emin_floor_up(f);
+    }
+  }
   HIPCHK(hipMemcpyAsync(dst, tmp.data(), n * sizeof(R), hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
   return EMMA_RT_OK;
@@ -470,8 +490,10 @@ static int download_r(double* dst, const void* src, size_t n, hipStream_t s) {
   return EMMA_RT_OK;
 }
 
-static int upload_rf(emma_rt_ctx* c, void* dst, const double* src, size_t n) {
-  return c->rbytes == 8 ? upload_r<double>(dst, src, n, c->stream) : upload_r<float>(dst, src, n, c->stream);
+static int upload_rf(emma_rt_ctx* c, void* dst, const double* src, size_t n, bool photon_floor = false,
+                     size_t ncell = 0) {
+  const emma_rt_params* fl = photon_floor ? &c->p : nullptr;
+  return c->rbytes == 8 ? upload_r<double>(dst, src, n, c->stream) : upload_r<float>(dst, src, n, c->stream, fl, ncell);
 }
 
 static int download_rf(emma_rt_ctx* c, double* dst, const void* src, size_t n) {
@@ -487,7 +509,7 @@ extern "C" int emma_rt_set_level(emma_rt_ctx* c, int level, const double* e, con
   const size_t gn = (size_t)c->p.ngrp * L->ncell;
   const int b = L->cur;
   int st;
-  if ((st = upload_rf(c, L->e[b], e, gn)) || (st = upload_rf(c, L->fx[b], fx, gn)) ||
+  if ((st = upload_rf(c, L->e[b], e, gn, true, L->ncell)) || (st = upload_rf(c, L->fx[b], fx, gn)) ||
       (st = upload_rf(c, L->fy[b], fy, gn)) || (st = upload_rf(c, L->fz[b], fz, gn)) ||
       (st = upload_rf(c, L->src, src, gn)))
     return st;

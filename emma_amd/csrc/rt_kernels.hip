@@ -287,7 +287,13 @@ __device__ __forceinline__ int finish_leaf(const LevelArgs<R>& a, const ChemCons
 #pragma unroll
   for (int g = 0; g < GA; g++) {
     if (g < ng) {
-      gst(a.eo + g * n, cell, (R)e[g]);
+      R ev = (R)e[g];
+      // FP32 storage: a density the chemistry left at (or just above) its
+      // floor must not round below it, or the next step would stop at the
+      // floor test (chem_utils.c:382); only the rounding is corrected
+      if constexpr (sizeof(R) == 4)
+        if (e[g] >= K.emin_grp[g] && ev < K.emin_grp_up[g]) ev = K.emin_grp_up[g];
+      gst(a.eo + g * n, cell, ev);
       gst(a.fxo + g * n, cell, fx[g]);
       gst(a.fyo + g * n, cell, fy[g]);
       gst(a.fzo + g * n, cell, fz[g]);

@@ -46,6 +46,7 @@ struct ChemConst {
   double alphai[kMaxGrp];
   double factgrp[kMaxGrp];
   double emin_grp[kMaxGrp];            // EMIN*factgrp/unit_N
+  float emin_grp_up[kMaxGrp];          // the smallest float >= emin_grp (mixed-path storage)
   // products of the conversion factors above, for the mixed-precision path
   // (which multiplies where the FP64 path divides)
   double k_e;      // unit_N / a^3            code N -> photons/m^3

@@ -436,3 +436,26 @@ def test_domain_boundaries(gpu_lib, oracle, table, G, reflect, precision, amr):
         else:
             assert err["x"] < MIXED_TOL and err["temp"] < MIXED_TOL and err["e"] < MIXED_TOL, (l, err)
             assert max(err["fx"], err["fy"], err["fz"]) < MIXED_TOL, (l, err)
+
+
+def test_mixed_photon_floor(gpu_lib, oracle):
+    """Cells whose photon density the chemistry clamps to its floor
+    (EMIN * factgrp / unit_N, chem_utils.c:382, 772) keep running: stored in
+    FP32, the floor 1e-11 would round down below itself (float(1e-11) <
+    1e-11) and stop the next step at the floor test; the mixed path stores
+    the smallest float at or above the floor instead, after the chemistry and
+    when the state is set from FP64.  A level at the floor
+    with no sources, four sub-cycles, against the oracle (one group: with
+    several, group 0's floor EMIN * factgrp[0] lies below the test's EMIN and
+    the reference itself stops there)."""
+    from emma_amd.params import e_min_code
+
+    W = synth.uniform_workload(3, "SB99_equiv_mono_TH", seed=31)
+    F = W.fields[3]
+    F.e[:] = e_min_code(W.params)
+    F.fx[:] = 0.0
+    F.fy[:] = 0.0
+    F.fz[:] = 0.0
+    g = run_pair(oracle, W, rt.MIXED, steps=4, mode="graph")
+    err = field_errors(g.get(3), oracle.get(3), W.params.cloc())
+    assert err["x"] < MIXED_TOL and err["temp"] < MIXED_TOL and err["e"] < MIXED_TOL, err

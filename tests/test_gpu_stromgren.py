@@ -1,0 +1,71 @@
+"""End-to-end known answer on the GPU: the Stromgren sphere test EMMA's
+init/stromgen setup runs (the WRADTEST source at the box corner, TRANSXM..ZP
+with REFXM/YM/ZM, the reference's check in utils/strom.i).  A uniform
+n_H = 1e-3 cm^-3 box of 6.6 kpc, x = 1.2e-3, T = 1e4 K, 5e48 photons/s from
+the corner cell (an eighth of the sphere), one group (SB99_equiv_mono_TH),
+c/100.  The ionised volume V of the octant gives the front radius
+R = (6 V / pi)^(1/3); the analytic front is R_S (1 - exp(-t/t_rec))^(1/3)
+with R_S = (3 Ndot / (4 pi n^2 alpha_B))^(1/3) and t_rec = 1 / (n alpha_B),
+alpha_B the reference's own fit (chem_utils.c:52-62) at the ionised gas's
+mean temperature (about 1.3-1.5e4 K under photoheating and cooling; at 1e4 K
+these are the 5.4 kpc and 122.4 Myr of utils/strom.i)."""
+import math
+
+import numpy as np
+import pytest
+
+from emma_amd import rt, synth
+from emma_amd import tree as T
+from emma_amd.params import ATOMIC_TABLES, MYR, RunParams, e_min_code, static_units
+from emma_amd.synth import Fields
+
+pytestmark = pytest.mark.gpu
+
+BOX_KPC = 6.6
+N_H = 1e3  # m^-3
+NDOT = 5e48
+KPC = 3.0857e19
+
+
+def front_history(precision, level=5, t_end=300.0, every=30.0):
+    P = RunParams.from_table(ATOMIC_TABLES["SB99_equiv_mono_TH"], 0.01, **static_units(N_H, BOX_KPC))
+    tr = T.with_boundaries(T.uniform_tree(level), trans=range(6), reflect=(0, 2, 4))
+    L = tr.level(level)
+    n = L.ncell
+    x0, t0 = 1.2e-3, 1e4
+    F = Fields(e=np.full((1, n), 10 * e_min_code(P)), fx=np.zeros((1, n)), fy=np.zeros((1, n)),
+               fz=np.zeros((1, n)), src=np.zeros((1, n)), nhplus=np.full(n, x0),
+               eint=np.asarray(P.eint_code(np.ones(n), x0, t0)), nh=np.ones(n), temp=np.full(n, t0))
+    g = rt.RadGPU(P, precision)
+    g.setup(P, tr, {level: F}, tphy=1e-9)  # the source shines from tphy > 0 on
+    g.set_oct_pos(level, synth.oct_corners(L))
+    g.set_stromgren_source(synth.Star(x=0.0, y=0.0, z=0.0, srcint=NDOT / (2e2 * 1.989e30)))
+    dt_myr = P.dt_rad(level) * P.unit_t / MYR
+    chunk = int(round(every / dt_myr))
+    dv = (BOX_KPC / (1 << level)) ** 3
+    hist, t = [], 0.0
+    while t < t_end - 1e-6:
+        g.coarse_step(1.0, chunk)
+        t += chunk * dt_myr
+        G = g.get(level)
+        x = G.nhplus / G.nh
+        ion = x > 0.5
+        radius = (6.0 * np.sum(x) * dv / math.pi) ** (1.0 / 3.0)
+        hist.append((t, radius, float(np.sum(G.temp * x * ion) / np.sum(x * ion)), float(x.max())))
+    g.close()
+    return hist
+
+
+def test_stromgren_front(gpu_lib, oracle):
+    runs = {p: front_history(p) for p in (rt.MIXED, rt.FP64)}
+    for (t, r_m, _, _), (_, r_d, _, _) in zip(runs[rt.MIXED], runs[rt.FP64]):
+        assert abs(r_m - r_d) < 1e-3 * r_d, (t, r_m, r_d)  # the two precisions agree
+    for t, radius, t_hii, xmax in runs[rt.MIXED]:
+        alpha_b = float(oracle.rates(np.array([t_hii]), np.array([1.0]), np.array([N_H]))["alpha_b"][0])
+        r_s = (3.0 * NDOT / (4.0 * math.pi * N_H**2 * alpha_b)) ** (1.0 / 3.0) / KPC
+        t_rec = 1.0 / (N_H * alpha_b) / MYR
+        r_an = r_s * (1.0 - math.exp(-t / t_rec)) ** (1.0 / 3.0)
+        print("t %.0f Myr: R %.3f kpc, analytic %.3f (T_HII %.0f K, R_S %.2f kpc, t_rec %.0f Myr)" %
+              (t, radius, r_an, t_hii, r_s, t_rec))
+        assert xmax > 0.999
+        assert abs(radius - r_an) < 0.08 * r_an, (t, radius, r_an)
