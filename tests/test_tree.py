@@ -66,3 +66,37 @@ def test_c4_level_counts():
     tr = T.amr_tree(7, [80000, 25000, 6000, 1200], S.lognormal_sampler(g, 1.5), rng)
     assert tr.noct() == [262144, 80000, 25000, 6000, 1200]
     tr.validate()
+
+
+def test_domain_boundary_codes():
+    """with_boundaries marks exactly the faces on the domain faces asked for
+    (by the reference's position test, rad_utils.c:1106-1213), reflective
+    where asked, on every level, and leaves the periodic neighbour table
+    (neicell) and the other entries alone; the codes are the C-ABI's."""
+    import os
+    import re
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = open(os.path.join(root, "include", "emma_rt.h")).read()
+    defs = dict(re.findall(r"#define (EMMA_RT_NEI_\w+) \((-?\d+)(?: - 1)?\)", hdr))
+    assert T.NEI_TRANSMISSIVE == -2147483647 - 1 and "EMMA_RT_NEI_TRANSMISSIVE" in defs
+    assert T.NEI_REFLECTIVE == int(defs["EMMA_RT_NEI_REFLECTIVE"])
+    W = synth.amr_workload(3, [60, 100], "SB99_equiv_mono_TH", seed=3, sigma=1.5, grid=16)
+    tb = T.with_boundaries(W.tree, trans=(0, 1, 3, 4), reflect=(0, 4))
+    for l in range(W.tree.lmin, W.tree.lmax + 1):
+        L, B = W.tree.level(l), tb.level(l)
+        last = (1 << (l - 1)) - 1
+        for f in range(6):
+            d, s = f >> 1, f & 1
+            on = L.pos[:, d] == (last if s else 0)
+            code = B.nei_oct[:, f]
+            if f in (0, 1, 3, 4):
+                want = T.NEI_REFLECTIVE if f in (0, 4) else T.NEI_TRANSMISSIVE
+                assert (code[on] == want).all(), (l, f)
+                assert (code[~on] == L.nei_oct[~on, f]).all(), (l, f)
+            else:
+                assert (code == L.nei_oct[:, f]).all(), (l, f)
+        assert np.array_equal(B.neicell, L.neicell)
+    corners = synth.oct_corners(W.tree.level(W.tree.lmin))
+    assert np.count_nonzero(tb.level(W.tree.lmin).nei_oct[:, 0] == T.NEI_REFLECTIVE) == np.count_nonzero(corners[:, 0] == 0.0)
+    tb.validate()
