@@ -27,31 +27,53 @@ NDOT = 5e48
 KPC = 3.0857e19
 
 
-def front_history(precision, level=5, t_end=300.0, every=30.0):
+def stromgren_tree(level, shell):
+    """the level, with the cells whose centre lies in the shell (r0, r1) (box
+    units) around the corner refined into a second level when given.  The
+    source cells stay unrefined: the reference's test source fills the cells
+    within 2^-lcoarse of the corner at the coarse level's rate per volume, so
+    finer cells there would change the luminosity (src_utils.c:343-390)."""
+    L = T.uniform_level(level)
+    if shell is None:
+        return T.build_tree([L])
+    r = np.linalg.norm((L.cell_pos() + 0.5) / (1 << level), axis=1)
+    split = np.nonzero((r > shell[0]) & (r < shell[1]))[0]
+    return T.build_tree([L, T.refine(L, split)])
+
+
+def front_history(precision, level=5, t_end=300.0, every=30.0, shell=None):
     P = RunParams.from_table(ATOMIC_TABLES["SB99_equiv_mono_TH"], 0.01, **static_units(N_H, BOX_KPC))
-    tr = T.with_boundaries(T.uniform_tree(level), trans=range(6), reflect=(0, 2, 4))
-    L = tr.level(level)
-    n = L.ncell
+    tr = T.with_boundaries(stromgren_tree(level, shell), trans=range(6), reflect=(0, 2, 4))
     x0, t0 = 1.2e-3, 1e4
-    F = Fields(e=np.full((1, n), 10 * e_min_code(P)), fx=np.zeros((1, n)), fy=np.zeros((1, n)),
-               fz=np.zeros((1, n)), src=np.zeros((1, n)), nhplus=np.full(n, x0),
-               eint=np.asarray(P.eint_code(np.ones(n), x0, t0)), nh=np.ones(n), temp=np.full(n, t0))
+    fields = {}
+    for l in range(tr.lmin, tr.lmax + 1):
+        n = tr.level(l).ncell
+        fields[l] = Fields(e=np.full((1, n), 10 * e_min_code(P)), fx=np.zeros((1, n)), fy=np.zeros((1, n)),
+                           fz=np.zeros((1, n)), src=np.zeros((1, n)), nhplus=np.full(n, x0),
+                           eint=np.asarray(P.eint_code(np.ones(n), x0, t0)), nh=np.ones(n), temp=np.full(n, t0))
     g = rt.RadGPU(P, precision)
-    g.setup(P, tr, {level: F}, tphy=1e-9)  # the source shines from tphy > 0 on
-    g.set_oct_pos(level, synth.oct_corners(L))
+    g.setup(P, tr, fields, tphy=1e-9)  # the source shines from tphy > 0 on
+    for l in range(tr.lmin, tr.lmax + 1):
+        g.set_oct_pos(l, synth.oct_corners(tr.level(l)))
     g.set_stromgren_source(synth.Star(x=0.0, y=0.0, z=0.0, srcint=NDOT / (2e2 * 1.989e30)))
     dt_myr = P.dt_rad(level) * P.unit_t / MYR
     chunk = int(round(every / dt_myr))
-    dv = (BOX_KPC / (1 << level)) ** 3
     hist, t = [], 0.0
     while t < t_end - 1e-6:
         g.coarse_step(1.0, chunk)
         t += chunk * dt_myr
-        G = g.get(level)
-        x = G.nhplus / G.nh
-        ion = x > 0.5
-        radius = (6.0 * np.sum(x) * dv / math.pi) ** (1.0 / 3.0)
-        hist.append((t, radius, float(np.sum(G.temp * x * ion) / np.sum(x * ion)), float(x.max())))
+        vol = xs = xt = 0.0
+        xmax = 0.0
+        for l in range(tr.lmin, tr.lmax + 1):  # leaf cells of every level
+            G = g.get(l)
+            leaf = tr.level(l).child < 0
+            x = (G.nhplus / G.nh)[leaf]
+            ion = x > 0.5
+            vol += np.sum(x) * (BOX_KPC / (1 << l)) ** 3
+            xs += np.sum(x * ion) * 0.5 ** (3 * l)
+            xt += np.sum(G.temp[leaf] * x * ion) * 0.5 ** (3 * l)
+            xmax = max(xmax, float(x.max()))
+        hist.append((t, (6.0 * vol / math.pi) ** (1.0 / 3.0), xt / xs, xmax))
     g.close()
     return hist
 
@@ -69,3 +91,4 @@ def test_stromgren_front(gpu_lib, oracle):
               (t, radius, r_an, t_hii, r_s, t_rec))
         assert xmax > 0.999
         assert abs(radius - r_an) < 0.08 * r_an, (t, radius, r_an)
+
