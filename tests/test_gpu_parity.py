@@ -171,12 +171,20 @@ def test_level_size_limit(gpu_lib, table, G):
 @pytest.mark.parametrize("table,G", [("SB99_equiv_mono_TH", 1), ("Starburst99_1grpT_3grpS", 3)])
 @pytest.mark.parametrize("mode", ["host", "graph"])
 @pytest.mark.parametrize("precision", [rt.FP64, rt.MIXED])
-def test_coarserad(gpu_lib, oracle, table, G, mode, precision):
+@pytest.mark.parametrize("faces", ["periodic", "trans_ref"])
+def test_coarserad(gpu_lib, oracle, table, G, mode, precision, faces):
     """COARSERAD: transport on the coarse level only, chemistry on every leaf,
-    putE2coarse before and distribE after the coarse step."""
+    putE2coarse before and distribE after the coarse step; also with the
+    domain faces of EMMA's Stromgren setup (TRANS* on all six, REF* on the
+    minus faces), the flag set init/stromgen/param.mk combines with
+    COARSERAD."""
+    from emma_amd import tree as T
     from oracle import COARSERAD, coarse_step_coarserad
 
     W = synth.amr_workload(3, [60, 100, 80], table, seed=20 + G, sigma=1.5, nsrc=3, grid=16)
+    if faces == "trans_ref":
+        tb = T.with_boundaries(W.tree, trans=range(6), reflect=(0, 2, 4))
+        W = synth.Workload(W.name, W.params, tb, W.fields, W.nsub)
     g = rt.RadGPU(W.params, precision)
     g.setup(W.params, W.tree, W.fields, mode=rt.COARSERAD)
     oracle.setup(W.params, W.tree, W.fields, mode=COARSERAD)
