@@ -41,7 +41,7 @@ def stromgren_tree(level, shell):
     return T.build_tree([L, T.refine(L, split)])
 
 
-def front_history(precision, level=5, t_end=300.0, every=30.0, shell=None):
+def front_history(precision, level=5, t_end=300.0, every=30.0, shell=None, coupled=False):
     P = RunParams.from_table(ATOMIC_TABLES["SB99_equiv_mono_TH"], 0.01, **static_units(N_H, BOX_KPC))
     tr = T.with_boundaries(stromgren_tree(level, shell), trans=range(6), reflect=(0, 2, 4))
     x0, t0 = 1.2e-3, 1e4
@@ -52,9 +52,19 @@ def front_history(precision, level=5, t_end=300.0, every=30.0, shell=None):
                            fz=np.zeros((1, n)), src=np.zeros((1, n)), nhplus=np.full(n, x0),
                            eint=np.asarray(P.eint_code(np.ones(n), x0, t0)), nh=np.ones(n), temp=np.full(n, t0))
     g = rt.RadGPU(P, precision)
-    g.setup(P, tr, fields, tphy=1e-9)  # the source shines from tphy > 0 on
+    # the source shines from tphy > 0 on; coupled: EMMA's init/stromgen flag
+    # set, COARSERAD with WRADHYD (FillRad copies n_H, eint, x n_H from a
+    # static hydro state, the final update writes p and x n_H back)
+    g.setup(P, tr, fields, tphy=1e-9, mode=rt.COARSERAD if coupled else rt.RADSTEP)
     for l in range(tr.lmin, tr.lmax + 1):
         g.set_oct_pos(l, synth.oct_corners(tr.level(l)))
+        if coupled:
+            F = fields[l]
+            z = np.zeros(F.ncell)
+            g.set_hydro(l, synth.Hydro(d=F.nh.copy(), u=z, v=z, w=z, p=(synth.GAMMA - 1.0) * F.eint,
+                                       dX=F.nhplus.copy(), E=z))
+    if coupled:
+        g.set_coupling(True)
     g.set_stromgren_source(synth.Star(x=0.0, y=0.0, z=0.0, srcint=NDOT / (2e2 * 1.989e30)))
     dt_myr = P.dt_rad(level) * P.unit_t / MYR
     chunk = int(round(every / dt_myr))
@@ -92,3 +102,15 @@ def test_stromgren_front(gpu_lib, oracle):
         assert xmax > 0.999
         assert abs(radius - r_an) < 0.08 * r_an, (t, radius, r_an)
 
+
+
+def test_stromgren_front_coupled(gpu_lib):
+    """The uniform sphere under the reference setup's flag set, COARSERAD
+    with the hydro coupling: the same front as the plain run (the hydro
+    state only carries n_H, eint and x n_H between the steps)."""
+    plain = front_history(rt.MIXED, t_end=120.0)
+    coupled = front_history(rt.MIXED, t_end=120.0, coupled=True)
+    for (t, r_p, _, _), (_, r_c, _, xmax) in zip(plain, coupled):
+        print("t %.0f Myr: R %.3f kpc (COARSERAD + hydro coupling) vs %.3f" % (t, r_c, r_p))
+        assert xmax > 0.999
+        assert abs(r_c - r_p) < 1e-3 * r_p, (t, r_c, r_p)
