@@ -567,11 +567,16 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
       Uy[g] = gld(a.fy + g * n, cell);
       Uz[g] = gld(a.fz + g * n, cell);
     }
+    // the chemistry inputs: early on complete levels (their latency hides
+    // behind the transport); on sparse levels, whose per-direction halo
+    // loads need the registers, when the chemistry starts (finish_leaf)
+    if constexpr (!PerDir) {
 #pragma unroll
-    for (int g = 0; g < G; g++) Src[g] = gld(a.src + g * n, cell);
-    pre[0] = gld(a.nhplus, cell);
-    pre[1] = gld(a.eint, cell);
-    pre[2] = gld(a.nh, cell);
+      for (int g = 0; g < G; g++) Src[g] = gld(a.src + g * n, cell);
+      pre[0] = gld(a.nhplus, cell);
+      pre[1] = gld(a.eint, cell);
+      pre[2] = gld(a.nh, cell);
+    }
   }
   sCancel[lc] = (unsigned char)(ch >= 0 && !a.keep_split_flux);
 
@@ -861,7 +866,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
     } else {
       leaf = 1;
       nit_acc = (unsigned)finish_leaf<float, G, true, kRefluxTiled>(a, K, cell, ch, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz,
-                                                                    Src, pre);
+                                                                    PerDir ? nullptr : Src, PerDir ? nullptr : pre);
     }
   }
   if (a.counters != nullptr) block_counters<kTileWaves>(a.counters, nit_acc, leaf);
