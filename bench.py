@@ -1,38 +1,40 @@
 """Benchmark of the M1 + chemistry radiation step on one MI355X.
 
 Workloads (seeded synthetic states; see emma_amd/synth.py):
-  c3 (default) 128^3 periodic level (level 7, 2,097,152 cells), 3 photon groups,
-               64 point sources, lognormal n_H; mixed precision.  A step is one
-               radiative sub-cycle of the level; the K timed sub-cycles are one
-               replay of their captured graph (captured and run once on the
-               same starting state, untimed).
-  c2           64^3 periodic level, 1 group; FP64 plain.  A step is one sub-cycle.
-  c5           256^3 coarse level (level 8) + 300000/75000 refined octs, 3 groups,
-               4096 sources, c/100; timed like c4.
+  c5 (default) reionisation-like box: 256^3 coarse level (level 8, 16.8 M cells)
+               + 300000/75000 refined octs (levels 9-10), lognormal sigma=2,
+               3 groups, 4096 sources with dN/dL ~ L^-2, c/100; mixed precision.
+               A step is one coarse step of the RADSTEP level recursion (level l
+               takes 2^(l-8) level steps, 7 launches), one replay of its HIP
+               graph, from the state after the first coarse step (run_amr).
   c4           AMR levels 7-11 with 262144/80000/25000/6000/1200 octs, 3 groups,
-               mixed precision.  A step is one coarse step of the RADSTEP level
-               recursion (level l takes 2^(l-7) sub-steps, 31 launches), replayed as
-               one HIP graph from the state after the first coarse step (run_amr).
+               mixed precision; a step is one coarse step (31 launches), timed
+               like c5.  The c5 line carries a c4 block as well.
+  c3           128^3 periodic level (level 7, 2,097,152 cells), 3 groups, 64
+               sources; a step is one radiative sub-cycle, the K timed
+               sub-cycles one replay of their captured graph.
+  c2           64^3 periodic level, 1 group; FP64 plain.
 
 Prints ONE JSON line.  value = leaf cell-updates/s of the timed region (state
 resident in HBM; the library call is synchronous, so the wall clock brackets
-complete device work).  roofline.achieved = algorithmic bytes of the timed
-level-step launches / their device time, taken from the same graph replays run
-once more right after the timed region between HIP events on the library's
-stream (the events add host overhead, so the value's replays run without them;
-the gaps between launches inside the graph count as kernel time);
-mean_kernel_us = that device time / launches.  fp64_plain: the same
-workload stepped by the FP64 kernel.  cpu_baseline: the reference's CPU path (oracle/_ref, compiled from the
-reference's own sources) or, where that build is absent, the oracle (its CPU
-restatement), on up to 16 host cores, one process each, on a bounded sample
-drawn like the workload.
+complete device work).  roofline = the dominant level-step kernel (the level
+with the most device time per step): algorithmic bytes of one launch (leaf
+cells x bytes per cell-update) over its mean launch duration, measured right
+after the timed region with HIP events around every kernel on the library's
+stream (coarse steps launched eagerly for this); config.device_ms_per_step is
+the device time of one graph replay between a pair of HIP events, and
+config.step_alg_frac the whole step's algorithmic bytes over it.  fp64_plain:
+the same workload stepped by the FP64 kernel.  cpu_baseline: the reference's
+CPU path (oracle/_ref, compiled from the reference's own sources) or, where that
+build is absent, the oracle (its CPU restatement), one process per host core
+of the job's CPU share, on a bounded sample drawn like the workload.
 
 --stencil-path measures the drop-in path instead (advanceradGPU's contract: host
 struct RGRID batches in, rfieldnew and fluxes back) on the 64^3, 1-group level in
 batches of 8192 octs, --steps sweeps over the level after --warmup; its line
 carries the PCIe-inclusive rate under its own metric name.
 
---calibrate-copy runs a plain device copy of known size (for scaling the
+--calibrate-copy runs a plain device copy kernel of known size (for scaling the
 FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc runs).
 """
 from __future__ import annotations
@@ -86,7 +88,7 @@ def _cpu_worker(args):
     from emma_amd import synth
     from oracle import Oracle, Reference
 
-    if name in ("c3", "c4"):
+    if name in ("c3", "c4", "c5"):
         W = synth.uniform_workload(5, "Starburst99_1grpT_3grpS", seed=seed, nsrc=1)
     else:
         W = synth.uniform_workload(5, "SB99_equiv_mono_TH", seed=seed)
@@ -104,6 +106,20 @@ def _cpu_worker(args):
     return W.tree.level(5).ncell * n, el
 
 
+def host_cores() -> int:
+    """Host cores this job may use: the thread count the machine allots to
+    the job (OMP_NUM_THREADS, which the GPU machines set to the job's CPU
+    share) or else the CPUs the process may run on (its affinity mask);
+    os.cpu_count() counts the whole host, which other jobs share."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        return max(1, os.cpu_count() or 1)
+
+
 def cpu_baseline(name: str, seconds: float = 10.0, cores: int = 0):
     """The reference CPU path (oracle/_ref, compiled from the reference's own
     sources) or, where that build is absent, the oracle, on `cores` host
@@ -115,7 +131,7 @@ def cpu_baseline(name: str, seconds: float = 10.0, cores: int = 0):
     G = 1 if name == "c2" else 3
     use_ref = ref_available(G)
     if cores <= 0:
-        cores = max(1, min(16, os.cpu_count() or 1))
+        cores = host_cores()
     ctx = mp.get_context("spawn")
     try:
         with ctx.Pool(cores) as pool:
@@ -133,6 +149,8 @@ def cpu_baseline(name: str, seconds: float = 10.0, cores: int = 0):
         "value": rate,
         "unit": "cell-updates/s",
         "cores": cores,
+        "per_core": rate / cores,
+        "host_cpus_visible": os.cpu_count(),
         "kind": "reference" if use_ref else "port",
         "sample": f"{'reference' if use_ref else 'oracle'} RadSolver (clean_new_rad + RadSolver) on a 32^3 periodic "
                   f"level (level 5, 32768 cells) drawn like the workload, one process per core, {nsub} sub-cycles "
@@ -154,7 +172,53 @@ def level_bytes(W, precision):
     return {l: W.tree.leaf_cells(l) * Bc for l in range(W.tree.lmin, W.tree.lmax + 1)}
 
 
+def _dominant(W, precision, level_us, launches_per_step):
+    """The level-step kernel with the most device time per step: its level,
+    mean launch duration and roofline (leaf cells x algorithmic bytes per
+    cell-update over that duration)."""
+    lb = level_bytes(W, precision)
+    per_step = {l: level_us[l] * launches_per_step[l] for l in level_us}
+    l = max(per_step, key=per_step.get)
+    achieved = lb[l] / (level_us[l] * 1e-6) / 1e9
+    return l, achieved
+
+
+def _halo_per_dir(L) -> bool:
+    """The tiled kernel's variant choice (made by the library at tree
+    creation): some wave of 64 cells has more than 64 faces whose neighbour
+    oct lies outside the cell's 32-oct tile."""
+    import numpy as np
+
+    nei = L.nei_oct.reshape(-1, 6)
+    cells = np.arange(L.ncell)
+    oct_, ic, oct0 = cells >> 3, cells & 7, (cells // 256) * 32
+    cnt = np.zeros(L.ncell, dtype=np.int64)
+    for d in range(3):
+        no = nei[oct_, 2 * d + ((ic >> d) & 1)]
+        cnt += ~((no >= oct0) & (no < oct0 + 32))
+    pad = (-L.ncell) % 64
+    return bool(np.pad(cnt, (0, pad)).reshape(-1, 64).sum(1).max() > 64)
+
+
+def _kernel_name(W, level, precision):
+    if precision != "mixed" or W.params.ngrp not in (1, 3):
+        return f"emma::level_step_kernel<{'double' if precision == 'fp64' else 'float'}, {W.params.ngrp}, ...>"
+    per_dir = _halo_per_dir(W.tree.level(level))
+    return f"emma::level_step_tiled<{W.params.ngrp}, {'true' if per_dir else 'false'}>"
+
+
+def _level_table(W, precision, level_us, launches_per_step):
+    lb = level_bytes(W, precision)
+    return {str(l): {"leaf_cells": W.tree.leaf_cells(l), "launches_per_step": launches_per_step[l],
+                     "us_per_launch": level_us[l],
+                     "frac": lb[l] / (level_us[l] * 1e-6) / 1e9 / HBM_PEAK_GBS if level_us[l] > 0 else None}
+            for l in sorted(level_us)}
+
+
 def run_uniform(W, precision, steps, warmup):
+    """A step is one radiative sub-cycle of the level.  The K timed sub-cycles
+    are one replay of their captured graph (captured and run once on the same
+    starting state, untimed)."""
     from emma_amd import rt
 
     lvl = W.tree.lmin
@@ -176,33 +240,38 @@ def run_uniform(W, precision, steps, warmup):
     g.coarse_step(1.0, steps)  # single level: `steps` sub-cycles, one launch each
     wall = time.perf_counter() - t0
     st = g.stats()
-    # the same replay again, between HIP events, for the device time
+    # the same replay again, between one pair of HIP events
     g.set(lvl, 0, start)
     g.enable_timing(rt.TIMING_GRAPH)
     g.coarse_step(1.0, steps)
-    kernel_s = g.stats()["kernel_ms"] * 1e-3
-    bytes_total = level_bytes(W, precision)[lvl] * st["level_steps"]
-    return wall, st, kernel_s, bytes_total, st["level_steps"]
+    graph_ms = g.stats()["kernel_ms"]
+    # and launched eagerly with events around every kernel (kernel durations)
+    g.set(lvl, 0, start)
+    g.enable_timing(rt.TIMING_LAUNCH)
+    g.coarse_step(1.0, steps)
+    ms, n = g.level_times()[lvl]
+    g.close()
+    return {"wall": wall, "stats": st, "graph_ms": graph_ms, "level_us": {lvl: ms / max(n, 1) * 1e3},
+            "launches_per_step": {lvl: 1}, "steps": steps, "alg_bytes_per_step": level_bytes(W, precision)[lvl]}
 
 
 def run_amr(W, precision, steps, warmup):
-    """Every timed coarse step is the second coarse step of the run: on this
-    seeded input dense neutral cells drain to the EMIN floor and the reference
-    path aborts in the third coarse step (chem_utils.c:382-385; the oracle
+    """A step is one coarse step of the RADSTEP level recursion (level l takes
+    2^(l-lmin) level steps), one replay of its captured HIP graph.  Every
+    timed coarse step is the second coarse step of the run: on these seeded
+    inputs dense neutral cells drain to the EMIN floor, where the reference
+    path aborts within a few coarse steps (chem_utils.c:382-385; the oracle
     reproduces it), so the state after the first coarse step (taken from the
-    drawn state, untimed) is restored (host -> device, outside the timed region)
-    before every timed coarse step.  A coarse step is one replay of the captured
-    graph."""
+    drawn state, untimed) is restored from its device-side copy
+    (emma_rt_restore_state, outside the timed region) before every timed
+    coarse step."""
     from emma_amd import rt
 
     g = rt.RadGPU(W.params, rt.MIXED if precision == "mixed" else rt.FP64)
     g.setup(W.params, W.tree, W.fields)
     g.coarse_step(1.0, 1)  # first coarse step, from the drawn state
-    start = {l: g.get(l) for l in W.fields}
-
-    def restore():
-        for l, F in start.items():
-            g.set(l, 0, F)
+    g.save_state()
+    restore = g.restore_state
 
     # capture + instantiate the graphs of both buffer parities of the coarse level
     for _ in range(max(warmup, 2)):
@@ -218,16 +287,53 @@ def run_amr(W, precision, steps, warmup):
         st = g.stats()
         for k in tot:
             tot[k] += st[k]
-    # the same replays again, between HIP events, for the device time
+    # the same replays again, each between one pair of HIP events
     g.enable_timing(rt.TIMING_GRAPH)
-    kernel_ms = 0.0
+    graph_ms = 0.0
     for _ in range(steps):
         restore()
         g.coarse_step(1.0, 1)
-        kernel_ms += g.stats()["kernel_ms"]
+        graph_ms += g.stats()["kernel_ms"]
+    # and launched eagerly with events around every kernel (kernel durations)
+    g.enable_timing(rt.TIMING_LAUNCH)
+    acc = {}
+    nrep = max(1, min(steps, 5))
+    for _ in range(nrep):
+        restore()
+        g.coarse_step(1.0, 1)
+        for l, (ms, n) in g.level_times().items():
+            a = acc.setdefault(l, [0.0, 0])
+            a[0] += ms
+            a[1] += n
+    g.close()
+    level_us = {l: a[0] / max(a[1], 1) * 1e3 for l, a in acc.items() if a[1] > 0}
+    launches = {l: 1 << (l - W.tree.lmin) for l in level_us}
     lb = level_bytes(W, precision)
     per_coarse = sum(b << (l - W.tree.lmin) for l, b in lb.items())
-    return wall, tot, kernel_ms * 1e-3, per_coarse * steps, tot["level_steps"]
+    return {"wall": wall, "stats": tot, "graph_ms": graph_ms, "level_us": level_us, "launches_per_step": launches,
+            "steps": steps, "alg_bytes_per_step": per_coarse}
+
+
+def measure(name, W, precision, steps, warmup):
+    """One workload's timed region and device times, as the JSON fields of a
+    bench line (value, roofline of the dominant kernel, per-level table)."""
+    runner = run_amr if name in ("c4", "c5") else run_uniform
+    r = runner(W, precision, steps, warmup)
+    cu = r["stats"]["cell_updates"]
+    lvl, achieved = _dominant(W, precision, r["level_us"], r["launches_per_step"])
+    step_ms = r["graph_ms"] / steps
+    return r, {
+        "value": cu / r["wall"],
+        "ms_per_step": r["wall"] * 1e3 / steps,
+        "device_ms_per_step": step_ms,
+        "step_alg_frac": r["alg_bytes_per_step"] / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "chem_substeps_per_cell_update": r["stats"]["chem_substeps"] / max(cu, 1),
+        "dominant_level": lvl,
+        "kernel": _kernel_name(W, lvl, precision),
+        "kernel_us": r["level_us"][lvl],
+        "achieved": achieved,
+        "levels": _level_table(W, precision, r["level_us"], r["launches_per_step"]),
+    }
 
 
 def stencil_path(steps: int, warmup: int, stride: int = 8192) -> dict:
@@ -288,17 +394,22 @@ def stencil_path(steps: int, warmup: int, stride: int = 8192) -> dict:
 
 
 def copy_calibration(nbytes: int = 1 << 30, reps: int = 10):
+    """A shader copy of known size: an elementwise kernel reading `nbytes` and
+    writing `nbytes` (b = a + 1; a same-dtype copy_ may go to the DMA engines,
+    which the shader counters do not see)."""
     import torch
 
     a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
     b = torch.empty_like(a)
+    torch.add(a, 1.0, out=b)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
-        b.copy_(a)
+        torch.add(a, 1.0, out=b)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps,
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps + 1,
+                      "kernel": "torch elementwise add (b = a + 1), float32",
                       "GBps": 2 * nbytes * reps / el / 1e9}), flush=True)
 
 
@@ -307,12 +418,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c4", "c5"])
+    ap.add_argument("--workload", default="c5", choices=["c3", "c2", "c4", "c5"])
     ap.add_argument("--precision", default=None, choices=["mixed", "fp64"])
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--cpu-cores", type=int, default=0, help="host cores for the CPU baseline (0: up to 16)")
+    ap.add_argument("--cpu-cores", type=int, default=0,
+                    help="host cores for the CPU baseline (0: the job's CPU share, see host_cores())")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c4", action="store_true", help="c5 only: leave out the c4 block")
     ap.add_argument("--no-fp64-plain", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--stencil-path", action="store_true")
@@ -330,22 +443,20 @@ def main():
 
     precision = args.precision or ("fp64" if args.workload == "c2" else "mixed")
     W = make_workload(args.workload, args.seed)
-    runner = run_amr if args.workload in ("c4", "c5") else run_uniform
-    wall, st, kernel_s, bytes_per_unit, launches = runner(W, precision, args.steps, args.warmup)
-    cell_updates = st["cell_updates"]
-    value = cell_updates / wall
-    # device time of the timed graph replays (HIP events on the library's
-    # stream around each replay) over their algorithmic bytes and launches
-    achieved = bytes_per_unit / kernel_s / 1e9
-    mean_launch_us = kernel_s / max(launches, 1) * 1e6
+    print(f"bench: workload {args.workload} ({precision}), {args.steps} steps after {args.warmup} warm-up",
+          file=sys.stderr, flush=True)
+    _, m = measure(args.workload, W, precision, args.steps, args.warmup)
+    step = {"c4": "coarse step: 31 level steps (levels 7-11), one HIP graph replay",
+            "c5": "coarse step: 7 level steps (levels 8-10), one HIP graph replay"}.get(args.workload,
+                                                                                    "radiative sub-cycle")
     out = {
         "metric": "M1+chem cell-updates/s (mixed+batched vs FP64 plain, 1 GPU) and % HBM peak",
-        "value": value,
+        "value": m["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -354,31 +465,51 @@ def main():
         "config": {
             "workload": WORKLOAD_NAMES[args.workload],
             "cells": sum(W.tree.level(l).ncell for l in range(W.tree.lmin, W.tree.lmax + 1)),
+            "leaf_cell_updates_per_step": W.leaf_cell_updates_per_coarse_step(),
             "groups": W.params.ngrp,
             "precision": precision,
-            "step": ("second coarse step (31 level steps, HIP graph)" if args.workload == "c4"
-                     else "second coarse step (7 level steps, HIP graph)" if args.workload == "c5"
-                     else "radiative sub-cycle"),
-            "chem_substeps_per_cell_update": st["chem_substeps"] / max(cell_updates, 1),
-            "mean_kernel_us": mean_launch_us,
+            "step": step,
+            "device_ms_per_step": m["device_ms_per_step"],
+            "step_alg_frac": m["step_alg_frac"],
+            "chem_substeps_per_cell_update": m["chem_substeps_per_cell_update"],
+            "levels": m["levels"],
             "parallelism": "1 GPU",
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": m["achieved"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": m["achieved"] / HBM_PEAK_GBS,
             "traffic": load_traffic(f"{args.workload}_{precision}"),
+            "kernel": m["kernel"],
+            "level": m["dominant_level"],
+            "kernel_us": m["kernel_us"],
             "alg_bytes_per_cell_update": alg_bytes_per_cell(W.params.ngrp, precision),
         },
     }
     if precision == "mixed" and not args.no_fp64_plain:
         nsteps = max(4, args.steps // 4)
-        w2, st2, _, _, _ = runner(W, "fp64", nsteps, args.warmup)
-        v2 = st2["cell_updates"] / w2
-        out["fp64_plain"] = {"value": v2, "unit": "cell-updates/s", "steps": nsteps, "speedup_of_value": value / v2}
+        print(f"bench: fp64 plain, {nsteps} steps", file=sys.stderr, flush=True)
+        _, m2 = measure(args.workload, W, "fp64", nsteps, min(args.warmup, 2))
+        out["fp64_plain"] = {"value": m2["value"], "unit": "cell-updates/s", "steps": nsteps,
+                             "device_ms_per_step": m2["device_ms_per_step"], "kernel": m2["kernel"],
+                             "kernel_us": m2["kernel_us"], "frac": m2["achieved"] / HBM_PEAK_GBS,
+                             "speedup_of_value": m["value"] / m2["value"]}
+    if args.workload == "c5" and not args.no_c4:
+        # the batched AMR tree of levels 7-11 beside it (more, smaller levels)
+        del W
+        W4 = make_workload("c4", 4)
+        n4 = min(args.steps, 10)
+        print(f"bench: c4 block, {n4} steps", file=sys.stderr, flush=True)
+        _, m4 = measure("c4", W4, precision, n4, min(args.warmup, 3))
+        out["c4"] = {"workload": WORKLOAD_NAMES["c4"], "step": "coarse step: 31 level steps, one HIP graph replay",
+                     "value": m4["value"], "unit": "cell-updates/s", "steps": n4, "ms_per_step": m4["ms_per_step"],
+                     "device_ms_per_step": m4["device_ms_per_step"], "step_alg_frac": m4["step_alg_frac"],
+                     "kernel": m4["kernel"], "level": m4["dominant_level"], "kernel_us": m4["kernel_us"],
+                     "frac": m4["achieved"] / HBM_PEAK_GBS, "levels": m4["levels"]}
     if not args.no_cpu_baseline:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds, args.cpu_cores)
     print(json.dumps(out), flush=True)
 

@@ -62,6 +62,18 @@ struct DevLevel {
   // corners for the test source; allocated on first use
   double* hyd[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // d u v w p dX E
   double* opos = nullptr;
+  // star particles of the level's leaf cells (CSR over cells), set by
+  // emma_rt_set_stars; star_first == nullptr: none
+  int* star_first = nullptr;
+  double* star_mass = nullptr;
+  double* star_age = nullptr;
+  int* star_rs = nullptr;
+  int* star_is = nullptr;
+  // device-side restart copy (emma_rt_save_state): one allocation, the
+  // parts in the order of snap_parts(), and the buffer parity it was taken at
+  char* snap = nullptr;
+  size_t snap_bytes = 0;
+  int snap_cur = -1;
 };
 
 struct GraphKey {
@@ -101,10 +113,16 @@ struct emma_rt_ctx {
   // FillRad / final-update coupling (WRADHYD) and the Stromgren test source
   int hydro = 0;
   int star = 0;
-  double star_x = 0., star_y = 0., star_z = 0., star_mass = 0., star_lifetime = 0., star_srcint = 0.,
-         star_tb0 = 0.;
+  double star_x = 0., star_y = 0., star_z = 0., star_mass = 0., star_lifetime = 0., star_srcint = 0.;
+  // star-particle sources (stars_sources): srcint, tlife and the
+  // DECREASE_EMMISIVITY_AFTER_TLIFE switch of the reference
+  int stars = 0, stars_decrease = 0;
+  double stars_srcint = 0., stars_tlife = 1.;
   std::vector<hipEvent_t> ev;  // pairs
   size_t ev_used = 0;
+  std::vector<int> ev_level;   // level index of each pair, -1 for a whole graph replay
+  double level_ms[kMaxLevels];  // device time per level of the last timed call
+  int level_launches[kMaxLevels];
   std::map<GraphKey, std::pair<hipGraphExec_t, std::vector<int>>> graphs;
   // stencil path
   void* d_stencil = nullptr;
@@ -133,6 +151,9 @@ static void free_tree(emma_rt_ctx* c) {
     (void)hipFree(L.t_last);
     for (int k = 0; k < 7; k++) (void)hipFree(L.hyd[k]);
     (void)hipFree(L.opos);
+    (void)hipFree(L.snap);
+    for (void* p : {(void*)L.star_first, (void*)L.star_mass, (void*)L.star_age, (void*)L.star_rs, (void*)L.star_is})
+      (void)hipFree(p);
   }
   c->lev.clear();
   for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.first);
@@ -208,6 +229,12 @@ static double dtrad_of(const emma_rt_params& p, int level, double aexp) {
 static int check_params(const emma_rt_params* p) {
   if (p == nullptr || p->ngrp < 1 || p->ngrp > EMMA_RT_MAX_GRP) return EMMA_RT_ERR_ARG;
   if (!(p->unit_t > 0) || !(p->unit_v > 0) || !(p->unit_N > 0) || !(p->clight > 0)) return EMMA_RT_ERR_ARG;
+  // time groups of the atomic table (readAtomic, parameters.c:455-468):
+  // ngrp = NGRP_SPACE x ngrp_time, bounds in years, non-decreasing
+  if (p->ngrp_time < 1 || p->ngrp % p->ngrp_time != 0) return EMMA_RT_ERR_ARG;
+  for (int i = 0; i < p->ngrp_time; i++)
+    if (!(p->time_bound[i] <= p->time_bound[i + 1])) return EMMA_RT_ERR_ARG;
+  if (p->lcoarse < 0) return EMMA_RT_ERR_ARG;
   return EMMA_RT_OK;
 }
 
@@ -262,6 +289,8 @@ extern "C" int emma_rt_create(const emma_rt_params* p, emma_rt_ctx** out) {
   emma_rt_ctx* c = new emma_rt_ctx();
   c->p = *p;
   memset(&c->stats, 0, sizeof(c->stats));
+  memset(c->level_ms, 0, sizeof(c->level_ms));
+  memset(c->level_launches, 0, sizeof(c->level_launches));
   if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
       hipMalloc((void**)&c->d_status, 4 * sizeof(int)) != hipSuccess ||
       hipMalloc((void**)&c->d_counters, kCounterSlots * kCounterStride * sizeof(unsigned long long)) != hipSuccess ||
@@ -692,6 +721,7 @@ static int timed_launch(emma_rt_ctx* c, int k, double dtnew, double aexp, int ch
     e0 = next_event(c);
     e1 = next_event(c);
     if (!e0 || !e1) return EMMA_RT_ERR_HIP;
+    c->ev_level.push_back(k);
     HIPCHK(hipEventRecord(e0, c->stream));
   }
   HIPCHK(launch_level(c, k, dtnew, aexp, chemonly, true));
@@ -705,6 +735,7 @@ static int begin_call(emma_rt_ctx* c) {
   HIPCHK(launch_fill((unsigned int*)c->d_counters, 2 * kCounterSlots * kCounterStride, 0u, c->stream));
   memset(&c->stats, 0, sizeof(c->stats));
   c->ev_used = 0;
+  c->ev_level.clear();
   return EMMA_RT_OK;
 }
 
@@ -725,12 +756,22 @@ static int end_call(emma_rt_ctx* c) {
   c->stats.chem_substeps = (long long)cnt[0];
   c->stats.cell_updates = (long long)cnt[1];
   double ms = 0.;
+  for (int k = 0; k < kMaxLevels; k++) {
+    c->level_ms[k] = 0.;
+    c->level_launches[k] = 0;
+  }
   for (size_t i = 0; i + 1 < c->ev_used; i += 2) {
     float t = 0.f;
     HIPCHK(hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]));
     ms += t;
+    const int k = c->ev_level[i / 2];
+    if (k >= 0) {
+      c->level_ms[k] += t;
+      c->level_launches[k]++;
+    }
   }
   c->stats.kernel_ms = ms;
+  c->stats.timed_spans = (int)(c->ev_used / 2);
   return status[0];
 }
 
@@ -784,7 +825,7 @@ static bool opos_ready(emma_rt_ctx* c) {
 template <typename R>
 static hipError_t launch_fill_cells(emma_rt_ctx* c, int k, double aexp) {
   DevLevel& L = c->lev[k];
-  if (!(c->hydro || c->star) || L.noct == 0) return hipSuccess;
+  if (!(c->hydro || c->star || c->stars) || L.noct == 0) return hipSuccess;
   FillArgs<R> f;
   memset(&f, 0, sizeof(f));
   f.ncell = L.ncell;
@@ -813,10 +854,37 @@ static hipError_t launch_fill_cells(emma_rt_ctx* c, int k, double aexp) {
     f.zs = c->star_z;
     const double tcur = c->tphy;
     const double tcur_in_yrs = tcur * c->p.unit_t / MYR * 1e6;
-    f.star_active = (fmod(tcur_in_yrs, c->star_lifetime) < c->star_lifetime / nsub) &&
-                    (tcur_in_yrs >= c->star_tb0) && (tcur_in_yrs < INFINITY);
+    f.star_g0 = f.star_g1 = 0;
+    if (fmod(tcur_in_yrs, c->star_lifetime) < c->star_lifetime / nsub) {
+      const int ns = c->p.ngrp / c->p.ngrp_time;  // NGRP_SPACE
+      for (int it = 0; it < c->p.ngrp_time; it++)
+        if (tcur_in_yrs >= c->p.time_bound[it] && tcur_in_yrs < c->p.time_bound[it + 1]) {
+          f.star_g0 = it * ns;
+          f.star_g1 = it * ns + ns;
+        }
+    }
     const double srcint = c->star_mass * SOLAR_MASS * c->star_srcint * (tcur > 0 ? 1. : 0) * nsub / 8;
     f.star_src = srcint / pow(f.X0 * c->p.unit_l, 3) * c->p.unit_t / c->p.unit_N * pow(aexp, 2);
+  }
+  if (c->stars) {
+    // stars_sources (src_utils.c:209-335): the uniform factors on the host;
+    // a level without particles gets its sources cleared
+    f.stars = 1;
+    f.stars_decrease = c->stars_decrease;
+    f.nspace = c->p.ngrp / c->p.ngrp_time;
+    f.star_first = L.star_first;
+    f.star_mass = L.star_mass;
+    f.star_age = L.star_age;
+    f.star_rs = L.star_rs;
+    f.star_is = L.star_is;
+    f.stars_unit_d = c->p.unit_d;
+    f.stars_srcint = c->stars_srcint;
+    f.stars_dx3 = pow(pow(0.5, L.level), 3);
+    f.stars_unit_t = c->p.unit_t;
+    f.stars_unit_N = c->p.unit_N;
+    f.stars_pa2 = pow(aexp, 2);
+    f.stars_tphy = c->tphy;
+    f.stars_tlife = c->stars_tlife;
   }
   f.pa3 = pow(aexp, 3);
   f.pa5 = pow(aexp, 5);
@@ -890,17 +958,66 @@ extern "C" int emma_rt_set_oct_pos(emma_rt_ctx* c, int level, const double* xyz)
 }
 
 extern "C" int emma_rt_set_stromgren_source(emma_rt_ctx* c, int enable, double x, double y, double z, double mass,
-                                            double lifetime, double srcint, double time_bound0) {
+                                            double lifetime, double srcint) {
   if (c == nullptr) return EMMA_RT_ERR_ARG;
   if (enable && !(lifetime > 0)) return EMMA_RT_ERR_ARG;
   c->star = enable ? 1 : 0;
+  if (enable) c->stars = 0;  // one source kind per build (WRADTEST or STARS)
   c->star_x = x;
   c->star_y = y;
   c->star_z = z;
   c->star_mass = mass;
   c->star_lifetime = lifetime;
   c->star_srcint = srcint;
-  c->star_tb0 = time_bound0;
+  drop_graphs(c);
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_set_stars(emma_rt_ctx* c, int level, int nstar, const int* cell, const double* mass,
+                                 const double* age, const int* radiative_state, const int* is_star) {
+  DevLevel* L = level_of(c, level);
+  if (L == nullptr) return EMMA_RT_ERR_STATE;
+  if (nstar < 0 || (nstar > 0 && (!cell || !mass || !age || !radiative_state || !is_star))) return EMMA_RT_ERR_ARG;
+  const int ntime = c->p.ngrp_time;
+  std::vector<int> first((size_t)L->ncell + 1, 0);
+  for (int k = 0; k < nstar; k++) {
+    const int j = cell[k];
+    if (j < 0 || j >= L->ncell || (k > 0 && j < cell[k - 1])) return EMMA_RT_ERR_ARG;  // cells ascending
+    if (L->hchild[j] >= 0) return EMMA_RT_ERR_ARG;  // particles sit in leaf cells
+    if (radiative_state[k] < 0 || radiative_state[k] >= ntime) return EMMA_RT_ERR_ARG;
+    first[j + 1]++;
+  }
+  for (int j = 0; j < L->ncell; j++) first[j + 1] += first[j];
+  for (void* p : {(void*)L->star_first, (void*)L->star_mass, (void*)L->star_age, (void*)L->star_rs, (void*)L->star_is})
+    (void)hipFree(p);
+  L->star_first = nullptr;
+  L->star_mass = L->star_age = nullptr;
+  L->star_rs = L->star_is = nullptr;
+  drop_graphs(c);
+  if (nstar == 0) return EMMA_RT_OK;
+  const size_t n = (size_t)nstar;
+  HIPCHK(dalloc(&L->star_first, (size_t)L->ncell + 1));
+  HIPCHK(dalloc(&L->star_mass, n));
+  HIPCHK(dalloc(&L->star_age, n));
+  HIPCHK(dalloc(&L->star_rs, n));
+  HIPCHK(dalloc(&L->star_is, n));
+  HIPCHK(hipMemcpy(L->star_first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(L->star_mass, mass, n * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(L->star_age, age, n * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(L->star_rs, radiative_state, n * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(L->star_is, is_star, n * sizeof(int), hipMemcpyHostToDevice));
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_set_star_sources(emma_rt_ctx* c, int enable, double srcint, double tlife,
+                                        int decrease_after_tlife) {
+  if (c == nullptr) return EMMA_RT_ERR_ARG;
+  if (enable && !(tlife > 0)) return EMMA_RT_ERR_ARG;
+  c->stars = enable ? 1 : 0;
+  if (enable) c->star = 0;  // one source kind per build (WRADTEST or STARS, src_utils.c:458-500)
+  c->stars_srcint = srcint;
+  c->stars_tlife = tlife;
+  c->stars_decrease = decrease_after_tlife ? 1 : 0;
   drop_graphs(c);
   return EMMA_RT_OK;
 }
@@ -1021,22 +1138,27 @@ extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
     for (int i = 0; i < ncoarse && rs == EMMA_RT_OK; i++) rs = recurse(c, 0, aexp);
     c->timing = timing;
     hipError_t ce = hipStreamEndCapture(c->stream, &g);
-    if (rs) return rs;
-    HIPCHK(ce);
+    std::vector<int> endpar;
+    for (auto& L : c->lev) endpar.push_back(L.cur);
+    // nothing ran during the capture: restore the buffer parity the replay
+    // starts from (the launch below advances it), also on the error paths
+    for (size_t k = 0; k < c->lev.size(); k++) c->lev[k].cur = key.parity[k];
+    if (rs || ce != hipSuccess) {
+      if (ce == hipSuccess) (void)hipGraphDestroy(g);
+      if (rs) return rs;
+      HIPCHK(ce);
+    }
     hipGraphExec_t ex;
     hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     HIPCHK(ie);
-    std::vector<int> endpar;
-    for (auto& L : c->lev) endpar.push_back(L.cur);
-    // restore the parity the replay starts from; the launch below advances it
-    for (size_t k = 0; k < c->lev.size(); k++) c->lev[k].cur = key.parity[k];
     it = c->graphs.emplace(key, std::make_pair(ex, endpar)).first;
     c->stats.level_steps = 0;
   }
   if (c->timing == 2) {  // one event pair around the replay
     hipEvent_t e0 = next_event(c), e1 = next_event(c);
     if (!e0 || !e1) return EMMA_RT_ERR_HIP;
+    c->ev_level.push_back(-1);
     HIPCHK(hipEventRecord(e0, c->stream));
     HIPCHK(hipGraphLaunch(it->second.first, c->stream));
     HIPCHK(hipEventRecord(e1, c->stream));
@@ -1053,7 +1175,80 @@ extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
 
 extern "C" int emma_rt_enable_timing(emma_rt_ctx* c, int enable) {
   if (c == nullptr || enable < 0 || enable > 2) return EMMA_RT_ERR_ARG;
-  c->timing = enable ? 1 : 0;
+  c->timing = enable;
+  return EMMA_RT_OK;
+}
+
+// ---------------------------------------------------------------- restart copy
+// The arrays a level step changes, for the current buffer parity: the state
+// (double-buffered), the source and n_H (FillRad writes them), the reflux
+// accumulators, the reionisation times and the hydro fields the injection
+// writes.  (pointer, bytes) pairs; absent arrays are skipped.
+static std::vector<std::pair<void*, size_t>> snap_parts(emma_rt_ctx* c, DevLevel& L) {
+  std::vector<std::pair<void*, size_t>> v;
+  const size_t gn = (size_t)c->p.ngrp * L.ncell * c->rbytes, cn = (size_t)L.ncell * sizeof(double);
+  const int b = L.cur;
+  for (void* p : {L.e[b], L.fx[b], L.fy[b], L.fz[b], L.src}) v.push_back({p, gn});
+  for (void* p : {(void*)L.nhplus[b], (void*)L.eint[b], (void*)L.temp[b], (void*)L.nh}) v.push_back({p, cn});
+  const size_t accn = (size_t)L.nslot * 4 * (c->p.ngrp == 1 ? 1 : (c->p.ngrp == 3 ? 3 : kMaxGrp));
+  if (accn) v.push_back({L.racc, accn * c->rbytes});
+  if (L.t_first) v.push_back({(void*)L.t_first, cn});
+  if (L.t_last) v.push_back({(void*)L.t_last, cn});
+  for (int k = 4; k < 7; k++)
+    if (L.hyd[k]) v.push_back({(void*)L.hyd[k], cn});
+  return v;
+}
+
+extern "C" int emma_rt_save_state(emma_rt_ctx* c) {
+  if (c == nullptr || !c->has_tree) return EMMA_RT_ERR_STATE;
+  for (auto& L : c->lev) {
+    if (L.noct == 0) continue;
+    const auto parts = snap_parts(c, L);
+    size_t need = 0;
+    for (auto& p : parts) need += (p.second + 255) & ~(size_t)255;
+    if (L.snap_bytes < need) {
+      (void)hipFree(L.snap);
+      L.snap = nullptr;
+      L.snap_bytes = 0;
+      HIPCHK(hipMalloc((void**)&L.snap, need));
+      L.snap_bytes = need;
+    }
+    size_t off = 0;
+    for (auto& p : parts) {
+      HIPCHK(hipMemcpyAsync(L.snap + off, p.first, p.second, hipMemcpyDeviceToDevice, c->stream));
+      off += (p.second + 255) & ~(size_t)255;
+    }
+    L.snap_cur = L.cur;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_restore_state(emma_rt_ctx* c) {
+  if (c == nullptr || !c->has_tree) return EMMA_RT_ERR_STATE;
+  for (auto& L : c->lev)
+    if (L.noct > 0 && L.snap_cur < 0) return EMMA_RT_ERR_STATE;
+  for (auto& L : c->lev) {
+    if (L.noct == 0) continue;
+    L.cur = L.snap_cur;
+    const auto parts = snap_parts(c, L);
+    size_t off = 0;
+    for (auto& p : parts) {
+      if (off + p.second > L.snap_bytes) return EMMA_RT_ERR_STATE;  // arrays added since the save
+      HIPCHK(hipMemcpyAsync(p.first, L.snap + off, p.second, hipMemcpyDeviceToDevice, c->stream));
+      off += (p.second + 255) & ~(size_t)255;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_get_level_times(emma_rt_ctx* c, int nlev, double* ms, int* launches) {
+  if (c == nullptr || nlev < 0 || nlev > kMaxLevels) return EMMA_RT_ERR_ARG;
+  for (int k = 0; k < nlev; k++) {
+    if (ms) ms[k] = c->level_ms[k];
+    if (launches) launches[k] = c->level_launches[k];
+  }
   return EMMA_RT_OK;
 }
 
@@ -1125,17 +1320,30 @@ extern "C" int emma_rt_create_pinned_stencil(emma_rt_ctx* c, int stride, int rea
   return EMMA_RT_OK;
 }
 
+extern "C" int emma_rt_alloc_pinned(size_t bytes, void** out) {
+  if (out == nullptr || bytes == 0) return EMMA_RT_ERR_ARG;
+  *out = nullptr;
+  HIPCHK(hipHostMalloc(out, bytes, hipHostMallocDefault));
+  memset(*out, 0, bytes);
+  return EMMA_RT_OK;
+}
+
 extern "C" int emma_rt_destroy_pinned_stencil(void* stencil) {
   if (stencil != nullptr) HIPCHK(hipHostFree(stencil));
   return EMMA_RT_OK;
 }
 
-extern "C" int emma_rt_advance_stencil(emma_rt_ctx* c, void* stencil, int nread, double dxcur, double dtnew,
-                                       double aexp, int chemonly) {
+extern "C" int emma_rt_advance_stencil(emma_rt_ctx* c, void* stencil, int nread, int level, double dxcur,
+                                       double dtnew, double aexp, int chemonly) {
   if (c == nullptr || stencil == nullptr || nread < 0) return EMMA_RT_ERR_ARG;
   if (c->d_stencil == nullptr) return EMMA_RT_ERR_STATE;
   if (nread > c->stride) return EMMA_RT_ERR_ARG;
+  if (c->mode == EMMA_RT_COARSERAD && c->p.lcoarse < 1) return EMMA_RT_ERR_STATE;
   if (nread == 0) return EMMA_RT_OK;
+  // condadvec of advancerad / advanceradGPU (rad_utils.c:1808-1812,
+  // rad_utils_gpu.cu:1058-1062)
+  const int coarserad = c->mode == EMMA_RT_COARSERAD;
+  const int transport = coarserad ? (level == c->p.lcoarse && !chemonly) : 1;
   RgridLayout L;
   rgrid_layout(c->p.ngrp, c->stencil_rbytes, &L);
   int st = begin_call(c);
@@ -1158,11 +1366,13 @@ extern "C" int emma_rt_advance_stencil(emma_rt_ctx* c, void* stencil, int nread,
   }
   const ChemConst K = make_chem(c->p, dtnew, aexp);
   const double cl = cloc_of(c->p, aexp);
-  HIPCHK(launch_stencil(c->d_stencil, L, c->p.ngrp, c->stencil_rbytes, nread, cl, dtnew / dxcur, chemonly, K,
-                        c->d_status, c->d_counters, c->stream));
+  HIPCHK(launch_stencil(c->d_stencil, L, c->p.ngrp, c->stencil_rbytes, nread, cl, dtnew / dxcur, chemonly, coarserad,
+                        transport, K, c->d_status, c->d_counters, c->stream));
+  // without transport the kernel writes no fluxes: rfieldnew alone comes back
+  const size_t nback = transport ? back : rtype;
   for (int ic = 0; ic < 8; ic++) {
     const size_t o = noff + ic * fsize + rfn;
-    HIPCHK(hipMemcpy2DAsync(h + o, rsize, d + o, rsize, back, (size_t)nread, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpy2DAsync(h + o, rsize, d + o, rsize, nback, (size_t)nread, hipMemcpyDeviceToHost, c->stream));
   }
   return end_call(c);
 }

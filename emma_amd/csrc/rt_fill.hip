@@ -47,8 +47,31 @@ __global__ __launch_bounds__(256) void fill_level_kernel(FillArgs<R> f) {
     const double zc = f.opos[3 * oct + 2] + (ic >> 2) * f.dxcur + f.dxcur * 0.5;
     const double rx = xc - f.xs, ry = yc - f.ys, rz = zc - f.zs;
     const double Rd = sqrt(rx * rx + ry * ry + rz * rz);
-    const bool on = (Rd <= f.X0) && f.star_active && (xc > 0.) && (yc > 0.) && (zc > 0.);
-    for (int g = 0; g < f.ng; g++) f.src[g * n + cell] = on ? (R)f.star_src : R(0);
+    // putsource clears every group, stromgren_source sets the frequency
+    // groups of the time group tcur lies in
+    const bool on = (Rd <= f.X0) && (xc > 0.) && (yc > 0.) && (zc > 0.);
+    for (int g = 0; g < f.ng; g++) f.src[g * n + cell] = (on && g >= f.star_g0 && g < f.star_g1) ? (R)f.star_src : R(0);
+  }
+  if (f.stars) {  // putsource -> stars_sources (src_utils.c:209-335, 446-520)
+    double acc[kMaxGrp];
+    for (int g = 0; g < f.ng; g++) acc[g] = 0.;
+    const int k0 = f.star_first ? f.star_first[cell] : 0, k1 = f.star_first ? f.star_first[cell + 1] : 0;
+    for (int k = k0; k < k1; k++) {
+      const int is = f.star_is[k];
+      if (is < 6 || is == 100) continue;
+      const double age = f.stars_tphy - f.star_age[k];
+      const double t = age / f.stars_tlife;
+      const double src =
+          f.star_mass[k] * f.stars_unit_d * f.stars_srcint / f.stars_dx3 * f.stars_unit_t / f.stars_unit_N * f.stars_pa2;
+      double v;
+      if (!f.stars_decrease || age < f.stars_tlife)
+        v = src * (t > 0 ? 1. : 0.);
+      else
+        v = src * (t < 1. ? 1. : pow(t, -4.)) * (t > 0 ? 1. : 0);
+      const int g0 = f.star_rs[k] * f.nspace;
+      for (int g = g0; g < g0 + f.nspace && g < f.ng; g++) acc[g] += v;
+    }
+    for (int g = 0; g < f.ng; g++) f.src[g * n + cell] = (R)acc[g];
   }
   if (f.hydro) f.temp[cell] = e2t(nhplus, eint, nh, f.pa3, f.pa5, f.pstar, f.unit_N);  // src_utils.c:835-838
 }

@@ -147,7 +147,8 @@ hipError_t launch_distrib_e(const TreeView<R>& t, hipStream_t s);
 // current state: with hydro, n_H = d (1-YHE), eint = p/(GAMMA-1), x n_H = dX;
 // split cells src = 0; unsplit cells the Stromgren test source (star) and,
 // with hydro, E2T.  The source term is uniform: the host evaluates its
-// value and the time tests, the kernel only selects the cells.
+// value, the time tests and the active time group, the kernel only selects
+// the cells.
 template <typename R>
 struct FillArgs {
   int ncell, ng, hydro, star;
@@ -162,9 +163,26 @@ struct FillArgs {
   const double* dX;
   const double* opos;  // octs' lower corners, 3 per oct
   double dxcur, X0, xs, ys, zs;
-  int star_active;  // time group and lifetime tests passed (stromgren_source)
-  double star_src;  // source value of every group when active
+  // groups [star_g0, star_g1) of the time group tcur lies in get star_src
+  // when the lifetime test passed (stromgren_source); empty otherwise
+  int star_g0, star_g1;
+  double star_src;
   double pa3, pa5, pstar, unit_N;  // E2T
+  // star particles (stars_sources, src_utils.c:209-335): the particles of
+  // leaf cell c are first[c] .. first[c+1]-1, in the order of the cell's
+  // particle list.  A particle with isStar >= 6 (other than 100, the AGN
+  // case) adds to the NGRP_SPACE groups of its radiative_state the value
+  // ((mass unit_d) srcint / dx^3) unit_t / unit_N a^2 while tphy > age
+  // (t = (tphy - age) / tlife > 0); with decrease (the reference's
+  // DECREASE_EMMISIVITY_AFTER_TLIFE) stars older than tlife keep radiating
+  // at that value times t^-4.  star_first == nullptr: no particles.
+  int stars, stars_decrease, nspace;
+  const int* star_first;
+  const double* star_mass;
+  const double* star_age;
+  const int* star_rs;
+  const int* star_is;
+  double stars_unit_d, stars_srcint, stars_dx3, stars_unit_t, stars_unit_N, stars_pa2, stars_tphy, stars_tlife;
 };
 template <typename R>
 hipError_t launch_fill_level(const FillArgs<R>& f, hipStream_t s);

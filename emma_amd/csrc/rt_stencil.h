@@ -21,9 +21,12 @@ struct RgridLayout {
 
 bool rgrid_layout(int ngrp, int real_bytes, RgridLayout* out);
 
+// coarserad: the COARSERAD build's sweeps and update (split neighbours'
+// fluxes kept, split cells updated); transport: condadvec, the sweeps and the
+// update run at all (rad_utils_gpu.cu:1058-1075)
 hipError_t launch_stencil(void* d_stencil, const RgridLayout& L, int ngrp, int real_bytes, int nread, double c,
-                          double dtsurdx, int chemonly, const ChemConst& K, int* status,
-                          unsigned long long* counters, hipStream_t s);
+                          double dtsurdx, int chemonly, int coarserad, int transport, const ChemConst& K,
+                          int* status, unsigned long long* counters, hipStream_t s);
 
 void stencil_free(void* p);
 

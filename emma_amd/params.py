@@ -59,6 +59,22 @@ ATOMIC_TABLES = {
 }
 
 
+# TIME_BOUND(MYR) lines of the tables with more than one time group (the
+# others have one group from 0 Myr)
+ATOMIC_TIME_BOUNDS = {
+    "Starburst99_3grpT_3grpS": (0.0, 3.6, 36.0),
+}
+
+
+def parse_atomic_time_bounds(text: str) -> tuple:
+    """The TIME_BOUND(MYR) entries of an EMMA atomic data file, in Myr
+    (``parameters.c:455-468`` reads them and closes them with infinity)."""
+    tok = text.split()
+    k = tok.index("NGRP_TIME")
+    ntime = int(tok[k + 1])
+    return tuple(float(t) for t in tok[k + 3: k + 3 + ntime])
+
+
 def parse_atomic(text: str) -> list[tuple[float, float, float, float]]:
     """Parse an EMMA atomic data file the way ``readAtomic`` does
     (``parameters.c:431-500``): a ``NGRP_SPACE n`` header with its bounds,
@@ -105,6 +121,9 @@ class _CParams(ctypes.Structure):
         ("H0", ctypes.c_double),
         ("om", ctypes.c_double),
         ("ov", ctypes.c_double),
+        ("lcoarse", ctypes.c_int),
+        ("ngrp_time", ctypes.c_int),
+        ("time_bound", ctypes.c_double * (MAX_GRP + 1)),
     ]
 
 
@@ -129,22 +148,34 @@ class RunParams:
     H0: float = 0.0
     om: float = 1.0
     ov: float = 0.0
+    lcoarse: int = 0
+    # time groups of the atomic table: ngrp = NGRP_SPACE x ngrp_time, bounds
+    # in years (readAtomic, parameters.c:455-468), the last one infinite
+    time_bound: Sequence[float] = (0.0, math.inf)
 
     @property
     def ngrp(self) -> int:
         return len(self.hnu)
 
+    @property
+    def ngrp_time(self) -> int:
+        return len(self.time_bound) - 1
+
     @classmethod
-    def from_table(cls, rows, clightorg: float, **units) -> "RunParams":
-        """Apply ``readAtomic``'s conversions (``parameters.c:490-497``):
-        hnu eV -> J; cross-sections m^2 -> m^3/s at the reduced light speed."""
+    def from_table(cls, rows, clightorg: float, time_bound_myr=(0.0,), **units) -> "RunParams":
+        """Apply ``readAtomic``'s conversions (``parameters.c:455-497``):
+        hnu eV -> J; cross-sections m^2 -> m^3/s at the reduced light speed;
+        time bounds Myr -> yr, closed by infinity."""
         hnu = [r[0] * ELECTRONVOLT for r in rows]
         ae = [clightorg * (r[1] * LIGHT_SPEED_IN_M_PER_S) for r in rows]
         ai = [clightorg * (r[2] * LIGHT_SPEED_IN_M_PER_S) for r in rows]
         fg = [r[3] for r in rows]
         if len(rows) > MAX_GRP:
             raise ValueError(f"at most {MAX_GRP} photon groups")
-        return cls(clight=clightorg, hnu=hnu, alphae=ae, alphai=ai, factgrp=fg, **units)
+        tb = tuple(t * 1e6 for t in time_bound_myr) + (math.inf,)
+        if len(rows) % (len(tb) - 1):
+            raise ValueError("the group count is not a multiple of the time groups")
+        return cls(clight=clightorg, hnu=hnu, alphae=ae, alphai=ai, factgrp=fg, time_bound=tb, **units)
 
     def to_c(self) -> _CParams:
         c = _CParams()
@@ -158,6 +189,10 @@ class RunParams:
             arr = getattr(c, k)
             for i, v in enumerate(getattr(self, k)):
                 arr[i] = v
+        c.lcoarse = int(self.lcoarse)
+        c.ngrp_time = self.ngrp_time
+        for i, v in enumerate(self.time_bound):
+            c.time_bound[i] = v
         return c
 
     def units7(self) -> np.ndarray:
@@ -204,7 +239,8 @@ def static_units(n_h: float = 1e3, box_kpc: float = 15.0) -> dict:
 
 
 def make_params(table: str = "SB99_equiv_mono_TH", clight: float = 0.1, n_h: float = 1e3, **kw) -> RunParams:
-    return RunParams.from_table(ATOMIC_TABLES[table], clight, **static_units(n_h), **kw)
+    return RunParams.from_table(ATOMIC_TABLES[table], clight, ATOMIC_TIME_BOUNDS.get(table, (0.0,)),
+                                **static_units(n_h), **kw)
 
 
 def e_min_code(params: RunParams, aexp: float = 1.0) -> float:

@@ -60,7 +60,8 @@ EXPORTS = [
     "emma_rt_get_xion_times", "emma_rt_set_tphy", "emma_rt_radsolver", "emma_rt_coarse_step",
     "emma_rt_get_stats", "emma_rt_enable_timing", "emma_rt_set_mode", "emma_rt_fillrad",
     "emma_rt_set_hydro", "emma_rt_get_hydro", "emma_rt_set_coupling", "emma_rt_set_oct_pos",
-    "emma_rt_set_stromgren_source",
+    "emma_rt_set_stromgren_source", "emma_rt_get_level_times", "emma_rt_alloc_pinned",
+    "emma_rt_save_state", "emma_rt_restore_state", "emma_rt_set_stars", "emma_rt_set_star_sources",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -78,6 +79,7 @@ class Stats(ctypes.Structure):
         ("status", ctypes.c_int),
         ("bad_level", ctypes.c_int),
         ("bad_cell", ctypes.c_int),
+        ("timed_spans", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -113,8 +115,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.emma_rt_check_layout.argtypes = [ctypes.c_int, ctypes.c_int, _lp]
     L.emma_rt_create_radstencil.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     L.emma_rt_destroy_radstencil.argtypes = [ctypes.c_void_p]
-    L.emma_rt_advance_stencil.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
-                                          ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    L.emma_rt_advance_stencil.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
     L.emma_rt_create_pinned_stencil.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                                 ctypes.POINTER(ctypes.c_void_p)]
     L.emma_rt_destroy_pinned_stencil.argtypes = [ctypes.c_void_p]
@@ -134,7 +136,13 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.emma_rt_get_hydro.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_dp] * 3
     L.emma_rt_set_coupling.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.emma_rt_set_oct_pos.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
-    L.emma_rt_set_stromgren_source.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_double] * 7
+    L.emma_rt_set_stromgren_source.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_double] * 6
+    L.emma_rt_get_level_times.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _ip]
+    L.emma_rt_set_stars.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _ip, _ip]
+    L.emma_rt_set_star_sources.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int]
+    L.emma_rt_save_state.argtypes = [ctypes.c_void_p]
+    L.emma_rt_restore_state.argtypes = [ctypes.c_void_p]
     _lib = L
     return L
 
@@ -246,6 +254,13 @@ class RadGPU:
         self._check(self.lib.emma_rt_get_level(self.h, level, *[_d(getattr(F, k)) for k in Fields.NAMES]))
         return F
 
+    def save_state(self):
+        """device-side restart copy of every level's state"""
+        self._check(self.lib.emma_rt_save_state(self.h))
+
+    def restore_state(self):
+        self._check(self.lib.emma_rt_restore_state(self.h))
+
     def enable_xion_times(self, on: bool = True):
         self._check(self.lib.emma_rt_enable_xion_times(self.h, int(on)))
 
@@ -293,10 +308,22 @@ class RadGPU:
     def set_stromgren_source(self, star):
         """star: emma_amd.synth.Star, or None to switch the source off"""
         if star is None:
-            self._check(self.lib.emma_rt_set_stromgren_source(self.h, 0, *([0.0] * 7)))
+            self._check(self.lib.emma_rt_set_stromgren_source(self.h, 0, *([0.0] * 6)))
             return
         self._check(self.lib.emma_rt_set_stromgren_source(self.h, 1, star.x, star.y, star.z, star.mass,
-                                                          star.lifetime, star.srcint, star.time_bound0))
+                                                          star.lifetime, star.srcint))
+
+    def set_stars(self, level: int, S):
+        """star particles of a level's leaf cells (emma_amd.synth.StarParticles)"""
+        a = S.arrays()
+        self._check(self.lib.emma_rt_set_stars(self.h, level, S.n, a[0].ctypes.data_as(_ip), _d(a[1]), _d(a[2]),
+                                               a[3].ctypes.data_as(_ip), a[4].ctypes.data_as(_ip)))
+
+    def set_star_sources(self, srcint: float, tlife: float, decrease_after_tlife: bool = False,
+                         enable: bool = True):
+        """FillRad's star-particle sources (stars_sources)"""
+        self._check(self.lib.emma_rt_set_star_sources(self.h, int(enable), float(srcint), float(tlife),
+                                                      int(decrease_after_tlife)))
 
     # ---- the reference's calls
     def fillrad(self, level: int, aexp: float = 1.0):
@@ -321,6 +348,13 @@ class RadGPU:
         self.lib.emma_rt_get_stats(self.h, ctypes.byref(s))
         return s.as_dict()
 
+    def level_times(self) -> dict:
+        """{level: (device ms, launches)} of the last call's event-timed level steps"""
+        nl = self.tree.lmax - self.tree.lmin + 1
+        ms, n = np.zeros(nl), np.zeros(nl, dtype=np.int32)
+        self._check(self.lib.emma_rt_get_level_times(self.h, nl, _d(ms), n.ctypes.data_as(_ip)))
+        return {self.tree.lmin + k: (float(ms[k]), int(n[k])) for k in range(nl)}
+
     # ---- stencil path
     def create_radstencil(self, stride: int, real_bytes: int = 8):
         self._check(self.lib.emma_rt_create_radstencil(self.h, stride, real_bytes))
@@ -332,10 +366,13 @@ class RadGPU:
         return PinnedStencil(self.lib, p, stride, self.params.ngrp, real_bytes)
 
     def advance_stencil(self, stencil: np.ndarray, nread: int, dxcur: float, dtnew: float, aexp: float = 1.0,
-                        chemonly: int = 0):
+                        chemonly: int = 0, level: int = None):
         """Run the sweeps, update and chemistry on a host buffer holding
-        ``nread`` struct RGRID records (modified in place)."""
+        ``nread`` struct RGRID records of ``level`` (modified in place; the
+        level defaults to the one of dxcur = 0.5^level)."""
         assert stencil.flags.c_contiguous
-        st = self.lib.emma_rt_advance_stencil(self.h, stencil.ctypes.data_as(ctypes.c_void_p), nread, dxcur, dtnew,
-                                              aexp, chemonly)
+        if level is None:
+            level = int(round(-np.log2(dxcur)))
+        st = self.lib.emma_rt_advance_stencil(self.h, stencil.ctypes.data_as(ctypes.c_void_p), nread, level, dxcur,
+                                              dtnew, aexp, chemonly)
         self._check(st, self.stats() if st < 0 and st != ERR_HIP else None)

@@ -91,8 +91,8 @@ def hydro_like(F: "Fields", rng: np.random.Generator, vmax: float = 0.1) -> Hydr
 @dataclasses.dataclass
 class Star:
     """The WRADTEST point source (stromgren_source, src_utils.c:339-412):
-    position in box units, mass [solar masses], lifetime [yr], srcint
-    [photons/s/kg] and the lower bound of its time group [yr]."""
+    position in box units, mass [solar masses], lifetime [yr] and srcint
+    [photons/s/kg]; its time groups are the atomic table's (RunParams)."""
 
     x: float = 0.5
     y: float = 0.5
@@ -100,7 +100,45 @@ class Star:
     mass: float = 2e2
     lifetime: float = 1e10
     srcint: float = 5e48 / (2e2 * 1.989e30)
-    time_bound0: float = 0.0
+
+
+@dataclasses.dataclass
+class StarParticles:
+    """Star particles of one level's leaf cells (struct PART fields the
+    radiation sources read, prototypes.h:722-759), sorted by cell and, within
+    a cell, in the order of the cell's particle list."""
+
+    cell: np.ndarray             # int32 level cell index
+    mass: np.ndarray             # code units
+    age: np.ndarray              # formation time, the unit of tphy
+    radiative_state: np.ndarray  # int32 time group
+    is_star: np.ndarray          # int32 isStar (>= 6 radiates, 100 = AGN)
+
+    @property
+    def n(self) -> int:
+        return int(self.cell.shape[0])
+
+    def arrays(self):
+        return (np.ascontiguousarray(self.cell, dtype=np.int32), np.ascontiguousarray(self.mass, dtype=np.float64),
+                np.ascontiguousarray(self.age, dtype=np.float64),
+                np.ascontiguousarray(self.radiative_state, dtype=np.int32),
+                np.ascontiguousarray(self.is_star, dtype=np.int32))
+
+
+def random_stars(L: T.Level, n: int, rng: np.random.Generator, ngrp_time: int = 1, tphy: float = 1.0,
+                 tlife: float = 1.0) -> StarParticles:
+    """n particles in random leaf cells of level L (several per cell happen),
+    sorted by cell with a stable order inside a cell: radiating stars
+    (isStar 6..8), non-radiating particles (isStar 0..5) and AGN (100);
+    ages before and after tphy and beyond tphy - tlife."""
+    leaves = np.nonzero(L.child < 0)[0]
+    cell = np.sort(rng.choice(leaves, size=n, replace=True)).astype(np.int32)
+    kind = rng.random(n)
+    is_star = np.where(kind < 0.7, rng.integers(6, 9, n), np.where(kind < 0.9, rng.integers(0, 6, n), 100))
+    age = tphy - rng.uniform(-0.5, 3.0, n) * tlife
+    return StarParticles(cell=cell, mass=10.0 ** rng.uniform(-9, -6, n), age=age,
+                         radiative_state=rng.integers(0, ngrp_time, n).astype(np.int32),
+                         is_star=is_star.astype(np.int32))
 
 
 def oct_corners(L: T.Level) -> np.ndarray:

@@ -34,6 +34,8 @@
 #ifndef EMMA_RT_H
 #define EMMA_RT_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -65,7 +67,12 @@ extern "C" {
    units (struct UNITS, prototypes.h:171-179), clight/fudgecool/ncvgcool
    (prototypes.h:366-371), the atomic group table as readAtomic leaves it
    (parameters.c:490-497: hnu in J, alphae/alphai in m^3/s at the reduced
-   light speed) and the COSMOPARAM entries chemrad uses (chem_utils.c:738). */
+   light speed), the COSMOPARAM entries chemrad uses (chem_utils.c:738), the
+   coarse level (RUNPARAMS.lcoarse: advancerad's COARSERAD test,
+   rad_utils.c:1808-1812, and the test source's radius, src_utils.c:342), and
+   the atomic table's time groups (readAtomic, parameters.c:455-468):
+   ngrp = NGRP_SPACE x ngrp_time groups, time group i covering
+   [time_bound[i], time_bound[i+1]) years, time_bound[ngrp_time] = INFINITY. */
 typedef struct emma_rt_params {
   int ngrp;
   int ncvgcool;
@@ -74,6 +81,9 @@ typedef struct emma_rt_params {
   double unit_l, unit_v, unit_t, unit_n, unit_mass, unit_d, unit_N;
   double hnu[EMMA_RT_MAX_GRP], alphae[EMMA_RT_MAX_GRP], alphai[EMMA_RT_MAX_GRP], factgrp[EMMA_RT_MAX_GRP];
   double H0, om, ov;
+  int lcoarse;
+  int ngrp_time;
+  double time_bound[EMMA_RT_MAX_GRP + 1];
 } emma_rt_params;
 
 typedef struct emma_rt_ctx emma_rt_ctx;
@@ -88,6 +98,7 @@ typedef struct emma_rt_stats {
   int status;               /* first error raised on the device, EMMA_RT_OK if none */
   int bad_level;            /* level and cell of that error */
   int bad_cell;
+  int timed_spans;          /* event-bracketed spans summed into kernel_ms */
 } emma_rt_stats;
 
 /* ---- device management: interface.cu:61-84 (countdevices, initlocaldevice,
@@ -123,15 +134,25 @@ int emma_rt_destroy_radstencil(emma_rt_ctx *ctx);
    ngrp (zeroed), so the copies of emma_rt_advance_stencil run at DMA speed. */
 int emma_rt_create_pinned_stencil(emma_rt_ctx *ctx, int stride, int real_bytes, void **out);
 int emma_rt_destroy_pinned_stencil(void *stencil);
-/* rad_utils_gpu.cu:1054-1086: copy `nread` RGRIDs host->device, run
-   rad_sweepX/Y/Z + updatefieldrad + chemrad semantics (CPU semantics of
+/* `bytes` of page-locked host memory, no context needed (the allocation of
+   create_pinned_stencil_rad, rad_utils_gpu.cu:39-43, which emma.c:605-606
+   makes before the parameters exist); free with
+   emma_rt_destroy_pinned_stencil */
+int emma_rt_alloc_pinned(size_t bytes, void **out);
+/* rad_utils_gpu.cu:1054-1086: copy `nread` RGRIDs of `level` host->device,
+   run rad_sweepX/Y/Z + updatefieldrad + chemrad semantics (CPU semantics of
    rad_utils.c/chem_utils.c), copy back.  Only the parts the kernel reads (oct
    slots, New.cell[*].rfieldnew) go to the device and only New.cell[*].rfieldnew
    and rflux, the parts that change, come back.  `stencil` is host memory in
    the reference's RGRID layout (page-locked memory from
-   emma_rt_create_pinned_stencil is fastest). */
-int emma_rt_advance_stencil(emma_rt_ctx *ctx, void *stencil, int nread, double dxcur, double dtnew, double aexp,
-                            int chemonly);
+   emma_rt_create_pinned_stencil is fastest).  In EMMA_RT_COARSERAD mode
+   (emma_rt_set_mode) the batch follows the COARSERAD build: the sweeps and
+   the update run only when level == params.lcoarse and !chemonly (condadvec,
+   rad_utils.c:1808-1812), fluxes of split neighbours are kept and split
+   cells are transport-updated (rad_utils.c:518-526, 1548-1552); without the
+   sweeps only rfieldnew comes back. */
+int emma_rt_advance_stencil(emma_rt_ctx *ctx, void *stencil, int nread, int level, double dxcur, double dtnew,
+                            double aexp, int chemonly);
 
 /* ---- resident-level path ---- */
 /* nei_oct entries of faces on a non-periodic domain face, the WRADTEST
@@ -213,9 +234,14 @@ int emma_rt_fillrad(emma_rt_ctx *ctx, int level, double aexp);
    emma_rt_set_stromgren_source enables stromgren_source (src_utils.c:339-412)
    in FillRad: unsplit cells whose centre lies within 2^-lcoarse of (x, y, z)
    and in the positive octant get mass*SOLAR_MASS*srcint/8 photons/s spread over
-   (2^-lcoarse unit_l)^3, in code units, while tphy (years) is in the source's
-   time group [time_bound0, inf); the others get 0.  It needs the octs' lower
-   corners (struct OCT x, y, z; 3 doubles per oct) from emma_rt_set_oct_pos.
+   (2^-lcoarse unit_l)^3, in code units, in the NGRP_SPACE groups of the time
+   group tphy (years) lies in (emma_rt_params.time_bound), while
+   fmod(tphy, lifetime) < lifetime; every other group and cell gets 0.  As
+   under the reference's TESTCOSMO build (FillRad reads cosmo->tphy =
+   a2t(aexp), src_utils.c:706, advanceamr.c:279, 860), the source's time is
+   the context's tphy, one value per call (aexp is one value per call too).
+   It needs the octs' lower corners (struct OCT x, y, z; 3 doubles per oct)
+   from emma_rt_set_oct_pos.
    Calls return EMMA_RT_ERR_STATE when a level lacks what the coupling or the
    source reads. */
 int emma_rt_set_hydro(emma_rt_ctx *ctx, int level, const double *d, const double *u, const double *v,
@@ -224,7 +250,39 @@ int emma_rt_get_hydro(emma_rt_ctx *ctx, int level, double *p, double *dX, double
 int emma_rt_set_coupling(emma_rt_ctx *ctx, int hydro);
 int emma_rt_set_oct_pos(emma_rt_ctx *ctx, int level, const double *xyz);
 int emma_rt_set_stromgren_source(emma_rt_ctx *ctx, int enable, double x, double y, double z, double mass,
-                                 double lifetime, double srcint, double time_bound0);
+                                 double lifetime, double srcint);
+
+/* Star particles as sources: putsource -> stars_sources (src_utils.c:209-335,
+   446-520), the source of the reference's STARS builds, in FillRad on the
+   device.  emma_rt_set_stars hands over the star particles of one level's
+   leaf cells (struct PART, prototypes.h:722-759, linked from CELL.phead):
+   listed cell by cell in ascending cell order and, within a cell, in the
+   order of the cell's particle list (the order the reference adds them in);
+   cell = the level's cell index, mass (code units), age (the unit of
+   tphy), radiative_state (the time group the particle radiates in, <
+   ngrp_time), is_star (isStar).  nstar = 0 removes them.
+   emma_rt_set_star_sources switches FillRad to these sources (and the
+   Stromgren source off): every unsplit cell's src is cleared, then each of
+   its particles with isStar >= 6, other than 100 (the AGN case, not built),
+   adds to the NGRP_SPACE groups of its radiative_state
+   ((mass unit_d) srcint / dx^3) unit_t / unit_N aexp^2 while tphy > age;
+   srcint is RUNPARAMS.srcint [photons/s/kg], tlife STARSPARAM.tlife;
+   decrease_after_tlife is the reference's DECREASE_EMMISIVITY_AFTER_TLIFE
+   flag (stars older than tlife radiate that value times
+   ((tphy - age)/tlife)^-4). */
+int emma_rt_set_stars(emma_rt_ctx *ctx, int level, int nstar, const int *cell, const double *mass, const double *age,
+                      const int *radiative_state, const int *is_star);
+int emma_rt_set_star_sources(emma_rt_ctx *ctx, int enable, double srcint, double tlife, int decrease_after_tlife);
+
+/* A restart point on the device: emma_rt_save_state copies every level's
+   current state (rfield: e, fx, fy, fz, src, nhplus, eint, nh, temp), the
+   reflux accumulators, the reionisation times and the hydro fields the
+   injection writes into a device copy (allocated by the first save),
+   emma_rt_restore_state copies them back.  The reference restarts from its
+   dump of the whole tree (io.c); this keeps a run's state on the GPU, e.g. to
+   repeat a coarse step from the same state. */
+int emma_rt_save_state(emma_rt_ctx *ctx);
+int emma_rt_restore_state(emma_rt_ctx *ctx);
 
 int emma_rt_get_stats(emma_rt_ctx *ctx, emma_rt_stats *out);
 /* device timing into emma_rt_stats.kernel_ms: EMMA_RT_TIMING_LAUNCH puts
@@ -237,6 +295,12 @@ int emma_rt_get_stats(emma_rt_ctx *ctx, emma_rt_stats *out);
 #define EMMA_RT_TIMING_LAUNCH 1
 #define EMMA_RT_TIMING_GRAPH 2
 int emma_rt_enable_timing(emma_rt_ctx *ctx, int enable);
+/* per-level device time of the last call's event-timed level-step launches
+   (EMMA_RT_TIMING_LAUNCH, or single level steps): ms[k] and launches[k] for
+   level lmin + k, k < nlev (either pointer may be NULL).  The reference times
+   the same phases on the host (advancerad's tg/th/tu/ts, rad_utils.c:
+   1796-1845; RadSolver's return value, rad_utils.c:2020-2021). */
+int emma_rt_get_level_times(emma_rt_ctx *ctx, int nlev, double *ms, int *launches);
 
 #ifdef __cplusplus
 }

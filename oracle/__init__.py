@@ -30,11 +30,24 @@ REF_SO = {1: os.path.join(HERE, "_ref", "libemma_ref_g1.so"), 3: os.path.join(HE
           9: os.path.join(HERE, "_ref", "libemma_ref_g9.so")}
 REF_SO_CR = {1: os.path.join(HERE, "_ref", "libemma_ref_g1_cr.so"), 3: os.path.join(HERE, "_ref", "libemma_ref_g3_cr.so")}
 REF_SO_HYD = {1: os.path.join(HERE, "_ref", "libemma_ref_g1_hyd.so"),
-              3: os.path.join(HERE, "_ref", "libemma_ref_g3_hyd.so")}
+              3: os.path.join(HERE, "_ref", "libemma_ref_g3_hyd.so"),
+              9: os.path.join(HERE, "_ref", "libemma_ref_g9_hyd.so")}
 # non-periodic domain faces: TRANS* on all six faces (G=1); plus REF* on the
 # minus faces (G=3)
 REF_SO_TRANS = {1: os.path.join(HERE, "_ref", "libemma_ref_g1_tr.so"),
                 3: os.path.join(HERE, "_ref", "libemma_ref_g3_trref.so")}
+# GPUAXL builds: the reference's RadSolver takes its GPU branch, advanceradGPU
+# and the device set-up coming from emma_amd/dropin/rad_utils_hip.c over the
+# HIP library (RADSTEP, and COARSERAD)
+REF_SO_GPU = {1: os.path.join(HERE, "_ref", "libemma_ref_g1_gpu.so"),
+              3: os.path.join(HERE, "_ref", "libemma_ref_g3_gpu.so")}
+REF_SO_GPU_CR = {1: os.path.join(HERE, "_ref", "libemma_ref_g1_gpucr.so"),
+                 3: os.path.join(HERE, "_ref", "libemma_ref_g3_gpucr.so")}
+# STARS builds: FillRad's sources from star particles (stars_sources); the
+# *dec build adds DECREASE_EMMISIVITY_AFTER_TLIFE
+REF_SO_STARS = {3: os.path.join(HERE, "_ref", "libemma_ref_g3_stars.so"),
+                9: os.path.join(HERE, "_ref", "libemma_ref_g9_stars.so")}
+REF_SO_STARS_DEC = {9: os.path.join(HERE, "_ref", "libemma_ref_g9_starsdec.so")}
 
 RADSTEP = 0
 COARSERAD = 1
@@ -59,9 +72,15 @@ class _OrcHydro(ctypes.Structure):
     _fields_ = [(k, _dp) for k in ("d", "u", "v", "w", "p", "dX", "E")]
 
 
+class _OrcStars(ctypes.Structure):
+    _fields_ = [("on", ctypes.c_int), ("decrease", ctypes.c_int), ("srcint", ctypes.c_double),
+                ("tlife", ctypes.c_double), ("nstar", ctypes.c_int), ("cell", _ip), ("mass", _dp), ("age", _dp),
+                ("radiative_state", _ip), ("is_star", _ip)]
+
+
 class _OrcStar(ctypes.Structure):
     _fields_ = [("on", ctypes.c_int)] + [(k, ctypes.c_double) for k in
-                                        ("x", "y", "z", "mass", "lifetime", "srcint", "time_bound0")]
+                                        ("x", "y", "z", "mass", "lifetime", "srcint")]
 
 
 def _chydro(H) -> _OrcHydro:
@@ -129,7 +148,8 @@ class Oracle:
                                       ctypes.POINTER(_OrcFields)]
         L.orc_distribE.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_fillrad_cells.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_OrcFields), _ip,
-                                        _dp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+                                        _dp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                        ctypes.c_double]
         L.orc_rad_to_hydro.argtypes = [ctypes.POINTER(_OrcFields), ctypes.POINTER(_OrcHydro)]
         self.params = None
         self.tree = None
@@ -138,6 +158,8 @@ class Oracle:
         self.hydro = {}
         self.opos = {}
         self.star = None
+        self.stars = {}
+        self.star_params = None
 
     # ---- scalar / vector helpers
     def eddington(self, fx, fy, fz, e, c, i, j):
@@ -216,6 +238,8 @@ class Oracle:
         self.hydro = {}
         self.opos = {}
         self.star = None
+        self.stars = {}
+        self.star_params = None
 
     def get(self, level, which=0) -> Fields:
         return (self.new if which else self.old)[level].copy()
@@ -293,6 +317,16 @@ class Oracle:
 
     def set_stromgren_source(self, star):
         self.star = star
+        if star is not None:
+            self.star_params = None
+
+    def set_stars(self, level, S):
+        self.stars[level] = S
+
+    def set_star_sources(self, srcint, tlife, decrease_after_tlife=False, enable=True):
+        self.star_params = (float(srcint), float(tlife), bool(decrease_after_tlife)) if enable else None
+        if enable:
+            self.star = None
 
     def fillrad(self, level, aexp=1.0):
         """FillRad: the COARSERAD block (split coarse cells gather their leaves'
@@ -300,7 +334,7 @@ class Oracle:
         if self.mode == COARSERAD and level == self.tree.lmin:
             nc = _cfields(self.new[level])
             self._levels_call(self.lib.orc_putE2coarse, ctypes.byref(nc))
-        if not (self.coupling or self.star is not None):
+        if not (self.coupling or self.star is not None or self.star_params is not None):
             return
         L = self.tree.level(level)
         if L.noct == 0:
@@ -310,13 +344,23 @@ class Oracle:
         st = None
         if self.star is not None:
             st = _OrcStar(1, *[float(getattr(self.star, k)) for k in
-                               ("x", "y", "z", "mass", "lifetime", "srcint", "time_bound0")])
+                               ("x", "y", "z", "mass", "lifetime", "srcint")])
+        sp = None
+        if self.star_params is not None and level in self.stars:
+            a = self.stars[level].arrays()
+            self._star_arrays = a
+            sp = _OrcStars(1, int(self.star_params[2]), self.star_params[0], self.star_params[1], self.stars[level].n,
+                           _i(a[0]), _d(a[1]), _d(a[2]), _i(a[3]), _i(a[4]))
+        elif self.star_params is not None:  # no particles on this level: sources cleared
+            sp = _OrcStars(1, int(self.star_params[2]), self.star_params[0], self.star_params[1], 0,
+                           None, None, None, None, None)
         opos = self.opos.get(level, np.zeros(3 * L.noct))
         for F in (self.old[level], self.new[level]):  # rfield and rfieldnew
             cf = _cfields(F)
             self.lib.orc_fillrad_cells(ctypes.byref(self.cparams), level, self.tree.lmin, ctypes.byref(cf), _i(ch),
                                        _d(opos), ctypes.byref(H) if H is not None else None,
-                                       ctypes.byref(st) if st is not None else None, self.tphy, aexp)
+                                       ctypes.byref(st) if st is not None else None,
+                                       ctypes.byref(sp) if sp is not None else None, self.tphy, aexp)
 
     def radsolver(self, level, dtnew, aexp=1.0, chemonly=0):
         n = self.advancerad(level, dtnew, aexp, chemonly)
@@ -376,12 +420,17 @@ def coarse_step_coarserad(solver, tree, params, aexp=1.0):
     adv(tree.lmin)
 
 
-def _ref_table(coarserad: bool, hydro: bool, boundary: bool) -> dict:
+def _ref_table(coarserad: bool, hydro: bool, boundary: bool, gpu: bool = False, stars: int = 0) -> dict:
+    if stars:
+        return REF_SO_STARS_DEC if stars == 2 else REF_SO_STARS
+    if gpu:
+        return REF_SO_GPU_CR if coarserad else REF_SO_GPU
     return REF_SO_TRANS if boundary else REF_SO_HYD if hydro else REF_SO_CR if coarserad else REF_SO
 
 
-def ref_available(ngrp: int, coarserad: bool = False, hydro: bool = False, boundary: bool = False) -> bool:
-    return os.path.exists(_ref_table(coarserad, hydro, boundary).get(ngrp, ""))
+def ref_available(ngrp: int, coarserad: bool = False, hydro: bool = False, boundary: bool = False,
+                  gpu: bool = False, stars: int = 0) -> bool:
+    return os.path.exists(_ref_table(coarserad, hydro, boundary, gpu, stars).get(ngrp, ""))
 
 
 class Reference:
@@ -389,10 +438,15 @@ class Reference:
     coarserad: the COARSERAD build; hydro: the WRADHYD build (FillRad with the
     hydro copy and the Stromgren source, RadSolver's injection into the hydro
     fields); boundary: the builds with non-periodic domain faces (REF_SO_TRANS),
-    which decide from the octs' positions (set_oct_pos)."""
+    which decide from the octs' positions (set_oct_pos); gpu: the GPUAXL
+    builds, whose RadSolver reaches the GPU through advanceradGPU of the
+    drop-in file (set_params then also runs emma.c's device set-up)."""
 
-    def __init__(self, ngrp: int, coarserad: bool = False, hydro: bool = False, boundary: bool = False):
-        path = _ref_table(coarserad, hydro, boundary)[ngrp]
+    def __init__(self, ngrp: int, coarserad: bool = False, hydro: bool = False, boundary: bool = False,
+                 gpu: bool = False, stars: int = 0):
+        path = _ref_table(coarserad, hydro, boundary, gpu, stars)[ngrp]
+        self.gpu = gpu
+        self.stars_build = stars
         self.mode = COARSERAD if coarserad else RADSTEP
         self.hydro_build = hydro
         if not os.path.exists(path):
@@ -402,7 +456,7 @@ class Reference:
         assert L.refdrv_ngrp() == ngrp
         self.ngrp = ngrp
         L.refdrv_set_params.argtypes = [_dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp,
-                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         L.refdrv_build.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _ip]
         L.refdrv_set_level.argtypes = [ctypes.c_int, ctypes.c_int] + [_dp] * 9
         L.refdrv_get_level.argtypes = [ctypes.c_int, ctypes.c_int] + [_dp] * 9
@@ -427,10 +481,18 @@ class Reference:
         L.refdrv_set_oct_pos.argtypes = [ctypes.c_int, _dp]
         assert L.refdrv_coarserad() == (1 if coarserad else 0)
         assert L.refdrv_wradhyd() == (1 if hydro else 0)
+        assert L.refdrv_gpuaxl() == (1 if gpu else 0)
+        assert L.refdrv_stars() == (1 if stars else 0)
+        if gpu:
+            L.refdrv_gpu_init.argtypes = [ctypes.c_int]
+        if stars:
+            L.refdrv_set_star_params.argtypes = [ctypes.c_double, ctypes.c_double]
+            L.refdrv_set_stars.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _ip, _ip]
+            L.refdrv_fillrad.argtypes = [ctypes.c_int, ctypes.c_double]
         if hydro:
             L.refdrv_set_hydro.argtypes = [ctypes.c_int] + [_dp] * 6
             L.refdrv_get_hydro.argtypes = [ctypes.c_int] + [_dp] * 3
-            L.refdrv_set_stromgren.argtypes = [ctypes.c_double] * 7
+            L.refdrv_set_stromgren.argtypes = [ctypes.c_double] * 6
             L.refdrv_fillrad.argtypes = [ctypes.c_int, ctypes.c_double]
         self.stride = 4096
 
@@ -444,9 +506,12 @@ class Reference:
         assert p.ngrp == G
         u = p.units7()
         arr = [np.ascontiguousarray(np.asarray(getattr(p, k), dtype=np.float64)) for k in ("hnu", "alphae", "alphai", "factgrp")]
-        self._keep = (u, arr)
+        tb = np.ascontiguousarray(np.asarray(p.time_bound, dtype=np.float64))
+        self._keep = (u, arr, tb)
         self.lib.refdrv_set_params(_d(u), p.clight, p.fudgecool, int(p.ncvgcool), *[_d(a) for a in arr], p.H0, p.om,
-                                   p.ov, tphy)
+                                   p.ov, tphy, _d(tb))
+        if self.gpu:  # create_param_GPU snapshots the parameters (rad_utils_gpu.cu:57-106)
+            assert self.lib.refdrv_gpu_init(self.stride) == 0, "no GPU for the GPUAXL build"
 
     def setup(self, params, tree, fields: dict, tphy: float = 0.0):
         self.params = params
@@ -491,8 +556,18 @@ class Reference:
         else:
             coarse_step(self, self.tree, self.params, aexp)
 
+    def set_stars(self, level, S):
+        a = S.arrays()
+        self._stars_keep = a
+        assert self.lib.refdrv_set_stars(level, S.n, _i(a[0]), _d(a[1]), _d(a[2]), _i(a[3]), _i(a[4])) == 0
+
+    def set_star_sources(self, srcint, tlife, decrease_after_tlife=False, enable=True):
+        assert enable and bool(decrease_after_tlife) == (self.stars_build == 2), \
+            "star sources and DECREASE_EMMISIVITY_AFTER_TLIFE are compile-time flags of the reference"
+        self.lib.refdrv_set_star_params(float(srcint), float(tlife))
+
     def fillrad(self, level, aexp=1.0):
-        if self.hydro_build:
+        if self.hydro_build or self.stars_build:
             self.lib.refdrv_fillrad(level, aexp)
         elif self.mode == COARSERAD:
             self.lib.refdrv_fillrad_coarse(level, aexp)
@@ -516,7 +591,7 @@ class Reference:
         self.lib.refdrv_set_oct_pos(level, _d(xyz))
 
     def set_stromgren_source(self, star):
-        self.lib.refdrv_set_stromgren(star.x, star.y, star.z, star.mass, star.lifetime, star.srcint, star.time_bound0)
+        self.lib.refdrv_set_stromgren(star.x, star.y, star.z, star.mass, star.lifetime, star.srcint)
 
     def eddington(self, fx, fy, fz, e, c, i, j):
         fx, fy, fz, e = [np.ascontiguousarray(a, dtype=np.float64) for a in (fx, fy, fz, e)]

@@ -715,11 +715,16 @@ static void orc_stromgren(const orc_params *P, const orc_star *S, int level, int
   const double R = sqrt(pow(xc - S->x, 2) + pow(yc - S->y, 2) + pow(zc - S->z, 2));
   if ((R <= X0) && (fmod(tcur_in_yrs, S->lifetime) < S->lifetime / nsub)) {
     if ((xc > 0.) && (yc > 0.) && (zc > 0.)) {
-      /* one time group: [time_bound0, INFINITY) (parameters.c:459-468) */
-      if ((tcur_in_yrs >= S->time_bound0) && (tcur_in_yrs < INFINITY)) {
-        for (int g = 0; g < P->ngrp; g++) {
-          const double srcint = S->mass * O_SOLAR_MASS * S->srcint * (tcur > 0 ? 1. : 0) * nsub / 8;
-          src[g * ncell + cell] = srcint / pow(X0 * P->unit_l, 3) * P->unit_t / P->unit_N * pow(aexp, 2);
+      /* the NGRP_SPACE groups of the time group tcur lies in (src_utils.c:
+         366-392); the bounds as readAtomic leaves them (parameters.c:455-468) */
+      const int nspace = P->ngrp / P->ngrp_time;
+      for (int it = 0; it < P->ngrp_time; it++) {
+        if ((tcur_in_yrs >= P->time_bound[it]) && (tcur_in_yrs < P->time_bound[it + 1])) {
+          for (int is = 0; is < nspace; is++) {
+            const int g = it * nspace + is;
+            const double srcint = S->mass * O_SOLAR_MASS * S->srcint * (tcur > 0 ? 1. : 0) * nsub / 8;
+            src[g * ncell + cell] = srcint / pow(X0 * P->unit_l, 3) * P->unit_t / P->unit_N * pow(aexp, 2);
+          }
         }
       }
     }
@@ -728,9 +733,40 @@ static void orc_stromgren(const orc_params *P, const orc_star *S, int level, int
   }
 }
 
+/* stars_sources (src_utils.c:209-335) for one unsplit cell: clear every
+   group, then each particle of the cell with isStar >= 6 adds to the
+   NGRP_SPACE groups of its radiative_state (isStar == 100 is the AGN case,
+   empty without AGN); *k walks the cell-sorted particle list */
+static void orc_stars_cell(const orc_params *P, const orc_stars *S, int level, double tcur, double aexp, double *src,
+                           int ncell, int cell, int *k) {
+  const double dxcur = pow(0.5, level);
+  const int nspace = P->ngrp / P->ngrp_time;
+  for (int g = 0; g < P->ngrp; g++) src[g * ncell + cell] = 0.;
+  while (*k < S->nstar && S->cell[*k] < cell) (*k)++;
+  for (; *k < S->nstar && S->cell[*k] == cell; (*k)++) {
+    const int i = *k;
+    if (!(S->is_star[i] >= 6) || S->is_star[i] == 100) continue;
+    if (!S->decrease || (tcur - S->age[i]) < S->tlife) {
+      const double t = (tcur - S->age[i]) / S->tlife;
+      for (int ig = 0; ig < nspace; ig++)
+        src[(S->radiative_state[i] * nspace + ig) * ncell + cell] += (S->mass[i] * P->unit_d) * S->srcint /
+                                                                      pow(dxcur, 3) * P->unit_t / P->unit_N *
+                                                                      pow(aexp, 2) * (t > 0 ? 1. : 0.);
+    } else { /* DECREASE_EMMISIVITY_AFTER_TLIFE, src_utils.c:301-323 */
+      const double slope = -4.;
+      const double s = (S->mass[i] * P->unit_d) * S->srcint / pow(dxcur, 3) * P->unit_t / P->unit_N * pow(aexp, 2);
+      const double t = (tcur - S->age[i]) / S->tlife;
+      for (int ig = 0; ig < nspace; ig++)
+        src[(S->radiative_state[i] * nspace + ig) * ncell + cell] += s * (t < 1. ? 1. : pow(t, slope)) * (t > 0 ? 1 : 0);
+    }
+  }
+}
+
 int orc_fillrad_cells(const orc_params *P, int level, int lcoarse, orc_fields *F, const int *child,
-                      const double *opos, const orc_hydro *H, const orc_star *star, double tcur, double aexp) {
+                      const double *opos, const orc_hydro *H, const orc_star *star, const orc_stars *stars,
+                      double tcur, double aexp) {
   const int n = F->ncell;
+  int kstar = 0;
   for (int cell = 0; cell < n; cell++) {
     if (H != NULL) {
       F->nh[cell] = H->d[cell] * (1. - O_YHE);
@@ -744,6 +780,8 @@ int orc_fillrad_cells(const orc_params *P, int level, int lcoarse, orc_fields *F
     if (star != NULL && star->on) {
       for (int g = 0; g < P->ngrp; g++) F->src[g * n + cell] = 0.;
       orc_stromgren(P, star, level, lcoarse, opos + 3 * (cell / 8), cell % 8, tcur, aexp, F->src, n, cell);
+    } else if (stars != NULL && stars->on) {
+      orc_stars_cell(P, stars, level, tcur, aexp, F->src, n, cell, &kstar);
     }
     if (H != NULL) {
       orc_cell R;

@@ -58,6 +58,12 @@ typedef struct orc_params {
   double hnu[ORC_MAXGRP], alphae[ORC_MAXGRP], alphai[ORC_MAXGRP], factgrp[ORC_MAXGRP];
   /* struct COSMOPARAM (TESTCOSMO) */
   double H0, om, ov;
+  /* RUNPARAMS.lcoarse, and the atomic table's time groups (readAtomic,
+     parameters.c:455-468): ngrp = NGRP_SPACE x ngrp_time, time group i over
+     [time_bound[i], time_bound[i+1]) years, time_bound[ngrp_time] = inf */
+  int lcoarse;
+  int ngrp_time;
+  double time_bound[ORC_MAXGRP + 1];
 } orc_params;
 
 /* one level's radiation state (struct Rtype, SoA) */
@@ -132,17 +138,32 @@ typedef struct orc_star {
   double mass;         /* unitary_stars_test->mass [solar masses] */
   double lifetime;     /* unitary_stars_test->lifetime [yr] */
   double srcint;       /* RUNPARAMS.srcint [photons/s/kg] */
-  double time_bound0;  /* atomic.time_bound[0] [yr] (one time group) */
 } orc_star;
+
+/* the star particles of one level's leaf cells and the stars_sources
+   parameters (src_utils.c:209-335); on = 0 leaves src alone.  Particles are
+   listed by ascending cell, within a cell in particle-list order. */
+typedef struct orc_stars {
+  int on;
+  int decrease;    /* DECREASE_EMMISIVITY_AFTER_TLIFE */
+  double srcint;   /* RUNPARAMS.srcint [photons/s/kg] */
+  double tlife;    /* STARSPARAM.tlife (unit of tcur) */
+  int nstar;
+  const int *cell; /* level cell index */
+  const double *mass, *age;
+  const int *radiative_state, *is_star;
+} orc_stars;
 
 /* FillRad (src_utils.c:692-849) per cell of one level, init = 0, without the
    COARSERAD block (orc_putE2coarse): with hydro, n_H, eint and x n_H from the
    hydro state (738-762); split cells get src = 0 (765-772); unsplit cells the
    test source (putsource -> stromgren_source, 446-466) when star->on, then
    E2T with hydro (835-838).  child[cell] >= 0 marks split cells; opos holds
-   the octs' lower corners (3 per oct).  tcur = cosmo->tphy. */
+   the octs' lower corners (3 per oct).  tcur = cosmo->tphy.  With stars->on
+   the unsplit cells' source is stars_sources' instead (the STARS build). */
 int orc_fillrad_cells(const orc_params *P, int level, int lcoarse, orc_fields *F, const int *child,
-                      const double *opos, const orc_hydro *H, const orc_star *star, double tcur, double aexp);
+                      const double *opos, const orc_hydro *H, const orc_star *star, const orc_stars *stars,
+                      double tcur, double aexp);
 
 /* RadSolver's injection into the hydro state (rad_utils.c:1941-1951,
    1993-2004), every cell of the level after the final update:

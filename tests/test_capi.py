@@ -10,7 +10,7 @@ import pytest
 
 import golden_io as gio
 from emma_amd import rgrid, rt
-from emma_amd.params import ATOMIC_TABLES, parse_atomic, make_params
+from emma_amd.params import ATOMIC_TABLES, ATOMIC_TIME_BOUNDS, parse_atomic, parse_atomic_time_bounds, make_params
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -73,7 +73,7 @@ def test_null_arguments_rejected():
     assert lib.emma_rt_create(None, None) == rt.ERR_ARG
     assert lib.emma_rt_radsolver(None, 5, 1.0, 1.0, 0) == rt.ERR_STATE
     assert lib.emma_rt_coarse_step(None, 1.0, 1) == rt.ERR_STATE
-    assert lib.emma_rt_advance_stencil(None, None, 1, 1.0, 1.0, 1.0, 0) == rt.ERR_ARG
+    assert lib.emma_rt_advance_stencil(None, None, 1, 7, 1.0, 1.0, 1.0, 0) == rt.ERR_ARG
 
 
 @pytest.mark.parametrize("name", sorted(ATOMIC_TABLES))
@@ -81,6 +81,19 @@ def test_parse_atomic_files(name):
     """readAtomic's parse (parameters.c:431-500) of the reference's data files"""
     txt = open(os.path.join(gio.GOLDEN, "atomic", name + ".dat")).read()
     assert parse_atomic(txt) == ATOMIC_TABLES[name]
+    assert parse_atomic_time_bounds(txt) == ATOMIC_TIME_BOUNDS.get(name, (0.0,))
+
+
+def test_time_groups_of_the_nine_group_table():
+    """Starburst99_3grpT_3grpS: three time groups of three frequency groups,
+    bounds 0 / 3.6 / 36 Myr converted to years and closed by infinity
+    (parameters.c:455-468), handed to the library with the parameters"""
+    p = make_params("Starburst99_3grpT_3grpS")
+    assert p.ngrp == 9 and p.ngrp_time == 3
+    assert tuple(p.time_bound) == (0.0, 3.6e6, 36e6, float("inf"))
+    c = p.to_c()
+    assert c.ngrp_time == 3 and c.time_bound[1] == 3.6e6 and c.time_bound[3] == float("inf")
+    assert make_params("Starburst99_1grpT_3grpS").to_c().ngrp_time == 1
 
 
 def test_readatomic_conversions():

@@ -122,3 +122,34 @@ def test_injection_closed_forms_and_round_trip():
     G = o.get(3)
     np.testing.assert_allclose(G.eint, F.eint, rtol=4e-16)
     np.testing.assert_allclose(G.nhplus, F.nhplus, rtol=4e-16)
+
+
+def test_stromgren_source_time_groups_vs_reference():
+    """Nine groups = three time groups (0 / 3.6 / 36 Myr) of three frequency
+    groups (Starburst99_3grpT_3grpS): stromgren_source fills only the three
+    groups of the time group the run's time lies in (src_utils.c:366-392), the
+    other six stay 0 as putsource left them (src_utils.c:450-455).  The oracle
+    against the reference's WRADHYD build with NGRP_TIME = 3, bit for bit, at
+    a time in each of the three groups."""
+    if not ref_available(9, hydro=True):
+        pytest.skip("reference WRADHYD build with nine groups (oracle/_ref) not present")
+    W = synth.amr_workload(3, [40, 80, 50], "Starburst99_3grpT_3grpS", seed=4, sigma=1.5, grid=16)
+    yr_per_code = W.params.unit_t / 3.1536e13 * 1e6
+    for it, years in enumerate((1e6, 1e7, 1e8)):
+        tphy = years / yr_per_code
+        o, r = Oracle(), Reference(9, hydro=True)
+        for s in (o, r):
+            coupled(s, W, synth.Star(), tphy)
+        lit = 0
+        for l in range(W.tree.lmin, W.tree.lmax + 1):
+            o.fillrad(l)
+            r.fillrad(l)
+            for which in (0, 1):
+                A, B = o.get(l, which), r.get(l, which)
+                for k in Fields.NAMES:
+                    assert np.array_equal(getattr(A, k), getattr(B, k)), (l, which, k, years)
+            src = o.get(l).src
+            on = np.nonzero(np.any(src > 0, axis=1))[0]
+            assert set(on.tolist()) <= {3 * it, 3 * it + 1, 3 * it + 2}, (years, on)
+            lit += int(np.count_nonzero(src[3 * it] > 0))
+        assert lit > 0, years

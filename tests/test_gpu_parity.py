@@ -244,6 +244,83 @@ def test_hydro_coupling_and_source(gpu_lib, oracle, table, G, mode, precision):
 
 
 
+@pytest.mark.parametrize("years", [1e6, 1e7, 1e8])
+def test_source_time_groups(gpu_lib, oracle, years):
+    """Nine groups = three time groups (0 / 3.6 / 36 Myr) of three frequency
+    groups: the device source fills only the three groups of the time group
+    the run's time lies in (src_utils.c:366-392), as the oracle does (pinned
+    to the reference's NGRP_TIME = 3 WRADHYD build, test_coupling.py)."""
+    W = synth.amr_workload(3, [60, 100, 80], "Starburst99_3grpT_3grpS", seed=13, sigma=1.5, grid=16)
+    tphy = years / (W.params.unit_t / 3.1536e13 * 1e6)
+    g = coupled_pair(oracle, W, rt.FP64, synth.Star(x=0.5, y=0.5, z=0.5), tphy, "graph")
+    it = [1e6, 1e7, 1e8].index(years)
+    lit = 0
+    for l in range(W.tree.lmin, W.tree.lmax + 1):
+        A = g.get(l)
+        err = field_errors(A, oracle.get(l), W.params.cloc())
+        assert max(err.values()) < FP64_TOL, (l, err)
+        on = set(np.nonzero(np.any(A.src > 0, axis=1))[0].tolist())
+        assert on <= {3 * it, 3 * it + 1, 3 * it + 2}, (years, on)
+        lit += int(np.count_nonzero(A.src[3 * it] > 0))
+    assert lit > 0
+
+
+@pytest.mark.parametrize("precision", [rt.FP64, rt.MIXED])
+def test_save_restore_state(gpu_lib, precision):
+    """emma_rt_save_state / emma_rt_restore_state: a coarse step repeated from
+    the restored device copy gives the same state bit for bit, whatever ran
+    in between (including a different buffer parity)."""
+    W = synth.amr_workload(3, [60, 100, 80], "Starburst99_1grpT_3grpS", seed=13, sigma=1.5, nsrc=3, grid=16)
+    g = rt.RadGPU(W.params, precision)
+    g.setup(W.params, W.tree, W.fields)
+    with pytest.raises(rt.EmmaRTError):
+        g.restore_state()  # nothing saved yet
+    g.coarse_step(1.0, 1)
+    g.save_state()
+    g.coarse_step(1.0, 1)
+    A = {l: g.get(l) for l in W.fields}
+    g.coarse_step(1.0, 3)
+    g.restore_state()
+    g.coarse_step(1.0, 1)
+    for l in W.fields:
+        B = g.get(l)
+        for k in Fields.NAMES:
+            assert np.array_equal(getattr(A[l], k), getattr(B, k)), (l, k)
+
+
+@pytest.mark.parametrize("G", [3, 9])
+@pytest.mark.parametrize("precision", [rt.FP64, rt.MIXED])
+@pytest.mark.parametrize("decrease", [False, True], ids=["steady", "decrease"])
+def test_star_sources(gpu_lib, oracle, G, precision, decrease):
+    """Star particles as sources (stars_sources) on the device: FillRad's
+    sources against the oracle (pinned to the reference's STARS builds,
+    test_stars.py), then two coarse steps with FillRad before every level
+    step (captured graph) against the oracle's."""
+    from test_stars import star_setup, workload
+
+    W, stars = workload(G, 33)
+    g = rt.RadGPU(W.params, precision)
+    for s in (g, oracle):
+        star_setup(s, W, stars, decrease)
+    srctol = 1e-12 if precision == rt.FP64 else 1e-7
+    for l in range(W.tree.lmin, W.tree.lmax + 1):
+        g.fillrad(l)
+        oracle.fillrad(l)
+        a, b = g.get(l).src, oracle.get(l).src
+        assert np.count_nonzero(b) > 0 or W.tree.leaf_cells(l) == 0
+        assert np.array_equal(a == 0, b == 0), l
+        assert max_rel(a, b) < srctol, (l, max_rel(a, b))
+    for _ in range(2):
+        g.coarse_step(1.0, 1)
+        oracle.coarse_step()
+    tol = FP64_TOL if precision == rt.FP64 else MIXED_TOL
+    for l in range(W.tree.lmin, W.tree.lmax + 1):
+        err = field_errors(g.get(l), oracle.get(l), W.params.cloc())
+        print("stars G=%d prec=%d level %d" % (G, precision, l), err)
+        assert err["x"] < tol and err["temp"] < tol and err["e"] < tol and err["eint"] < tol, (l, err)
+        assert max(err["fx"], err["fy"], err["fz"]) < tol, (l, err)
+
+
 def outlier_errors(A: Fields, B: Fields, c: float):
     """per field: (max relative error, number of cells beyond MIXED_TOL)"""
     out = {}
@@ -294,7 +371,8 @@ def test_full_size_mixed_vs_fp64(gpu_lib, config):
 def test_timing_modes(gpu_lib):
     """TIMING_GRAPH brackets a coarse step's graph replay with one event pair,
     TIMING_LAUNCH every level-step kernel of an eager coarse step; both give
-    the same state, a positive device time and the same step counts."""
+    the same state, a positive device time and the same step counts, and the
+    number of event-bracketed spans tells which path ran."""
     W = synth.uniform_workload(4, "SB99_equiv_mono_TH", seed=2)
     out = {}
     for mode in (rt.TIMING_LAUNCH, rt.TIMING_GRAPH):
@@ -306,6 +384,7 @@ def test_timing_modes(gpu_lib):
         g.coarse_step(1.0, 4)
         st = g.stats()
         assert st["kernel_ms"] > 0 and st["level_steps"] == 4, (mode, st)
+        assert st["timed_spans"] == (4 if mode == rt.TIMING_LAUNCH else 1), (mode, st)
         out[mode] = g.get(4)
         with pytest.raises(rt.EmmaRTError):
             g.enable_timing(3)
