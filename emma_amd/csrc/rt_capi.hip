@@ -41,6 +41,7 @@ struct DevLevel {
   int npair = 0;  // reflux pairs: (leaf, face) with a finer oct behind the face
   int nslot = 0;  // accumulators per flux component and group: npair x sub-slots
   int halo_per_dir = 0;  // some wave has more than 64 halo faces (tiled kernel variant)
+  int chem_budget = 0;   // chemistry tries before a leaf goes to the straggler pass (0: none)
   void* e[2] = {nullptr, nullptr};
   void* fx[2] = {nullptr, nullptr};
   void* fy[2] = {nullptr, nullptr};
@@ -106,6 +107,14 @@ struct emma_rt_ctx {
   int* d_status = nullptr;
   unsigned long long* d_counters = nullptr;
   double* d_photons = nullptr;
+  // straggler list of the tiled mixed kernel (capacity: the largest level's
+  // cells) and one list-length word per level-step launch of a call (slot
+  // defer_seq, counted from 0 in every call; all cleared before the call)
+  int* d_defer = nullptr;
+  int* d_defer_ctl = nullptr;
+  int defer_cap = 0;
+  int defer_slots = 0;
+  int defer_seq = 0;
   emma_rt_stats stats;
   int timing = 0;
   int xion = 0;
@@ -156,6 +165,12 @@ static void free_tree(emma_rt_ctx* c) {
       (void)hipFree(p);
   }
   c->lev.clear();
+  (void)hipFree(c->d_defer);
+  (void)hipFree(c->d_defer_ctl);
+  c->d_defer = nullptr;
+  c->d_defer_ctl = nullptr;
+  c->defer_cap = 0;
+  c->defer_slots = 0;
   for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.first);
   c->graphs.clear();
   c->has_tree = false;
@@ -468,6 +483,12 @@ extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int
       HIPCHK(hipMemcpy(L.rpair, L.hrpair.data(), (size_t)L.noct * 6 * sizeof(int), hipMemcpyHostToDevice));
     }
   }
+  if (c->rbytes == 4 && (G == 1 || G == 3)) {  // the tiled kernel's straggler list
+    int cap = 0;
+    for (auto& L : c->lev) cap = std::max(cap, L.ncell);
+    HIPCHK(dalloc(&c->d_defer, (size_t)cap));
+    c->defer_cap = cap;
+  }
   c->has_tree = true;
   if (c->xion) {
     c->xion = 0;
@@ -633,6 +654,12 @@ static LevelArgs<R> level_args(emma_rt_ctx* c, int k, double dtnew, double aexp,
     A.update_split = 0;
   }
   A.halo_per_dir = L.halo_per_dir;
+  if (c->d_defer != nullptr && L.chem_budget > 0 && c->defer_seq < c->defer_slots) {
+    A.chem_budget = L.chem_budget;
+    A.defer_cap = c->defer_cap;
+    A.defer_list = c->d_defer;
+    A.defer_ctl = c->d_defer_ctl + c->defer_seq++;
+  }
   A.e = (const R*)L.e[a];
   A.fx = (const R*)L.fx[a];
   A.fy = (const R*)L.fy[a];
@@ -730,8 +757,33 @@ static int timed_launch(emma_rt_ctx* c, int k, double dtnew, double aexp, int ch
   return EMMA_RT_OK;
 }
 
+// list-length words for `n` level-step launches in one call (allocated
+// outside any stream capture: the calls below run before BeginCapture)
+static int ensure_defer_slots(emma_rt_ctx* c, long long n) {
+  if (c->d_defer == nullptr || n <= c->defer_slots) return EMMA_RT_OK;
+  if (n > (1LL << 24)) return EMMA_RT_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  (void)hipFree(c->d_defer_ctl);
+  c->d_defer_ctl = nullptr;
+  c->defer_slots = 0;
+  const int m = (int)((n + 255) & ~255LL);
+  HIPCHK(dalloc(&c->d_defer_ctl, (size_t)m));
+  c->defer_slots = m;
+  return EMMA_RT_OK;
+}
+
+// level-step launches of `ncoarse` coarse steps
+static long long steps_per_call(emma_rt_ctx* c, int ncoarse) {
+  long long steps = 0;
+  for (int k = 0; k < (int)c->lev.size(); k++)
+    if (c->lev[k].noct > 0) steps += (c->mode == EMMA_RT_COARSERAD) ? (long long)ncoarse : ((long long)ncoarse << k);
+  return steps;
+}
+
 static int begin_call(emma_rt_ctx* c) {
   HIPCHK(launch_fill((unsigned int*)c->d_status, 4, 0u, c->stream));
+  c->defer_seq = 0;
+  if (c->defer_slots > 0) HIPCHK(launch_fill((unsigned int*)c->d_defer_ctl, c->defer_slots, 0u, c->stream));
   HIPCHK(launch_fill((unsigned int*)c->d_counters, 2 * kCounterSlots * kCounterStride, 0u, c->stream));
   memset(&c->stats, 0, sizeof(c->stats));
   c->ev_used = 0;
@@ -745,16 +797,18 @@ static int end_call(emma_rt_ctx* c) {
   std::vector<unsigned long long> slots(kCounterSlots * kCounterStride);
   HIPCHK(hipMemcpy(status, c->d_status, sizeof(status), hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(slots.data(), c->d_counters, slots.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-  unsigned long long cnt[2] = {0ull, 0ull};
+  unsigned long long cnt[3] = {0ull, 0ull, 0ull};
   for (int i = 0; i < kCounterSlots; i++) {
     cnt[0] += slots[(size_t)i * kCounterStride + 0];
     cnt[1] += slots[(size_t)i * kCounterStride + 1];
+    cnt[2] += slots[(size_t)i * kCounterStride + 2];
   }
   c->stats.status = status[0];
   c->stats.bad_level = status[1];
   c->stats.bad_cell = status[2];
   c->stats.chem_substeps = (long long)cnt[0];
   c->stats.cell_updates = (long long)cnt[1];
+  c->stats.deferred_cells = (long long)cnt[2];
   double ms = 0.;
   for (int k = 0; k < kMaxLevels; k++) {
     c->level_ms[k] = 0.;
@@ -1060,6 +1114,8 @@ extern "C" int emma_rt_radsolver(emma_rt_ctx* c, int level, double dtnew, double
   if (L == nullptr) return EMMA_RT_ERR_STATE;
   int st = coupling_ready(c);
   if (st) return st;
+  st = ensure_defer_slots(c, 1);
+  if (st) return st;
   st = begin_call(c);
   if (st) return st;
   st = timed_launch(c, level - c->lmin, dtnew, aexp, chemonly);
@@ -1112,6 +1168,8 @@ extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
   if (c == nullptr || !c->has_tree) return EMMA_RT_ERR_STATE;
   if (ncoarse < 1) return EMMA_RT_ERR_ARG;
   int st = coupling_ready(c);
+  if (st) return st;
+  st = ensure_defer_slots(c, steps_per_call(c, ncoarse));
   if (st) return st;
   st = begin_call(c);
   if (st) return st;
@@ -1166,11 +1224,25 @@ extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
     HIPCHK(hipGraphLaunch(it->second.first, c->stream));
   }
   for (size_t k = 0; k < c->lev.size(); k++) c->lev[k].cur = it->second.second[k];
-  long long steps = 0;
-  for (int k = 0; k < (int)c->lev.size(); k++)
-    if (c->lev[k].noct > 0) steps += (c->mode == EMMA_RT_COARSERAD) ? (long long)ncoarse : ((long long)ncoarse << k);
-  c->stats.level_steps = steps;
+  c->stats.level_steps = steps_per_call(c, ncoarse);
   return end_call(c);
+}
+
+extern "C" int emma_rt_set_chem_budget(emma_rt_ctx* c, int level, int budget) {
+  if (c == nullptr || !c->has_tree || budget < 0) return EMMA_RT_ERR_ARG;
+  if (level != -1 && level_of(c, level) == nullptr) return EMMA_RT_ERR_ARG;
+  if (c->d_defer == nullptr && budget > 0) return EMMA_RT_ERR_ARG;  // FP64 or generic-group path: no tiled kernel
+  for (auto& L : c->lev)
+    if (level == -1 || L.level == level) L.chem_budget = budget;
+  drop_graphs(c);
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_get_chem_budget(emma_rt_ctx* c, int level, int* budget) {
+  DevLevel* L = level_of(c, level);
+  if (L == nullptr || budget == nullptr) return EMMA_RT_ERR_ARG;
+  *budget = L->chem_budget;
+  return EMMA_RT_OK;
 }
 
 extern "C" int emma_rt_enable_timing(emma_rt_ctx* c, int enable) {

@@ -23,6 +23,14 @@ struct LevelArgs {
   // (sparse levels: the per-direction halo variant of level_step_tiled)
   int halo_per_dir;
   int cnslot;  // accumulators in cracc
+  // chemistry stragglers (tiled mixed kernel): a leaf whose sub-step loop is
+  // still running after chem_budget tries (> 0; <= 0: no limit) stores its
+  // transported N, F, appends its cell to defer_list (capacity defer_cap)
+  // and leaves its chemistry to chem_straggler_kernel, which restarts it from
+  // the same inputs.  defer_ctl: this launch's list length (zero before it).
+  int chem_budget, defer_cap;
+  int* defer_list;
+  int* defer_ctl;
   // current state (read)
   const R* e;
   const R* fx;
@@ -79,7 +87,7 @@ struct LevelArgs {
   // numerics
   R c, hc, dtsurdx;
   int* status;                   // [code, level, cell]
-  unsigned long long* counters;  // kCounterSlots x [accepted sub-steps, leaf cell-updates] or null
+  unsigned long long* counters;  // kCounterSlots x [accepted sub-steps, leaf cell-updates, deferred leaves] or null
 };
 
 // Counters are spread over kCounterSlots cache lines (block b adds into slot
@@ -109,6 +117,9 @@ constexpr int kRefluxTiled = 1;
 constexpr int kTileCells = 256;
 constexpr int kTileOcts = kTileCells / 8;
 constexpr int kTileWaves = kTileCells / 64;
+// grid of the chemistry straggler pass (it loops over the list)
+constexpr int kStragglerThreads = 256;
+constexpr int kStragglerBlocks = 4096;
 
 // Most cells in one level with ng groups: cell and group*cell indices are
 // 32-bit ints and every byte offset into a per-group array (8-byte elements at

@@ -11,6 +11,8 @@
 // accumulators that the coarser level folds in on its next step.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "rt_kernels.h"
 #include "rt_physics_fast.h"
 
@@ -30,6 +32,11 @@ __device__ __forceinline__ void raise_status(int* status, int code, int level, i
     status[1] = level;
     status[2] = cell;
   }
+}
+
+// lanes of mask m below this lane
+__device__ __forceinline__ int lanes_below_mask(unsigned long long m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
 template <typename R>
@@ -251,9 +258,32 @@ __device__ __forceinline__ int finish_leaf(const LevelArgs<R>& a, const ChemCons
   double temp = 0.;
   int nit = 0;
   int st;
+  bool deferred = false;  // chemistry left to the straggler pass
   if constexpr (Fast) {
     double fscale[GA];
-    st = chem_cell_fast<G>(K, ng, e, fscale, src, nhplus, eint, nh, temp, a.chemonly, nit);
+    st = chem_cell_fast<G>(K, ng, e, fscale, src, nhplus, eint, nh, temp, a.chemonly, nit,
+                           Sub == kRefluxTiled && a.chem_budget > 0 ? a.chem_budget : -1);
+    if constexpr (Sub == kRefluxTiled) {
+      // budget spent: the transported N, F are stored unscaled (fscale = 1)
+      // and the cell goes on the straggler list, one atomic per wave
+      deferred = st == 1;
+      const unsigned long long dm = __ballot(deferred);
+      if (dm != 0ull) {
+        const int leader = __ffsll((long long)dm) - 1;
+        int base = 0;
+        if ((int)__lane_id() == leader) base = atomicAdd(a.defer_ctl, __popcll(dm));
+        base = __shfl(base, leader);
+        if (deferred) {
+          const int slot = base + lanes_below_mask(dm);
+          if (slot < a.defer_cap)
+            a.defer_list[slot] = cell;
+          else
+            raise_status(a.status, -1, a.level, cell);  // list smaller than the level (host sizes it)
+          st = 0;
+          nit = 0;
+        }
+      }
+    }
 #pragma unroll
     for (int g = 0; g < GA; g++) {
       if (g < ng) {
@@ -292,7 +322,7 @@ __device__ __forceinline__ int finish_leaf(const LevelArgs<R>& a, const ChemCons
       // floor must not round below it, or the next step would stop at the
       // floor test (chem_utils.c:382); only the rounding is corrected
       if constexpr (sizeof(R) == 4)
-        if (e[g] >= K.emin_grp[g] && ev < K.emin_grp_up[g]) ev = K.emin_grp_up[g];
+        if (!deferred && e[g] >= K.emin_grp[g] && ev < K.emin_grp_up[g]) ev = K.emin_grp_up[g];
       gst(a.eo + g * n, cell, ev);
       gst(a.fxo + g * n, cell, fx[g]);
       gst(a.fyo + g * n, cell, fy[g]);
@@ -519,10 +549,6 @@ __device__ __forceinline__ float lane_xor(float x, int m) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(t, 0x1B, 0xF, 0xF, true));              // quad_perm 3,2,1,0
 }
 
-__device__ __forceinline__ int lanes_below(unsigned long long m) {
-  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-}
-
 template <int G, bool PerDir>
 __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4))) void level_step_tiled(
     LevelArgs<float> a_by_value, ChemConst K_by_value) {
@@ -595,7 +621,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
       hbase[d] = nhalo;
       nhalo += __popcll(hball[d]);
       if (out) {
-        const int slot = hbase[d] + lanes_below(hball[d]);  // < kHaloCap (variant chosen from the topology)
+        const int slot = hbase[d] + lanes_below_mask(hball[d]);  // < kHaloCap (variant chosen from the topology)
         sItem[wv][slot] = (unsigned char)(lane * 4 + d);
         sItemNei[wv][slot] = noD[d];
       }
@@ -738,7 +764,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
         const bool inT = in_tile(noD[d]);
         // (unsigned: a domain-face entry is near INT_MIN; jl and j are then unused)
         const int jl = (int)(((unsigned)noD[d] - (unsigned)oct0) * 8u + (unsigned)(ic ^ (1 << d)));
-        const int slot = PerDir ? lane : hbase[d] + lanes_below(hball[d]);
+        const int slot = PerDir ? lane : hbase[d] + lanes_below_mask(hball[d]);
         const bool coarse_face = !inT && noD[d] < 0 && noD[d] > kNeiReflective;
         int iout, sout, cout;  // neighbour share, stride over groups, cancel flag: leaving face
         if (inT) {
@@ -872,6 +898,68 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   if (a.counters != nullptr) block_counters<kTileWaves>(a.counters, nit_acc, leaf);
 }
 
+// ------------------------------------------------------------------ chemistry stragglers
+// The leaves the tiled kernel put on the straggler list (sub-step loop still
+// running after chem_budget tries) take their whole chemistry here, one
+// thread each, from the same inputs: the transported N, F the tiled kernel
+// stored into the next state, the source and x n_H, eint, n_H of the current
+// state.  A wave then holds 64 long loops instead of one long loop among 63
+// finished lanes.  Each level-step launch has its own list-length word
+// (cleared by the host before the call), so nothing is reset here.
+template <int G>
+__global__ __launch_bounds__(kStragglerThreads) void chem_straggler_kernel(LevelArgs<float> a_by_value,
+                                                                            ChemConst K_by_value) {
+  const KernArgs<float> args = kernargs<float>();
+  const LevelArgs<float>& a = args.a;
+  const ChemConst& K = args.K;
+  const int count = min(*(volatile const int*)a.defer_ctl, a.defer_cap);
+  const int n = a.ncell;
+  unsigned int nit_acc = 0, ncell_acc = 0;
+  for (int i = blockIdx.x * kStragglerThreads + (int)threadIdx.x; i < count; i += gridDim.x * kStragglerThreads) {
+    ncell_acc++;
+    const int cell = gld(a.defer_list, i);
+    double e[G], src[G], fscale[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      e[g] = (double)gld(a.eo + g * n, cell);
+      src[g] = (double)gld(a.src + g * n, cell);
+    }
+    double nhplus = gld(a.nhplus, cell), eint = gld(a.eint, cell), temp = 0.;
+    const double nh = gld(a.nh, cell);
+    int nit = 0;
+    const int st = chem_cell_fast<G>(K, G, e, fscale, src, nhplus, eint, nh, temp, a.chemonly, nit);
+    if (st != 0) raise_status(a.status, st, a.level, cell);
+    nit_acc += (unsigned)nit;
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      float ev = (float)e[g];
+      if (e[g] >= K.emin_grp[g] && ev < K.emin_grp_up[g]) ev = K.emin_grp_up[g];
+      const float fs = (float)fscale[g];
+      const float fx = gld(a.fxo + g * n, cell) * fs, fy = gld(a.fyo + g * n, cell) * fs,
+                  fz = gld(a.fzo + g * n, cell) * fs;
+      gst(a.eo + g * n, cell, ev);
+      gst(a.fxo + g * n, cell, fx);
+      gst(a.fyo + g * n, cell, fy);
+      gst(a.fzo + g * n, cell, fz);
+    }
+    gst(a.nhpluso, cell, nhplus);
+    gst(a.einto, cell, eint);
+    gst(a.tempo, cell, temp);
+  }
+  if (a.counters != nullptr) {
+    // one atomic per wave that did work
+    for (int o = 32; o > 0; o >>= 1) {
+      nit_acc += __shfl_xor(nit_acc, o, 64);
+      ncell_acc += __shfl_xor(ncell_acc, o, 64);
+    }
+    unsigned long long* slot = a.counters + (size_t)(blockIdx.x % kCounterSlots) * kCounterStride;
+    if ((threadIdx.x & 63) == 0 && ncell_acc != 0u) {
+      atomicAdd(slot, (unsigned long long)nit_acc);
+      atomicAdd(slot + 2, (unsigned long long)ncell_acc);
+    }
+  }
+}
+
 // ------------------------------------------------------------------ COARSERAD transfers
 // One thread per cell of the coarse level; a split cell walks its subtree
 // depth first, children in index order like the reference's recursion
@@ -996,16 +1084,24 @@ hipError_t launch_level_step(const LevelArgs<R>& a0, const ChemConst& K, hipStre
       a.nblocks = nb;
       // sparse levels: per-direction halo faces
       const bool sparse = a.halo_per_dir != 0;
+      const bool defer = a.chem_budget > 0;
+      if (defer && (a.defer_list == nullptr || a.defer_ctl == nullptr || a.defer_cap < a.ncell))
+        return hipErrorInvalidValue;
+      // the list holds at most the level's cells
+      const dim3 sg(std::min(kStragglerBlocks, (a.ncell + kStragglerThreads - 1) / kStragglerThreads)),
+          sb(kStragglerThreads);
       if (a.ng == 1) {
         if (sparse)
           hipLaunchKernelGGL((level_step_tiled<1, true>), dim3(nb), dim3(kTileCells), 0, s, a, K);
         else
           hipLaunchKernelGGL((level_step_tiled<1, false>), dim3(nb), dim3(kTileCells), 0, s, a, K);
+        if (defer) hipLaunchKernelGGL((chem_straggler_kernel<1>), sg, sb, 0, s, a, K);
       } else {
         if (sparse)
           hipLaunchKernelGGL((level_step_tiled<3, true>), dim3(nb), dim3(kTileCells), 0, s, a, K);
         else
           hipLaunchKernelGGL((level_step_tiled<3, false>), dim3(nb), dim3(kTileCells), 0, s, a, K);
+        if (defer) hipLaunchKernelGGL((chem_straggler_kernel<3>), sg, sb, 0, s, a, K);
       }
       return hipGetLastError();
     }

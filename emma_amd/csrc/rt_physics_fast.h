@@ -286,9 +286,12 @@ __device__ __forceinline__ void chem_fast_end(const ChemConst& K, int ng, int ch
   temp = s.eint * kT * frcp64((s.nh * K.unit_N / a3f) * (1. + s.x0));
 }
 
+// budget >= 0: at most `budget` sub-step tries; returns 1 with e, fscale,
+// nhplus, eint_code, temp untouched (fscale = 1) when the cell still runs
 template <int G>
 __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fscale, const double* src,
-                              double& nhplus, double& eint_code, double nh, double& temp, int chemonly, int& nit) {
+                              double& nhplus, double& eint_code, double nh, double& temp, int chemonly, int& nit,
+                              int budget = -1) {
   constexpr int GA = ChemFast<G>::GA;
 #pragma unroll
   for (int g = 0; g < GA; g++)
@@ -297,7 +300,7 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fsc
   int st = chem_fast_begin<G>(K, ng, e, src, nhplus, eint_code, nh, s);
   nit = 0;
   if (st != 0) return st;
-  st = chem_fast_run<G>(K, ng, chemonly, s, -1);
+  st = chem_fast_run<G>(K, ng, chemonly, s, budget);
   nit = s.nit;
   if (st != 0) return st;
   chem_fast_end<G>(K, ng, chemonly, s, e, fscale, nhplus, eint_code, temp);

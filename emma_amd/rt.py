@@ -62,6 +62,7 @@ EXPORTS = [
     "emma_rt_set_hydro", "emma_rt_get_hydro", "emma_rt_set_coupling", "emma_rt_set_oct_pos",
     "emma_rt_set_stromgren_source", "emma_rt_get_level_times", "emma_rt_alloc_pinned",
     "emma_rt_save_state", "emma_rt_restore_state", "emma_rt_set_stars", "emma_rt_set_star_sources",
+    "emma_rt_set_chem_budget", "emma_rt_get_chem_budget",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -80,6 +81,7 @@ class Stats(ctypes.Structure):
         ("bad_level", ctypes.c_int),
         ("bad_cell", ctypes.c_int),
         ("timed_spans", ctypes.c_int),
+        ("deferred_cells", ctypes.c_longlong),
     ]
 
     def as_dict(self):
@@ -143,6 +145,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
                                            ctypes.c_int]
     L.emma_rt_save_state.argtypes = [ctypes.c_void_p]
     L.emma_rt_restore_state.argtypes = [ctypes.c_void_p]
+    L.emma_rt_set_chem_budget.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.emma_rt_get_chem_budget.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
     _lib = L
     return L
 
@@ -279,6 +283,16 @@ class RadGPU:
         their graph between one pair of events); True/False map to
         TIMING_LAUNCH/TIMING_OFF"""
         self._check(self.lib.emma_rt_enable_timing(self.h, int(mode)))
+
+    def set_chem_budget(self, budget: int, level: int = -1):
+        """chemistry tries before a leaf goes to the straggler pass (0: off),
+        for one level or every level (-1); mixed precision, 1 or 3 groups"""
+        self._check(self.lib.emma_rt_set_chem_budget(self.h, int(level), int(budget)))
+
+    def chem_budget(self, level: int) -> int:
+        b = ctypes.c_int(0)
+        self._check(self.lib.emma_rt_get_chem_budget(self.h, int(level), ctypes.byref(b)))
+        return b.value
 
     def set_mode(self, mode: int):
         """RADSTEP (per-level transport) or COARSERAD (coarse-level transport)"""

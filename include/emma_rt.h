@@ -99,6 +99,7 @@ typedef struct emma_rt_stats {
   int bad_level;            /* level and cell of that error */
   int bad_cell;
   int timed_spans;          /* event-bracketed spans summed into kernel_ms */
+  long long deferred_cells; /* leaves whose chemistry the straggler pass ran (emma_rt_set_chem_budget) */
 } emma_rt_stats;
 
 /* ---- device management: interface.cu:61-84 (countdevices, initlocaldevice,
@@ -283,6 +284,22 @@ int emma_rt_set_star_sources(emma_rt_ctx *ctx, int enable, double srcint, double
    repeat a coarse step from the same state. */
 int emma_rt_save_state(emma_rt_ctx *ctx);
 int emma_rt_restore_state(emma_rt_ctx *ctx);
+
+/* Chemistry stragglers (mixed precision, 1 or 3 groups).  The reference's
+   chemrad sub-steps each cell until its dt is covered (chem_utils.c:453-767),
+   so a few cells near ionisation fronts take many more sub-steps than the rest;
+   on the GPU a wave then runs as long as its slowest lane.  With a budget
+   b > 0 for a level, a leaf still sub-stepping after b tries keeps its
+   transported N, F and leaves its chemistry to a second pass over the list of
+   such leaves, which restarts it from the same inputs, one thread per leaf.
+   The results are those of the undivided loop (the same sub-steps, rounded
+   alike up to FP64 contraction); the choice only changes the speed.  level -1
+   sets every level; budget 0 turns the pass off (the default).  No counterpart
+   in the reference (its GPU chemistry, chem_utils_gpu.cu:155-582, runs one
+   thread per cell to the end).  EMMA_RT_ERR_ARG on the FP64 and generic-group
+   paths. */
+int emma_rt_set_chem_budget(emma_rt_ctx *ctx, int level, int budget);
+int emma_rt_get_chem_budget(emma_rt_ctx *ctx, int level, int *budget);
 
 int emma_rt_get_stats(emma_rt_ctx *ctx, emma_rt_stats *out);
 /* device timing into emma_rt_stats.kernel_ms: EMMA_RT_TIMING_LAUNCH puts
