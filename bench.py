@@ -183,39 +183,43 @@ def _dominant(W, precision, level_us, launches_per_step):
     return l, achieved
 
 
-def _halo_per_dir(L) -> bool:
-    """The tiled kernel's variant choice (made by the library at tree
-    creation): some wave of 64 cells has more than 64 faces whose neighbour
-    oct lies outside the cell's 32-oct tile."""
-    import numpy as np
-
-    nei = L.nei_oct.reshape(-1, 6)
-    cells = np.arange(L.ncell)
-    oct_, ic, oct0 = cells >> 3, cells & 7, (cells // 256) * 32
-    cnt = np.zeros(L.ncell, dtype=np.int64)
-    for d in range(3):
-        no = nei[oct_, 2 * d + ((ic >> d) & 1)]
-        cnt += ~((no >= oct0) & (no < oct0 + 32))
-    pad = (-L.ncell) % 64
-    return bool(np.pad(cnt, (0, pad)).reshape(-1, 64).sum(1).max() > 64)
+KERNEL_NAMES = {"listed": "emma::level_step_tiled<{G}, false>", "perdir": "emma::level_step_tiled<{G}, true>",
+                "latency": "emma::level_step_tiled_latency<{G}>"}
 
 
-def _kernel_name(W, level, precision):
+def _kernel_name(W, level, precision, tuning):
     if precision != "mixed" or W.params.ngrp not in (1, 3):
         return f"emma::level_step_kernel<{'double' if precision == 'fp64' else 'float'}, {W.params.ngrp}, ...>"
-    per_dir = _halo_per_dir(W.tree.level(level))
-    return f"emma::level_step_tiled<{W.params.ngrp}, {'true' if per_dir else 'false'}>"
+    t = tuning[level]
+    name = KERNEL_NAMES[t["variant"]].format(G=W.params.ngrp)
+    return name + (" + emma::chem_straggler_kernel<%d>" % W.params.ngrp if t["chem_budget"] > 0 else "")
 
 
-def _level_table(W, precision, level_us, launches_per_step):
+def _configure(g, W, precision, tuning):
+    """apply the tuning table (mixed precision) and report each level's
+    variant and chemistry budget"""
+    from emma_amd import rt, tune
+
+    if tuning is not None and precision == "mixed":
+        tune.apply(g, tuning)
+    out = {}
+    for l in range(W.tree.lmin, W.tree.lmax + 1):
+        if precision == "mixed" and W.params.ngrp in (1, 3):
+            t = g.level_tuning(l)
+            out[l] = {"variant": rt.VARIANT_NAMES[t["variant"]], "chem_budget": t["chem_budget"]}
+    return out
+
+
+def _level_table(W, precision, level_us, launches_per_step, tuning):
     lb = level_bytes(W, precision)
-    return {str(l): {"leaf_cells": W.tree.leaf_cells(l), "launches_per_step": launches_per_step[l],
-                     "us_per_launch": level_us[l],
-                     "frac": lb[l] / (level_us[l] * 1e-6) / 1e9 / HBM_PEAK_GBS if level_us[l] > 0 else None}
+    return {str(l): dict({"leaf_cells": W.tree.leaf_cells(l), "launches_per_step": launches_per_step[l],
+                          "us_per_launch": level_us[l],
+                          "frac": lb[l] / (level_us[l] * 1e-6) / 1e9 / HBM_PEAK_GBS if level_us[l] > 0 else None},
+                         **tuning.get(l, {}))
             for l in sorted(level_us)}
 
 
-def run_uniform(W, precision, steps, warmup):
+def run_uniform(W, precision, steps, warmup, tuning=None):
     """A step is one radiative sub-cycle of the level.  The K timed sub-cycles
     are one replay of their captured graph (captured and run once on the same
     starting state, untimed)."""
@@ -224,6 +228,7 @@ def run_uniform(W, precision, steps, warmup):
     lvl = W.tree.lmin
     g = rt.RadGPU(W.params, rt.MIXED if precision == "mixed" else rt.FP64)
     g.setup(W.params, W.tree, W.fields)
+    used = _configure(g, W, precision, tuning)
     dt = W.params.dt_rad(lvl)
     for _ in range(warmup):
         g.radsolver(lvl, dt)
@@ -252,10 +257,11 @@ def run_uniform(W, precision, steps, warmup):
     ms, n = g.level_times()[lvl]
     g.close()
     return {"wall": wall, "stats": st, "graph_ms": graph_ms, "level_us": {lvl: ms / max(n, 1) * 1e3},
-            "launches_per_step": {lvl: 1}, "steps": steps, "alg_bytes_per_step": level_bytes(W, precision)[lvl]}
+            "launches_per_step": {lvl: 1}, "steps": steps, "alg_bytes_per_step": level_bytes(W, precision)[lvl],
+            "tuning": used}
 
 
-def run_amr(W, precision, steps, warmup):
+def run_amr(W, precision, steps, warmup, tuning=None, tune_here=False, log=None):
     """A step is one coarse step of the RADSTEP level recursion (level l takes
     2^(l-lmin) level steps), one replay of its captured HIP graph.  Every
     timed coarse step is the second coarse step of the run: on these seeded
@@ -269,9 +275,16 @@ def run_amr(W, precision, steps, warmup):
 
     g = rt.RadGPU(W.params, rt.MIXED if precision == "mixed" else rt.FP64)
     g.setup(W.params, W.tree, W.fields)
+    used = _configure(g, W, precision, tuning)
     g.coarse_step(1.0, 1)  # first coarse step, from the drawn state
     g.save_state()
     restore = g.restore_state
+    tuned = None
+    if tune_here and precision == "mixed" and W.params.ngrp in (1, 3):
+        from emma_amd import tune
+
+        tuned = tune.tune(g, reps=3, log=log)  # leaves the context set to the fastest per level
+        used = _configure(g, W, precision, None)
 
     # capture + instantiate the graphs of both buffer parities of the coarse level
     for _ in range(max(warmup, 2)):
@@ -311,14 +324,17 @@ def run_amr(W, precision, steps, warmup):
     lb = level_bytes(W, precision)
     per_coarse = sum(b << (l - W.tree.lmin) for l, b in lb.items())
     return {"wall": wall, "stats": tot, "graph_ms": graph_ms, "level_us": level_us, "launches_per_step": launches,
-            "steps": steps, "alg_bytes_per_step": per_coarse}
+            "steps": steps, "alg_bytes_per_step": per_coarse, "tuning": used, "tuned": tuned}
 
 
-def measure(name, W, precision, steps, warmup):
+def measure(name, W, precision, steps, warmup, tuning=None, tune_here=False):
     """One workload's timed region and device times, as the JSON fields of a
     bench line (value, roofline of the dominant kernel, per-level table)."""
-    runner = run_amr if name in ("c4", "c5") else run_uniform
-    r = runner(W, precision, steps, warmup)
+    if name in ("c4", "c5"):
+        r = run_amr(W, precision, steps, warmup, tuning, tune_here,
+                    log=lambda m: print("bench: " + m, file=sys.stderr, flush=True))
+    else:
+        r = run_uniform(W, precision, steps, warmup, tuning)
     cu = r["stats"]["cell_updates"]
     lvl, achieved = _dominant(W, precision, r["level_us"], r["launches_per_step"])
     step_ms = r["graph_ms"] / steps
@@ -329,10 +345,11 @@ def measure(name, W, precision, steps, warmup):
         "step_alg_frac": r["alg_bytes_per_step"] / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "chem_substeps_per_cell_update": r["stats"]["chem_substeps"] / max(cu, 1),
         "dominant_level": lvl,
-        "kernel": _kernel_name(W, lvl, precision),
+        "kernel": _kernel_name(W, lvl, precision, r["tuning"]),
         "kernel_us": r["level_us"][lvl],
         "achieved": achieved,
-        "levels": _level_table(W, precision, r["level_us"], r["launches_per_step"]),
+        "levels": _level_table(W, precision, r["level_us"], r["launches_per_step"], r["tuning"]),
+        "tuned": r.get("tuned"),
     }
 
 
@@ -429,6 +446,11 @@ def main():
     ap.add_argument("--no-fp64-plain", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--stencil-path", action="store_true")
+    ap.add_argument("--no-tuning", action="store_true",
+                    help="the library's default per-level choices instead of the tuning table")
+    ap.add_argument("--tune", default=None, metavar="PATH",
+                    help="run the autotuner on the workload first (outside the timed region), use its choices "
+                         "and merge them into the table at PATH")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the library runs the radiation step on one GPU's shard "
@@ -441,11 +463,18 @@ def main():
         copy_calibration()
         return
 
+    from emma_amd import tune
+
     precision = args.precision or ("fp64" if args.workload == "c2" else "mixed")
     W = make_workload(args.workload, args.seed)
-    print(f"bench: workload {args.workload} ({precision}), {args.steps} steps after {args.warmup} warm-up",
-          file=sys.stderr, flush=True)
-    _, m = measure(args.workload, W, precision, args.steps, args.warmup)
+    table = None if (args.no_tuning or args.tune) else tune.load()
+    tuning_src = ("autotuned in this run" if args.tune else
+                  "library defaults" if not table else "table emma_amd/tuning.json")
+    print(f"bench: workload {args.workload} ({precision}), {args.steps} steps after {args.warmup} warm-up, "
+          f"per-level tuning: {tuning_src}", file=sys.stderr, flush=True)
+    _, m = measure(args.workload, W, precision, args.steps, args.warmup, table, bool(args.tune))
+    if args.tune and m["tuned"]:
+        tune.save(m["tuned"], WORKLOAD_NAMES[args.workload], args.tune)
     step = {"c4": "coarse step: 31 level steps (levels 7-11), one HIP graph replay",
             "c5": "coarse step: 7 level steps (levels 8-10), one HIP graph replay"}.get(args.workload,
                                                                                     "radiative sub-cycle")
@@ -473,6 +502,7 @@ def main():
             "step_alg_frac": m["step_alg_frac"],
             "chem_substeps_per_cell_update": m["chem_substeps_per_cell_update"],
             "levels": m["levels"],
+            "tuning": tuning_src,
             "parallelism": "1 GPU",
         },
         "roofline": {
@@ -502,7 +532,9 @@ def main():
         W4 = make_workload("c4", 4)
         n4 = min(args.steps, 10)
         print(f"bench: c4 block, {n4} steps", file=sys.stderr, flush=True)
-        _, m4 = measure("c4", W4, precision, n4, min(args.warmup, 3))
+        _, m4 = measure("c4", W4, precision, n4, min(args.warmup, 3), table, bool(args.tune))
+        if args.tune and m4["tuned"]:
+            tune.save(m4["tuned"], WORKLOAD_NAMES["c4"], args.tune)
         out["c4"] = {"workload": WORKLOAD_NAMES["c4"], "step": "coarse step: 31 level steps, one HIP graph replay",
                      "value": m4["value"], "unit": "cell-updates/s", "steps": n4, "ms_per_step": m4["ms_per_step"],
                      "device_ms_per_step": m4["device_ms_per_step"], "step_alg_frac": m4["step_alg_frac"],

@@ -33,6 +33,9 @@ static hipError_t& last_hip_error() {
 
 static_assert(emma::kNeiTransmissive == EMMA_RT_NEI_TRANSMISSIVE && emma::kNeiReflective == EMMA_RT_NEI_REFLECTIVE,
               "domain-face codes of the kernels and the C-ABI");
+static_assert(emma::kListed == EMMA_RT_VARIANT_LISTED && emma::kPerDir == EMMA_RT_VARIANT_PERDIR &&
+                  emma::kLatency == EMMA_RT_VARIANT_LATENCY,
+              "halo variants of the kernels and the C-ABI");
 
 namespace {
 
@@ -40,7 +43,8 @@ struct DevLevel {
   int level = 0, noct = 0, ncell = 0, cur = 0;
   int npair = 0;  // reflux pairs: (leaf, face) with a finer oct behind the face
   int nslot = 0;  // accumulators per flux component and group: npair x sub-slots
-  int halo_per_dir = 0;  // some wave has more than 64 halo faces (tiled kernel variant)
+  int halo_per_dir = 0;  // some wave has more than 64 halo faces (no kListed variant)
+  int variant = kListed;  // halo variant of the tiled kernel
   int chem_budget = 0;   // chemistry tries before a leaf goes to the straggler pass (0: none)
   void* e[2] = {nullptr, nullptr};
   void* fx[2] = {nullptr, nullptr};
@@ -106,6 +110,7 @@ struct emma_rt_ctx {
   std::vector<DevLevel> lev;  // index level - lmin
   int* d_status = nullptr;
   unsigned long long* d_counters = nullptr;
+  unsigned long long h_counters[kCounterSlots * kCounterStride];  // read back by end_call
   double* d_photons = nullptr;
   // straggler list of the tiled mixed kernel (capacity: the largest level's
   // cells) and one list-length word per level-step launch of a call (slot
@@ -316,7 +321,7 @@ extern "C" int emma_rt_create(const emma_rt_params* p, emma_rt_ctx** out) {
   if (launch_fill((unsigned int*)c->d_status, 4, 0u, c->stream) != hipSuccess ||
       launch_fill((unsigned int*)c->d_counters, 2 * kCounterSlots * kCounterStride, 0u, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess) {
-    delete c;
+    emma_rt_destroy(c);
     return EMMA_RT_ERR_HIP;
   }
   *out = c;
@@ -351,6 +356,17 @@ extern "C" int emma_rt_destroy(emma_rt_ctx* c) {
 template <typename T>
 static hipError_t dalloc(T** p, size_t n) {
   return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+// the halo variant of the tiled kernel a level gets unless the caller picks
+// one (emma_rt_set_level_variant): kListed on complete levels; on sparse
+// levels kLatency when the level has too few tiles to give every CU one,
+// kPerDir otherwise
+constexpr int kLatencyTiles = 256;
+static int default_variant(const DevLevel& L) {
+  if (!L.halo_per_dir) return kListed;
+  const int tiles = (L.ncell + kTileCells - 1) / kTileCells;
+  return tiles <= kLatencyTiles ? kLatency : kPerDir;
 }
 
 extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int* noct, const int* nei_oct,
@@ -445,6 +461,7 @@ extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int
       maxh = std::max(maxh, cnt);
     }
     L.halo_per_dir = maxh > 64 ? 1 : 0;
+    L.variant = default_variant(L);
   }
   // device allocation and upload
   for (int k = 0; k < nl; k++) {
@@ -653,7 +670,7 @@ static LevelArgs<R> level_args(emma_rt_ctx* c, int k, double dtnew, double aexp,
     A.keep_split_flux = 0;
     A.update_split = 0;
   }
-  A.halo_per_dir = L.halo_per_dir;
+  A.variant = L.variant;
   if (c->d_defer != nullptr && L.chem_budget > 0 && c->defer_seq < c->defer_slots) {
     A.chem_budget = L.chem_budget;
     A.defer_cap = c->defer_cap;
@@ -794,9 +811,9 @@ static int begin_call(emma_rt_ctx* c) {
 static int end_call(emma_rt_ctx* c) {
   HIPCHK(hipStreamSynchronize(c->stream));
   int status[4];
-  std::vector<unsigned long long> slots(kCounterSlots * kCounterStride);
+  const unsigned long long* slots = c->h_counters;
   HIPCHK(hipMemcpy(status, c->d_status, sizeof(status), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(slots.data(), c->d_counters, slots.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(c->h_counters, c->d_counters, sizeof(c->h_counters), hipMemcpyDeviceToHost));
   unsigned long long cnt[3] = {0ull, 0ull, 0ull};
   for (int i = 0; i < kCounterSlots; i++) {
     cnt[0] += slots[(size_t)i * kCounterStride + 0];
@@ -1238,6 +1255,29 @@ extern "C" int emma_rt_set_chem_budget(emma_rt_ctx* c, int level, int budget) {
   return EMMA_RT_OK;
 }
 
+extern "C" int emma_rt_set_level_variant(emma_rt_ctx* c, int level, int variant) {
+  if (c == nullptr || !c->has_tree) return EMMA_RT_ERR_ARG;
+  if (variant < EMMA_RT_VARIANT_AUTO || variant > EMMA_RT_VARIANT_LATENCY) return EMMA_RT_ERR_ARG;
+  if (level != -1 && level_of(c, level) == nullptr) return EMMA_RT_ERR_ARG;
+  for (auto& L : c->lev) {
+    if (level != -1 && L.level != level) continue;
+    if (variant == EMMA_RT_VARIANT_LISTED && L.halo_per_dir) return EMMA_RT_ERR_ARG;  // more halo faces than slots
+  }
+  for (auto& L : c->lev)
+    if (level == -1 || L.level == level) L.variant = variant == EMMA_RT_VARIANT_AUTO ? default_variant(L) : variant;
+  drop_graphs(c);
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_get_level_tuning(emma_rt_ctx* c, int level, int* variant, int* budget, int* halo_per_dir) {
+  DevLevel* L = level_of(c, level);
+  if (L == nullptr) return EMMA_RT_ERR_ARG;
+  if (variant) *variant = L->variant;
+  if (budget) *budget = L->chem_budget;
+  if (halo_per_dir) *halo_per_dir = L->halo_per_dir;
+  return EMMA_RT_OK;
+}
+
 extern "C" int emma_rt_get_chem_budget(emma_rt_ctx* c, int level, int* budget) {
   DevLevel* L = level_of(c, level);
   if (L == nullptr || budget == nullptr) return EMMA_RT_ERR_ARG;
@@ -1405,11 +1445,10 @@ extern "C" int emma_rt_destroy_pinned_stencil(void* stencil) {
   return EMMA_RT_OK;
 }
 
-extern "C" int emma_rt_advance_stencil(emma_rt_ctx* c, void* stencil, int nread, int level, double dxcur,
-                                       double dtnew, double aexp, int chemonly) {
-  if (c == nullptr || stencil == nullptr || nread < 0) return EMMA_RT_ERR_ARG;
-  if (c->d_stencil == nullptr) return EMMA_RT_ERR_STATE;
-  if (nread > c->stride) return EMMA_RT_ERR_ARG;
+// the stencil path on a device staging area `dbuf` holding at least nread
+// records of REAL = real_bytes
+static int advance_stencil_on(emma_rt_ctx* c, void* dbuf, int real_bytes, void* stencil, int nread, int level,
+                              double dxcur, double dtnew, double aexp, int chemonly) {
   if (c->mode == EMMA_RT_COARSERAD && c->p.lcoarse < 1) return EMMA_RT_ERR_STATE;
   if (nread == 0) return EMMA_RT_OK;
   // condadvec of advancerad / advanceradGPU (rad_utils.c:1808-1812,
@@ -1417,7 +1456,7 @@ extern "C" int emma_rt_advance_stencil(emma_rt_ctx* c, void* stencil, int nread,
   const int coarserad = c->mode == EMMA_RT_COARSERAD;
   const int transport = coarserad ? (level == c->p.lcoarse && !chemonly) : 1;
   RgridLayout L;
-  rgrid_layout(c->p.ngrp, c->stencil_rbytes, &L);
+  rgrid_layout(c->p.ngrp, real_bytes, &L);
   int st = begin_call(c);
   if (st) return st;
   // The kernel reads the 7 oct slots and each centre cell's rfieldnew and
@@ -1430,7 +1469,7 @@ extern "C" int emma_rt_advance_stencil(emma_rt_ctx* c, void* stencil, int nread,
   const size_t noff = (size_t)L.v[21], fsize = (size_t)L.v[12], rfn = (size_t)L.v[15], rtype = (size_t)L.v[0];
   const size_t back = fsize - rfn;  // rfieldnew + rflux
   char* h = (char*)stencil;
-  char* d = (char*)c->d_stencil;
+  char* d = (char*)dbuf;
   HIPCHK(hipMemcpy2DAsync(d + ooff, rsize, h + ooff, rsize, osize, (size_t)nread, hipMemcpyHostToDevice, c->stream));
   for (int ic = 0; ic < 8; ic++) {
     const size_t o = noff + ic * fsize + rfn;
@@ -1438,8 +1477,8 @@ extern "C" int emma_rt_advance_stencil(emma_rt_ctx* c, void* stencil, int nread,
   }
   const ChemConst K = make_chem(c->p, dtnew, aexp);
   const double cl = cloc_of(c->p, aexp);
-  HIPCHK(launch_stencil(c->d_stencil, L, c->p.ngrp, c->stencil_rbytes, nread, cl, dtnew / dxcur, chemonly, coarserad,
-                        transport, K, c->d_status, c->d_counters, c->stream));
+  HIPCHK(launch_stencil(dbuf, L, c->p.ngrp, real_bytes, nread, cl, dtnew / dxcur, chemonly, coarserad, transport, K,
+                        c->d_status, c->d_counters, c->stream));
   // without transport the kernel writes no fluxes: rfieldnew alone comes back
   const size_t nback = transport ? back : rtype;
   for (int ic = 0; ic < 8; ic++) {
@@ -1448,3 +1487,12 @@ extern "C" int emma_rt_advance_stencil(emma_rt_ctx* c, void* stencil, int nread,
   }
   return end_call(c);
 }
+
+extern "C" int emma_rt_advance_stencil(emma_rt_ctx* c, void* stencil, int nread, int level, double dxcur,
+                                       double dtnew, double aexp, int chemonly) {
+  if (c == nullptr || stencil == nullptr || nread < 0) return EMMA_RT_ERR_ARG;
+  if (c->d_stencil == nullptr) return EMMA_RT_ERR_STATE;
+  if (nread > c->stride) return EMMA_RT_ERR_ARG;
+  return advance_stencil_on(c, c->d_stencil, c->stencil_rbytes, stencil, nread, level, dxcur, dtnew, aexp, chemonly);
+}
+

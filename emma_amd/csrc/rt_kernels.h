@@ -19,9 +19,9 @@ struct LevelArgs {
   // level and not for chemonly calls, fluxes of split neighbours kept, split
   // cells transport-updated (their chemistry still skipped).
   int transport, keep_split_flux, update_split;
-  // mixed path: 1 when some wave of the level has more than 64 halo faces
-  // (sparse levels: the per-direction halo variant of level_step_tiled)
-  int halo_per_dir;
+  // mixed path, 1 or 3 groups: the halo variant of the tiled kernel
+  // (kListed, kPerDir or kLatency)
+  int variant;
   int cnslot;  // accumulators in cracc
   // chemistry stragglers (tiled mixed kernel): a leaf whose sub-step loop is
   // still running after chem_budget tries (> 0; <= 0: no limit) stores its
@@ -117,6 +117,22 @@ constexpr int kRefluxTiled = 1;
 constexpr int kTileCells = 256;
 constexpr int kTileOcts = kTileCells / 8;
 constexpr int kTileWaves = kTileCells / 64;
+// Halo variants of the tiled level step.
+//  kListed  each wave lists its halo faces (neighbours outside the tile) and
+//           evaluates them lane-parallel in one round; at most 64 per wave,
+//           which holds on complete levels (16 in x and y, 32 in z).
+//  kPerDir  each lane evaluates its own halo face of direction d at the
+//           start of pass d (up to 192 per wave: sparse AMR levels); the
+//           chemistry inputs are loaded when the chemistry starts (fewer
+//           live registers: four work-groups per CU).
+//  kLatency as kPerDir, but the halo faces of all three directions and the
+//           chemistry inputs are loaded up front: one memory round trip
+//           instead of four, with the registers that takes (two waves per
+//           SIMD).  For levels too small to fill the GPU, whose step is one
+//           work-group's latency chain.
+constexpr int kListed = 0;
+constexpr int kPerDir = 1;
+constexpr int kLatency = 2;
 // grid of the chemistry straggler pass (it loops over the list)
 constexpr int kStragglerThreads = 256;
 constexpr int kStragglerBlocks = 4096;

@@ -549,14 +549,23 @@ __device__ __forceinline__ float lane_xor(float x, int m) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(t, 0x1B, 0xF, 0xF, true));              // quad_perm 3,2,1,0
 }
 
-template <int G, bool PerDir>
-__global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4))) void level_step_tiled(
-    LevelArgs<float> a_by_value, ChemConst K_by_value) {
+// The halo variants kListed, kPerDir, kLatency (rt_kernels.h) share this
+// body; the kernels below instantiate it.
+// one halo face: the neighbour's state, its split flag, the face direction
+// and the side the neighbour lies on
+template <int G>
+struct HaloFace {
+  float v[G][4];
+  int child, d, side;
+  bool lane, coarse, refl;
+};
+
+template <int G, int V>
+__device__ __forceinline__ void tiled_step(const LevelArgs<float>& a, const ChemConst& K) {
+  constexpr bool PerDir = V != kListed;
+  constexpr bool Late = V == kPerDir;  // chemistry inputs loaded when the chemistry starts
   constexpr int kHaloCap = 64;                 // halo slots per wave
   constexpr int kList = PerDir ? 1 : kHaloCap;  // halo list entries per wave
-  const KernArgs<float> args = kernargs<float>();
-  const LevelArgs<float>& a = args.a;
-  const ChemConst& K = args.K;
   // shares: [g][A or B][cell] of the current direction, then the halo slots
   // [wave][slot][g]
   constexpr int kShares = G * 2 * kTileCells;
@@ -596,7 +605,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
     // the chemistry inputs: early on complete levels (their latency hides
     // behind the transport); on sparse levels, whose per-direction halo
     // loads need the registers, when the chemistry starts (finish_leaf)
-    if constexpr (!PerDir) {
+    if constexpr (!Late) {
 #pragma unroll
       for (int g = 0; g < G; g++) Src[g] = gld(a.src + g * n, cell);
       pre[0] = gld(a.nhplus, cell);
@@ -631,87 +640,88 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  // ---- one halo face per lane: issue the neighbour's loads ...
-  // Straight-line code (every lane loads, lanes without a halo face from
-  // cell 0): a load under a divergent branch makes the wait before the
-  // closures below cover it as well.
-  bool hlane = false, hcoarse = false;
-  int hd = 0, hside = 0, hchild = -1;
-  float hv[G][4];
-  // the neighbour across the face in direction hd that leaves the oct of
+  // the neighbour across the face in direction h.d that leaves the oct of
   // cell hcell, nei_oct entry hno: a same-level cell, an injected coarse
   // leaf, or on a domain face a copy of a cell of the same oct (its normal
   // flux negated when reflective)
-  auto load_neighbour = [&](int hno, int hcell) {
+  auto load_neighbour = [&](HaloFace<G>& h, int hno, int hcell) {
+    // only loads here: the reflective sign and nothing else that needs the
+    // loaded values is applied before store_item, so the loads of several
+    // faces can be in flight together
     const int hic = hcell & 7;
-    hside = (hic >> hd) & 1;
-    const bool hbound = hlane && hno <= kNeiReflective;
-    hcoarse = hlane && hno < 0 && !hbound;
-    const int js = hbound ? (hcell & ~7) + trans_source(2 * hd + hside, hic)
-                          : ((hlane && !hcoarse) ? hno * 8 + (hic ^ (1 << hd)) : 0);
-#pragma unroll
-    for (int g = 0; g < G; g++) {
-      hv[g][0] = gld(a.e + g * n, js);
-      hv[g][1] = gld(a.fx + g * n, js);
-      hv[g][2] = gld(a.fy + g * n, js);
-      hv[g][3] = gld(a.fz + g * n, js);
-    }
-    hchild = gld(a.child, js);
-    {  // reflective face: negate the normal component (hd differs between lanes: no indexing by it)
-      const bool refl = hbound && hno == kNeiReflective;
-      const float sx = (refl && hd == 0) ? -1.f : 1.f, sy = (refl && hd == 1) ? -1.f : 1.f,
-                  sz = (refl && hd == 2) ? -1.f : 1.f;
+    h.side = (hic >> h.d) & 1;
+    const bool hbound = h.lane && hno <= kNeiReflective;
+    h.coarse = h.lane && hno < 0 && !hbound;
+    h.refl = hbound && hno == kNeiReflective;
+    const int js = hbound ? (hcell & ~7) + trans_source(2 * h.d + h.side, hic)
+                          : ((h.lane && !h.coarse) ? hno * 8 + (hic ^ (1 << h.d)) : 0);
+    h.child = gld(a.child, js);
+    if (__ballot(h.coarse) == 0) {
 #pragma unroll
       for (int g = 0; g < G; g++) {
-        hv[g][1] *= sx;
-        hv[g][2] *= sy;
-        hv[g][3] *= sz;
+        h.v[g][0] = gld(a.e + g * n, js);
+        h.v[g][1] = gld(a.fx + g * n, js);
+        h.v[g][2] = gld(a.fy + g * n, js);
+        h.v[g][3] = gld(a.fz + g * n, js);
       }
-    }
-    if (__ballot(hcoarse) != 0) {  // injected coarse leaves (levels with a coarser level only)
-      const int jc = hcoarse ? -1 - hno : 0;
+    } else {  // injected coarse leaves (levels with a coarser level only): per-lane arrays
       const int nc = a.ncoarse;
+      const int j = h.coarse ? -1 - hno : js;
+      const int st = h.coarse ? nc : n;
+      const float* pe = h.coarse ? a.ce : a.e;
+      const float* px = h.coarse ? a.cfx : a.fx;
+      const float* py = h.coarse ? a.cfy : a.fy;
+      const float* pz = h.coarse ? a.cfz : a.fz;
 #pragma unroll
       for (int g = 0; g < G; g++) {
-        const float ce = gld(a.ce + g * nc, jc), cx = gld(a.cfx + g * nc, jc), cy = gld(a.cfy + g * nc, jc),
-                    cz = gld(a.cfz + g * nc, jc);
-        if (hcoarse) {
-          hv[g][0] = ce;
-          hv[g][1] = cx;
-          hv[g][2] = cy;
-          hv[g][3] = cz;
-        }
+        h.v[g][0] = pe[g * st + j];
+        h.v[g][1] = px[g * st + j];
+        h.v[g][2] = py[g * st + j];
+        h.v[g][3] = pz[g * st + j];
       }
     }
   };
   // complete levels: list entry `slot` (read by every lane, used by halo lanes)
-  auto load_item = [&](int slot) {
-    hlane = slot < nhalo;
+  auto load_item = [&](HaloFace<G>& h, int slot) {
+    h.lane = slot < nhalo;
     const int it = sItem[wv][PerDir ? 0 : slot];
-    hd = it & 3;
-    load_neighbour(sItemNei[wv][PerDir ? 0 : slot], (cell & ~63) + (it >> 2));
+    h.d = it & 3;
+    load_neighbour(h, sItemNei[wv][PerDir ? 0 : slot], (cell & ~63) + (it >> 2));
   };
   // sparse levels: this lane's own face in direction d
-  auto load_face = [&](int d) {
-    hlane = moves && !in_tile(noD[d]);
-    hd = d;
-    load_neighbour(noD[d], cell);
+  auto load_face = [&](HaloFace<G>& h, int d) {
+    h.lane = moves && !in_tile(noD[d]);
+    h.d = d;
+    load_neighbour(h, noD[d], cell);
   };
   // ... the neighbour's share (a neighbour on the plus side of the face,
   // side 1, contributes B, on the minus side A) and cancel flag
-  auto store_item = [&](int slot) {
-    if (hlane) {
-      sCancel[kTileCells + wv * kHaloCap + slot] = (unsigned char)(!hcoarse && !a.keep_split_flux && hchild >= 0);
+  auto store_item = [&](const HaloFace<G>& h, int slot) {
+    if (h.lane) {
+      sCancel[kTileCells + wv * kHaloCap + slot] = (unsigned char)(!h.coarse && !a.keep_split_flux && h.child >= 0);
+      // reflective face: the normal component negated (h.d differs between lanes: no indexing by it)
+      const float sx = (h.refl && h.d == 0) ? -1.f : 1.f, sy = (h.refl && h.d == 1) ? -1.f : 1.f,
+                  sz = (h.refl && h.d == 2) ? -1.f : 1.f;
 #pragma unroll
       for (int g = 0; g < G; g++) {
-        const Closure<float> q = closure_fast(hv[g][1], hv[g][2], hv[g][3], hv[g][0], c);
+        const float vx = h.v[g][1] * sx, vy = h.v[g][2] * sy, vz = h.v[g][3] * sz;
+        const Closure<float> q = closure_fast(vx, vy, vz, h.v[g][0], c);
         float X[4];
-        lf_share_lane(hd, hside ? -1.f : 1.f, hv[g][0], hv[g][1], hv[g][2], hv[g][3], q, c, X);
+        lf_share_lane(h.d, h.side ? -1.f : 1.f, h.v[g][0], vx, vy, vz, q, c, X);
         sShare[kShares + (wv * kHaloCap + slot) * G + g] = make_float4(X[0], X[1], X[2], X[3]);
       }
     }
   };
-  if constexpr (!PerDir) load_item(lane);
+  // ---- one halo face per lane: issue the neighbour's loads ...
+  // Straight-line code (every lane loads, lanes without a halo face from
+  // cell 0): a load under a divergent branch makes the wait before the
+  // closures below cover it as well.
+  HaloFace<G> hf[V == kLatency ? 3 : 1];
+  if constexpr (V == kLatency) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) load_face(hf[d], d);
+  }
+  if constexpr (!PerDir) load_item(hf[0], lane);
   // the reflux pairs of the cell's leaving faces that have a coarse leaf
   // behind them, loaded now for the refluxing in the direction passes
   int pairD[3] = {-1, -1, -1};
@@ -726,7 +736,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
     for (int g = 0; g < G; g++) Q[g] = closure_fast(Ux[g], Uy[g], Uz[g], Ue[g], c);
   }
-  if constexpr (!PerDir) store_item(lane);
+  if constexpr (!PerDir) store_item(hf[0], lane);
 
   float De[G], Dx[G], Dy[G], Dz[G];
 #pragma unroll
@@ -735,7 +745,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   if (transport) {
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      if constexpr (PerDir) load_face(d);  // latency behind this cell's shares
+      if constexpr (V == kPerDir) load_face(hf[0], d);  // latency behind this cell's shares
       if (valid) {
 #pragma unroll
         for (int g = 0; g < G; g++) {
@@ -746,7 +756,7 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
           sShare[(g * 2 + 1) * kTileCells + lc] = make_float4(X[0], X[1], X[2], X[3]);
         }
       }
-      if constexpr (PerDir) store_item(lane);  // own slot: written and read by this lane only
+      if constexpr (PerDir) store_item(hf[V == kLatency ? d : 0], lane);  // own slot: written and read by this lane only
       __syncthreads();
       float Rf[G][4];  // flux through a coarse face (refluxing), 0 elsewhere
       int rj = -1;     // the coarse leaf on that face
@@ -892,10 +902,24 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
     } else {
       leaf = 1;
       nit_acc = (unsigned)finish_leaf<float, G, true, kRefluxTiled>(a, K, cell, ch, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz,
-                                                                    PerDir ? nullptr : Src, PerDir ? nullptr : pre);
+                                                                    Late ? nullptr : Src, Late ? nullptr : pre);
     }
   }
   if (a.counters != nullptr) block_counters<kTileWaves>(a.counters, nit_acc, leaf);
+}
+
+template <int G, bool PerDir>
+__global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4))) void level_step_tiled(
+    LevelArgs<float> a_by_value, ChemConst K_by_value) {
+  const KernArgs<float> args = kernargs<float>();
+  tiled_step<G, PerDir ? kPerDir : kListed>(args.a, args.K);
+}
+
+template <int G>
+__global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(2))) void level_step_tiled_latency(
+    LevelArgs<float> a_by_value, ChemConst K_by_value) {
+  const KernArgs<float> args = kernargs<float>();
+  tiled_step<G, kLatency>(args.a, args.K);
 }
 
 // ------------------------------------------------------------------ chemistry stragglers
@@ -1083,7 +1107,7 @@ hipError_t launch_level_step(const LevelArgs<R>& a0, const ChemConst& K, hipStre
       nb = (nb + 7) & ~7;
       a.nblocks = nb;
       // sparse levels: per-direction halo faces
-      const bool sparse = a.halo_per_dir != 0;
+      const int v = a.variant;
       const bool defer = a.chem_budget > 0;
       if (defer && (a.defer_list == nullptr || a.defer_ctl == nullptr || a.defer_cap < a.ncell))
         return hipErrorInvalidValue;
@@ -1091,13 +1115,17 @@ hipError_t launch_level_step(const LevelArgs<R>& a0, const ChemConst& K, hipStre
       const dim3 sg(std::min(kStragglerBlocks, (a.ncell + kStragglerThreads - 1) / kStragglerThreads)),
           sb(kStragglerThreads);
       if (a.ng == 1) {
-        if (sparse)
+        if (v == kLatency)
+          hipLaunchKernelGGL((level_step_tiled_latency<1>), dim3(nb), dim3(kTileCells), 0, s, a, K);
+        else if (v == kPerDir)
           hipLaunchKernelGGL((level_step_tiled<1, true>), dim3(nb), dim3(kTileCells), 0, s, a, K);
         else
           hipLaunchKernelGGL((level_step_tiled<1, false>), dim3(nb), dim3(kTileCells), 0, s, a, K);
         if (defer) hipLaunchKernelGGL((chem_straggler_kernel<1>), sg, sb, 0, s, a, K);
       } else {
-        if (sparse)
+        if (v == kLatency)
+          hipLaunchKernelGGL((level_step_tiled_latency<3>), dim3(nb), dim3(kTileCells), 0, s, a, K);
+        else if (v == kPerDir)
           hipLaunchKernelGGL((level_step_tiled<3, true>), dim3(nb), dim3(kTileCells), 0, s, a, K);
         else
           hipLaunchKernelGGL((level_step_tiled<3, false>), dim3(nb), dim3(kTileCells), 0, s, a, K);

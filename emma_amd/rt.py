@@ -51,6 +51,12 @@ TIMING_OFF = 0
 TIMING_LAUNCH = 1
 TIMING_GRAPH = 2
 
+VARIANT_AUTO = -1
+VARIANT_LISTED = 0
+VARIANT_PERDIR = 1
+VARIANT_LATENCY = 2
+VARIANT_NAMES = {VARIANT_LISTED: "listed", VARIANT_PERDIR: "perdir", VARIANT_LATENCY: "latency"}
+
 EXPORTS = [
     "emma_rt_countdevices", "emma_rt_initlocaldevice", "emma_rt_checkdevice", "emma_rt_strerror",
     "emma_rt_create", "emma_rt_set_params", "emma_rt_destroy",
@@ -62,7 +68,7 @@ EXPORTS = [
     "emma_rt_set_hydro", "emma_rt_get_hydro", "emma_rt_set_coupling", "emma_rt_set_oct_pos",
     "emma_rt_set_stromgren_source", "emma_rt_get_level_times", "emma_rt_alloc_pinned",
     "emma_rt_save_state", "emma_rt_restore_state", "emma_rt_set_stars", "emma_rt_set_star_sources",
-    "emma_rt_set_chem_budget", "emma_rt_get_chem_budget",
+    "emma_rt_set_chem_budget", "emma_rt_get_chem_budget", "emma_rt_set_level_variant", "emma_rt_get_level_tuning",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -147,6 +153,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.emma_rt_restore_state.argtypes = [ctypes.c_void_p]
     L.emma_rt_set_chem_budget.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     L.emma_rt_get_chem_budget.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
+    L.emma_rt_set_level_variant.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.emma_rt_get_level_tuning.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _ip, _ip]
     _lib = L
     return L
 
@@ -293,6 +301,16 @@ class RadGPU:
         b = ctypes.c_int(0)
         self._check(self.lib.emma_rt_get_chem_budget(self.h, int(level), ctypes.byref(b)))
         return b.value
+
+    def set_level_variant(self, variant: int, level: int = -1):
+        """halo variant of the tiled kernel (VARIANT_*) for one level or all (-1)"""
+        self._check(self.lib.emma_rt_set_level_variant(self.h, int(level), int(variant)))
+
+    def level_tuning(self, level: int) -> dict:
+        v, b, h = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.lib.emma_rt_get_level_tuning(self.h, int(level), ctypes.byref(v), ctypes.byref(b),
+                                                      ctypes.byref(h)))
+        return {"variant": v.value, "chem_budget": b.value, "halo_per_dir": bool(h.value)}
 
     def set_mode(self, mode: int):
         """RADSTEP (per-level transport) or COARSERAD (coarse-level transport)"""

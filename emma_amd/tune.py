@@ -1,0 +1,130 @@
+"""Autotuner of the mixed-precision level step.
+
+The reference picks its GPU launch shape from run parameters (threads per
+block ``nthread``, streams ``nstream``, stencil batch ``hstride``:
+``parameters.c:571-572, 653-656``, ``rad_utils_gpu.cu:1025-1043``).  Here the
+per-level choices are
+
+* the halo variant of the tiled kernel (``VARIANT_LISTED`` / ``_PERDIR`` /
+  ``_LATENCY``, see ``emma_rt_set_level_variant``), and
+* the chemistry sub-step budget after which a leaf goes to the straggler
+  pass (``emma_rt_set_chem_budget``; 0 = no pass),
+
+which change the speed and not the results.  :func:`tune` times every
+candidate on the workload's own state (event-timed level steps of repeated
+coarse steps from one saved device state) and keeps, per level, the fastest.
+The table (:func:`save` / :func:`load`) is keyed by the level's shape --
+groups, cells, complete or sparse -- and :func:`apply` sets a context's
+levels from it.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+from . import rt
+
+TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+BUDGETS = (0, 2, 3, 4, 6, 8)
+
+
+def level_key(ngrp: int, ncell: int, halo_per_dir: bool) -> str:
+    return f"G{ngrp}-{'sparse' if halo_per_dir else 'complete'}-{ncell}"
+
+
+def _level_times(g, levels, reps):
+    """mean device time (us) of each level's step over `reps` coarse steps
+    from the saved state"""
+    acc = {l: [0.0, 0] for l in levels}
+    g.enable_timing(rt.TIMING_LAUNCH)
+    for _ in range(reps):
+        g.restore_state()
+        g.coarse_step(1.0, 1)
+        for l, (ms, n) in g.level_times().items():
+            if l in acc:
+                acc[l][0] += ms
+                acc[l][1] += n
+    g.enable_timing(rt.TIMING_OFF)
+    return {l: a[0] / max(a[1], 1) * 1e3 for l, a in acc.items()}
+
+
+def tune(g: "rt.RadGPU", reps: int = 3, budgets=BUDGETS, log=None) -> dict:
+    """Per level of the context's tree, the (variant, budget) with the
+    smallest event-timed level step.  The context must hold a mixed-precision
+    tree with 1 or 3 groups and a saved state (``g.save_state()``) to start
+    every timed coarse step from.  Returns {level: {variant, chem_budget,
+    us, default_us, candidates}}."""
+    tree = g.tree
+    levels = [l for l in range(tree.lmin, tree.lmax + 1) if tree.level(l).noct > 0]
+    shape = {l: g.level_tuning(l) for l in levels}
+    default = {l: (shape[l]["variant"], shape[l]["chem_budget"]) for l in levels}
+    t_default = _level_times(g, levels, reps)
+    cands = {l: {} for l in levels}
+    variants = [rt.VARIANT_LISTED, rt.VARIANT_PERDIR, rt.VARIANT_LATENCY]
+    for v in variants:
+        for b in budgets:
+            for l in levels:  # a variant a level cannot take keeps its default
+                ok = not (v == rt.VARIANT_LISTED and shape[l]["halo_per_dir"])
+                g.set_level_variant(v if ok else default[l][0], level=l)
+                g.set_chem_budget(b, level=l)
+            t = _level_times(g, levels, reps)
+            for l in levels:
+                if v == rt.VARIANT_LISTED and shape[l]["halo_per_dir"]:
+                    continue
+                cands[l][(v, b)] = t[l]
+            if log:
+                log(f"tune: variant {rt.VARIANT_NAMES[v]} budget {b}: " +
+                    ", ".join(f"{l}: {t[l]:.1f} us" for l in levels))
+    out = {}
+    for l in levels:
+        (v, b), us = min(cands[l].items(), key=lambda kv: kv[1])
+        g.set_level_variant(v, level=l)
+        g.set_chem_budget(b, level=l)
+        L = tree.level(l)
+        out[l] = {"key": level_key(g.params.ngrp, L.ncell, shape[l]["halo_per_dir"]),
+                  "variant": rt.VARIANT_NAMES[v], "chem_budget": b, "us": us, "default_us": t_default[l],
+                  "default": {"variant": rt.VARIANT_NAMES[default[l][0]], "chem_budget": default[l][1]},
+                  "candidates": {f"{rt.VARIANT_NAMES[cv]}/{cb}": round(ct, 2) for (cv, cb), ct in cands[l].items()}}
+    return out
+
+
+def load(path: str = TABLE) -> dict:
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def save(result: dict, workload: str, path: str = TABLE) -> dict:
+    """merge a tune() result into the table at `path` (entries keyed by level
+    shape; `workload` names where they were measured)"""
+    table = load(path)
+    for l, r in result.items():
+        table[r["key"]] = {"variant": r["variant"], "chem_budget": r["chem_budget"], "us": round(r["us"], 2),
+                           "default_us": round(r["default_us"], 2), "workload": workload, "level": int(l)}
+    with open(path, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+        f.write("\n")
+    return table
+
+
+def apply(g: "rt.RadGPU", table: dict = None) -> dict:
+    """set every level whose shape the table holds; returns {level: entry}"""
+    if table is None:
+        table = load()
+    names = {v: k for k, v in rt.VARIANT_NAMES.items()}
+    used = {}
+    if g.precision != rt.MIXED or g.params.ngrp not in (1, 3):
+        return used
+    for l in range(g.tree.lmin, g.tree.lmax + 1):
+        L = g.tree.level(l)
+        if L.noct == 0:
+            continue
+        sh = g.level_tuning(l)
+        e = table.get(level_key(g.params.ngrp, L.ncell, sh["halo_per_dir"]))
+        if e is None:
+            continue
+        g.set_level_variant(names[e["variant"]], level=l)
+        g.set_chem_budget(int(e["chem_budget"]), level=l)
+        used[l] = e
+    return used

@@ -300,6 +300,28 @@ int emma_rt_restore_state(emma_rt_ctx *ctx);
    paths. */
 int emma_rt_set_chem_budget(emma_rt_ctx *ctx, int level, int budget);
 int emma_rt_get_chem_budget(emma_rt_ctx *ctx, int level, int *budget);
+/* Halo variant of the mixed-precision level step (1 or 3 groups) for one
+   level or every level (-1).  The results are the same; the variants differ
+   in how a tile reaches its neighbours outside the tile (the halo):
+   LISTED  each wave lists its halo faces, one round (at most 64 per wave:
+           complete levels; EMMA_RT_ERR_ARG on a level with more);
+   PERDIR  each lane's own halo face per direction pass, chemistry inputs
+           loaded late (fewest live registers; large sparse levels);
+   LATENCY all halo faces and inputs loaded up front (one memory round trip;
+           small levels whose step is one work-group's latency);
+   AUTO    the library's choice at tree creation (LISTED where possible,
+           LATENCY on sparse levels of at most 256 tiles of 32 octs, PERDIR
+           otherwise).  The reference's analogue is the CUDA launch shape it
+           takes from the run parameters (nthread, nstream, rad_utils_gpu.cu:
+           1025-1043).  emma_rt_get_level_tuning reads a level's variant,
+           chemistry budget and whether it has more halo faces per wave than
+           LISTED holds (any pointer may be NULL). */
+#define EMMA_RT_VARIANT_AUTO (-1)
+#define EMMA_RT_VARIANT_LISTED 0
+#define EMMA_RT_VARIANT_PERDIR 1
+#define EMMA_RT_VARIANT_LATENCY 2
+int emma_rt_set_level_variant(emma_rt_ctx *ctx, int level, int variant);
+int emma_rt_get_level_tuning(emma_rt_ctx *ctx, int level, int *variant, int *chem_budget, int *halo_per_dir);
 
 int emma_rt_get_stats(emma_rt_ctx *ctx, emma_rt_stats *out);
 /* device timing into emma_rt_stats.kernel_ms: EMMA_RT_TIMING_LAUNCH puts
