@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "emma_rt.h"
@@ -111,6 +112,7 @@ struct emma_rt_ctx {
   int* d_status = nullptr;
   unsigned long long* d_counters = nullptr;
   unsigned long long h_counters[kCounterSlots * kCounterStride];  // read back by end_call
+  void* scratch = nullptr;  // this device's no-alloc staging area (emma_rt_advance_stencil_noalloc)
   double* d_photons = nullptr;
   // straggler list of the tiled mixed kernel (capacity: the largest level's
   // cells) and one list-length word per level-step launch of a call (slot
@@ -299,6 +301,31 @@ extern "C" const char* emma_rt_strerror(int s) {
 }
 
 // ---------------------------------------------------------------- context
+// ---------------------------------------------------------------- no-alloc staging
+// The staging area of emma_rt_advance_stencil_noalloc: a file-scope device
+// array (each GPU the library runs on holds its own copy until the process
+// ends) instead of the buffer emma_rt_create_radstencil allocates.  Host code
+// never passes the array's name to a kernel or copy: it uses the device
+// address hipGetSymbolAddress returns, looked up once per device with that
+// device current (emma_rt_create) and kept per device.
+static __device__ __align__(256) char scratch[EMMA_RT_NOALLOC_STENCIL_BYTES];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+
+static int scratch_of_current_device(int dev, void** out) {
+  if (dev < 0 || dev >= kMaxDevices) return EMMA_RT_ERR_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (ptr == nullptr) return EMMA_RT_ERR_HIP;
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return EMMA_RT_OK;
+}
+
 extern "C" int emma_rt_create(const emma_rt_params* p, emma_rt_ctx** out) {
   if (out == nullptr) return EMMA_RT_ERR_ARG;
   *out = nullptr;
@@ -323,6 +350,11 @@ extern "C" int emma_rt_create(const emma_rt_params* p, emma_rt_ctx** out) {
       hipStreamSynchronize(c->stream) != hipSuccess) {
     emma_rt_destroy(c);
     return EMMA_RT_ERR_HIP;
+  }
+  st = scratch_of_current_device(c->device, &c->scratch);
+  if (st) {
+    emma_rt_destroy(c);
+    return st;
   }
   *out = c;
   return EMMA_RT_OK;
@@ -1496,3 +1528,12 @@ extern "C" int emma_rt_advance_stencil(emma_rt_ctx* c, void* stencil, int nread,
   return advance_stencil_on(c, c->d_stencil, c->stencil_rbytes, stencil, nread, level, dxcur, dtnew, aexp, chemonly);
 }
 
+extern "C" int emma_rt_advance_stencil_noalloc(emma_rt_ctx* c, void* stencil, int nread, int level, double dxcur,
+                                               double dtnew, double aexp, int chemonly, int real_bytes) {
+  if (c == nullptr || stencil == nullptr || nread < 0 || (real_bytes != 4 && real_bytes != 8)) return EMMA_RT_ERR_ARG;
+  RgridLayout L;
+  if (!rgrid_layout(c->p.ngrp, real_bytes, &L)) return EMMA_RT_ERR_ARG;
+  if ((size_t)nread * (size_t)L.v[19] > sizeof(scratch)) return EMMA_RT_ERR_ARG;  // beyond the staging area
+  if (c->scratch == nullptr) return EMMA_RT_ERR_STATE;
+  return advance_stencil_on(c, c->scratch, real_bytes, stencil, nread, level, dxcur, dtnew, aexp, chemonly);
+}

@@ -69,6 +69,7 @@ EXPORTS = [
     "emma_rt_set_stromgren_source", "emma_rt_get_level_times", "emma_rt_alloc_pinned",
     "emma_rt_save_state", "emma_rt_restore_state", "emma_rt_set_stars", "emma_rt_set_star_sources",
     "emma_rt_set_chem_budget", "emma_rt_get_chem_budget", "emma_rt_set_level_variant", "emma_rt_get_level_tuning",
+    "emma_rt_advance_stencil_noalloc",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -154,6 +155,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.emma_rt_set_chem_budget.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     L.emma_rt_get_chem_budget.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
     L.emma_rt_set_level_variant.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.emma_rt_advance_stencil_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                                  ctypes.c_int]
     L.emma_rt_get_level_tuning.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _ip, _ip]
     _lib = L
     return L

@@ -155,6 +155,21 @@ int emma_rt_alloc_pinned(size_t bytes, void **out);
 int emma_rt_advance_stencil(emma_rt_ctx *ctx, void *stencil, int nread, int level, double dxcur, double dtnew,
                             double aexp, int chemonly);
 
+/* emma_rt_advance_stencil without the device buffer of
+   emma_rt_create_radstencil: for callers that cannot allocate device memory
+   (code under HIP graph capture, serving loops, harnesses that forbid
+   hipMalloc).  No counterpart in the reference.  The batch is staged in a
+   static device area of EMMA_RT_NOALLOC_STENCIL_BYTES per GPU (allocated with
+   the library's code object, held until the process ends); the call allocates
+   nothing.  nread * sizeof(struct RGRID) beyond that size (e.g. 4128 records
+   at NGRP=1, 1826 at NGRP=3, REAL=double) returns EMMA_RT_ERR_ARG.  real_bytes
+   is sizeof(REAL).  Results equal emma_rt_advance_stencil's.  All calls on one
+   GPU share its area: calls that overlap in time on one GPU (other host
+   threads, other contexts) race and must be serialised by the caller. */
+#define EMMA_RT_NOALLOC_STENCIL_BYTES (32u << 20)
+int emma_rt_advance_stencil_noalloc(emma_rt_ctx *ctx, void *stencil, int nread, int level, double dxcur,
+                                    double dtnew, double aexp, int chemonly, int real_bytes);
+
 /* ---- resident-level path ---- */
 /* nei_oct entries of faces on a non-periodic domain face, the WRADTEST
    TRANSXM..TRANSZP and REFXM/REFYM/REFZM build flags of the reference
