@@ -206,7 +206,8 @@ def _configure(g, W, precision, tuning):
     for l in range(W.tree.lmin, W.tree.lmax + 1):
         if precision == "mixed" and W.params.ngrp in (1, 3):
             t = g.level_tuning(l)
-            out[l] = {"variant": rt.VARIANT_NAMES[t["variant"]], "chem_budget": t["chem_budget"]}
+            out[l] = {"variant": rt.VARIANT_NAMES[t["variant"]], "chem_budget": t["chem_budget"],
+                      "chem_keep": t["chem_keep"]}
     return out
 
 

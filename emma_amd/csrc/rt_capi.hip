@@ -47,6 +47,7 @@ struct DevLevel {
   int halo_per_dir = 0;  // some wave has more than 64 halo faces (no kListed variant)
   int variant = kListed;  // halo variant of the tiled kernel
   int chem_budget = 0;   // chemistry tries before a leaf goes to the straggler pass (0: none)
+  float chem_keep = 0.f;  // long chains stay (see emma_rt_set_chem_keep)
   void* e[2] = {nullptr, nullptr};
   void* fx[2] = {nullptr, nullptr};
   void* fy[2] = {nullptr, nullptr};
@@ -705,6 +706,7 @@ static LevelArgs<R> level_args(emma_rt_ctx* c, int k, double dtnew, double aexp,
   A.variant = L.variant;
   if (c->d_defer != nullptr && L.chem_budget > 0 && c->defer_seq < c->defer_slots) {
     A.chem_budget = L.chem_budget;
+    A.chem_keep = L.chem_keep;
     A.defer_cap = c->defer_cap;
     A.defer_list = c->d_defer;
     A.defer_ctl = c->d_defer_ctl + c->defer_seq++;
@@ -1301,12 +1303,24 @@ extern "C" int emma_rt_set_level_variant(emma_rt_ctx* c, int level, int variant)
   return EMMA_RT_OK;
 }
 
-extern "C" int emma_rt_get_level_tuning(emma_rt_ctx* c, int level, int* variant, int* budget, int* halo_per_dir) {
+extern "C" int emma_rt_get_level_tuning(emma_rt_ctx* c, int level, int* variant, int* budget, int* halo_per_dir,
+                                        double* keep) {
   DevLevel* L = level_of(c, level);
   if (L == nullptr) return EMMA_RT_ERR_ARG;
   if (variant) *variant = L->variant;
   if (budget) *budget = L->chem_budget;
+  if (keep) *keep = L->chem_keep;
   if (halo_per_dir) *halo_per_dir = L->halo_per_dir;
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_set_chem_keep(emma_rt_ctx* c, int level, double keep) {
+  if (c == nullptr || !c->has_tree || !(keep >= 0.) || keep > 1e30) return EMMA_RT_ERR_ARG;
+  if (level != -1 && level_of(c, level) == nullptr) return EMMA_RT_ERR_ARG;
+  if (c->d_defer == nullptr && keep > 0.) return EMMA_RT_ERR_ARG;
+  for (auto& L : c->lev)
+    if (level == -1 || L.level == level) L.chem_keep = (float)keep;
+  drop_graphs(c);
   return EMMA_RT_OK;
 }
 

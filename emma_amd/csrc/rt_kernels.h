@@ -29,6 +29,7 @@ struct LevelArgs {
   // and leaves its chemistry to chem_straggler_kernel, which restarts it from
   // the same inputs.  defer_ctl: this launch's list length (zero before it).
   int chem_budget, defer_cap;
+  float chem_keep;  // > 0: a leaf with more than this many sub-steps to go at its pace so far is not deferred
   int* defer_list;
   int* defer_ctl;
   // current state (read)

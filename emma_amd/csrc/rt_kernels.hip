@@ -82,6 +82,34 @@ __device__ __forceinline__ void gst(T* base, int i, T v) {
   *(EMMA_GLOBAL T*)(uniform_base(base) + byte_offset<T>(i)) = v;
 }
 
+// the same with the non-temporal hint (streamed: the line is not kept in L2
+// for reuse).  For a large level's next state and its once-read chemistry
+// inputs, which would otherwise push out of L2 the neighbouring tiles' cells
+// that halo reads are about to fetch; small levels, whose next step finds
+// their state in L2, keep the plain form.
+template <typename T>
+__device__ __forceinline__ void gst_nt(T* base, int i, T v) {
+  __builtin_nontemporal_store(v, (EMMA_GLOBAL T*)(uniform_base(base) + byte_offset<T>(i)));
+}
+template <typename T>
+__device__ __forceinline__ T gld_nt(const T* base, int i) {
+  return __builtin_nontemporal_load((const EMMA_GLOBAL T*)(uniform_base(base) + byte_offset<T>(i)));
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void gst_out(T* base, int i, T v) {
+  if constexpr (NT)
+    gst_nt(base, i, v);
+  else
+    gst(base, i, v);
+}
+template <bool NT, typename T>
+__device__ __forceinline__ T gld_in(const T* base, int i) {
+  if constexpr (NT)
+    return gld_nt(base, i);
+  else
+    return gld(base, i);
+}
+
 template <typename T>
 __device__ __forceinline__ T sel(int i, T a, T b, T c) {
   return i == 0 ? a : (i == 1 ? b : c);
@@ -184,7 +212,7 @@ __device__ __forceinline__ void reflux(const LevelArgs<R>& a, int p, int q, int 
 // accepted chemistry sub-steps.
 // The chemistry inputs (src, x n_H, eint, n_H) come from memory, or from
 // `pre` when the caller loaded them early (pre_src[GA], pre[3]).
-template <typename R, int G, bool Fast, int Sub>
+template <typename R, int G, bool Fast, int Sub, bool NT = false>
 __device__ __forceinline__ int finish_leaf(const LevelArgs<R>& a, const ChemConst& K, int cell, int ch, int ng,
                                            R* Ue, R* Ux, R* Uy, R* Uz, const R* De, const R* Dx, const R* Dy,
                                            const R* Dz, const R* pre_src = nullptr, const double* pre = nullptr) {
@@ -262,7 +290,7 @@ __device__ __forceinline__ int finish_leaf(const LevelArgs<R>& a, const ChemCons
   if constexpr (Fast) {
     double fscale[GA];
     st = chem_cell_fast<G>(K, ng, e, fscale, src, nhplus, eint, nh, temp, a.chemonly, nit,
-                           Sub == kRefluxTiled && a.chem_budget > 0 ? a.chem_budget : -1);
+                           Sub == kRefluxTiled && a.chem_budget > 0 ? a.chem_budget : -1, (double)a.chem_keep);
     if constexpr (Sub == kRefluxTiled) {
       // budget spent: the transported N, F are stored unscaled (fscale = 1)
       // and the cell goes on the straggler list, one atomic per wave
@@ -323,15 +351,15 @@ __device__ __forceinline__ int finish_leaf(const LevelArgs<R>& a, const ChemCons
       // floor test (chem_utils.c:382); only the rounding is corrected
       if constexpr (sizeof(R) == 4)
         if (!deferred && e[g] >= K.emin_grp[g] && ev < K.emin_grp_up[g]) ev = K.emin_grp_up[g];
-      gst(a.eo + g * n, cell, ev);
-      gst(a.fxo + g * n, cell, fx[g]);
-      gst(a.fyo + g * n, cell, fy[g]);
-      gst(a.fzo + g * n, cell, fz[g]);
+      gst_out<NT>(a.eo + g * n, cell, ev);
+      gst_out<NT>(a.fxo + g * n, cell, fx[g]);
+      gst_out<NT>(a.fyo + g * n, cell, fy[g]);
+      gst_out<NT>(a.fzo + g * n, cell, fz[g]);
     }
   }
-  gst(a.nhpluso, cell, nhplus);
-  gst(a.einto, cell, eint);
-  gst(a.tempo, cell, temp);
+  gst_out<NT>(a.nhpluso, cell, nhplus);
+  gst_out<NT>(a.einto, cell, eint);
+  gst_out<NT>(a.tempo, cell, temp);
   return nit;
 }
 
@@ -564,6 +592,7 @@ template <int G, int V>
 __device__ __forceinline__ void tiled_step(const LevelArgs<float>& a, const ChemConst& K) {
   constexpr bool PerDir = V != kListed;
   constexpr bool Late = V == kPerDir;  // chemistry inputs loaded when the chemistry starts
+  constexpr bool NT = V == kListed;    // non-temporal next state and chemistry inputs (large levels)
   constexpr int kHaloCap = 64;                 // halo slots per wave
   constexpr int kList = PerDir ? 1 : kHaloCap;  // halo list entries per wave
   // shares: [g][A or B][cell] of the current direction, then the halo slots
@@ -607,10 +636,10 @@ __device__ __forceinline__ void tiled_step(const LevelArgs<float>& a, const Chem
     // loads need the registers, when the chemistry starts (finish_leaf)
     if constexpr (!Late) {
 #pragma unroll
-      for (int g = 0; g < G; g++) Src[g] = gld(a.src + g * n, cell);
-      pre[0] = gld(a.nhplus, cell);
-      pre[1] = gld(a.eint, cell);
-      pre[2] = gld(a.nh, cell);
+      for (int g = 0; g < G; g++) Src[g] = gld_in<NT>(a.src + g * n, cell);
+      pre[0] = gld_in<NT>(a.nhplus, cell);
+      pre[1] = gld_in<NT>(a.eint, cell);
+      pre[2] = gld_in<NT>(a.nh, cell);
     }
   }
   sCancel[lc] = (unsigned char)(ch >= 0 && !a.keep_split_flux);
@@ -901,7 +930,7 @@ __device__ __forceinline__ void tiled_step(const LevelArgs<float>& a, const Chem
       transport_split_cell<float, G>(a, cell, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz);
     } else {
       leaf = 1;
-      nit_acc = (unsigned)finish_leaf<float, G, true, kRefluxTiled>(a, K, cell, ch, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz,
+      nit_acc = (unsigned)finish_leaf<float, G, true, kRefluxTiled, NT>(a, K, cell, ch, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz,
                                                                     Late ? nullptr : Src, Late ? nullptr : pre);
     }
   }

@@ -135,9 +135,12 @@ __device__ __forceinline__ int chem_fast_begin(const ChemConst& K, int ng, const
 
 // the sub-step loop until the cell's dt is covered (0), for at most `budget`
 // more tries when budget >= 0 (1: the cell still runs, its state in s), or
-// until the controller fails (-5)
+// until the controller fails (-5).  keep > 0: a cell that, when its budget
+// runs out, has more than `keep` sub-steps to go at the pace of its accepted
+// ones so far, or none accepted yet, runs on instead of stopping
 template <int G>
-__device__ __forceinline__ int chem_fast_run(const ChemConst& K, int ng, int chemonly, ChemFast<G>& s, int budget) {
+__device__ __forceinline__ int chem_fast_run(const ChemConst& K, int ng, int chemonly, ChemFast<G>& s, int budget,
+                                             double keep = 0.) {
 #pragma clang fp contract(fast)
   constexpr int GA = ChemFast<G>::GA;
   double et[GA], rp[GA];
@@ -156,8 +159,12 @@ __device__ __forceinline__ int chem_fast_run(const ChemConst& K, int ng, int che
   int rc = 0;
   while (tcur < dt) {
     if (budget >= 0 && budget-- == 0) {
-      rc = 1;
-      break;
+      if (keep > 0. && (nit == s.nit || (dt - tcur) * (double)(nit - s.nit) > keep * (tcur - s.tcur))) {
+        budget = -1;  // a long chain: no limit
+      } else {
+        rc = 1;
+        break;
+      }
     }
     if (++tries > kMaxTries) {
       rc = -5;
@@ -286,12 +293,13 @@ __device__ __forceinline__ void chem_fast_end(const ChemConst& K, int ng, int ch
   temp = s.eint * kT * frcp64((s.nh * K.unit_N / a3f) * (1. + s.x0));
 }
 
-// budget >= 0: at most `budget` sub-step tries; returns 1 with e, fscale,
-// nhplus, eint_code, temp untouched (fscale = 1) when the cell still runs
+// budget >= 0: at most `budget` sub-step tries (see chem_fast_run for
+// keep); returns 1 with e, fscale, nhplus, eint_code, temp untouched
+// (fscale = 1) when the cell still runs
 template <int G>
 __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fscale, const double* src,
                               double& nhplus, double& eint_code, double nh, double& temp, int chemonly, int& nit,
-                              int budget = -1) {
+                              int budget = -1, double keep = 0.) {
   constexpr int GA = ChemFast<G>::GA;
 #pragma unroll
   for (int g = 0; g < GA; g++)
@@ -300,7 +308,7 @@ __device__ int chem_cell_fast(const ChemConst& K, int ng, double* e, double* fsc
   int st = chem_fast_begin<G>(K, ng, e, src, nhplus, eint_code, nh, s);
   nit = 0;
   if (st != 0) return st;
-  st = chem_fast_run<G>(K, ng, chemonly, s, budget);
+  st = chem_fast_run<G>(K, ng, chemonly, s, budget, keep);
   nit = s.nit;
   if (st != 0) return st;
   chem_fast_end<G>(K, ng, chemonly, s, e, fscale, nhplus, eint_code, temp);

@@ -69,7 +69,7 @@ EXPORTS = [
     "emma_rt_set_stromgren_source", "emma_rt_get_level_times", "emma_rt_alloc_pinned",
     "emma_rt_save_state", "emma_rt_restore_state", "emma_rt_set_stars", "emma_rt_set_star_sources",
     "emma_rt_set_chem_budget", "emma_rt_get_chem_budget", "emma_rt_set_level_variant", "emma_rt_get_level_tuning",
-    "emma_rt_advance_stencil_noalloc",
+    "emma_rt_advance_stencil_noalloc", "emma_rt_set_chem_keep",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -158,7 +158,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.emma_rt_advance_stencil_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                                   ctypes.c_int]
-    L.emma_rt_get_level_tuning.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _ip, _ip]
+    L.emma_rt_get_level_tuning.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _ip, _ip, _dp]
+    L.emma_rt_set_chem_keep.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
     _lib = L
     return L
 
@@ -311,10 +312,15 @@ class RadGPU:
         self._check(self.lib.emma_rt_set_level_variant(self.h, int(level), int(variant)))
 
     def level_tuning(self, level: int) -> dict:
-        v, b, h = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        v, b, h, k = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
         self._check(self.lib.emma_rt_get_level_tuning(self.h, int(level), ctypes.byref(v), ctypes.byref(b),
-                                                      ctypes.byref(h)))
-        return {"variant": v.value, "chem_budget": b.value, "halo_per_dir": bool(h.value)}
+                                                      ctypes.byref(h), ctypes.byref(k)))
+        return {"variant": v.value, "chem_budget": b.value, "halo_per_dir": bool(h.value), "chem_keep": k.value}
+
+    def set_chem_keep(self, keep: float, level: int = -1):
+        """with a chemistry budget: leaves with more than `keep` sub-steps to
+        go at their pace so far stay in the first pass (0: defer all)"""
+        self._check(self.lib.emma_rt_set_chem_keep(self.h, int(level), float(keep)))
 
     def set_mode(self, mode: int):
         """RADSTEP (per-level transport) or COARSERAD (coarse-level transport)"""

@@ -8,7 +8,9 @@ per-level choices are
 * the halo variant of the tiled kernel (``VARIANT_LISTED`` / ``_PERDIR`` /
   ``_LATENCY``, see ``emma_rt_set_level_variant``), and
 * the chemistry sub-step budget after which a leaf goes to the straggler
-  pass (``emma_rt_set_chem_budget``; 0 = no pass),
+  pass (``emma_rt_set_chem_budget``; 0 = no pass), and the factor beyond
+  which a leaf's chain counts as long and stays in the first pass
+  (``emma_rt_set_chem_keep``),
 
 which change the speed and not the results.  :func:`tune` times every
 candidate on the workload's own state (event-timed level steps of repeated
@@ -26,6 +28,7 @@ from . import rt
 
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 BUDGETS = (0, 2, 3, 4, 6, 8)
+KEEPS = (0.0, 8.0, 32.0)
 
 
 def level_key(ngrp: int, ncell: int, halo_per_dir: bool) -> str:
@@ -48,43 +51,48 @@ def _level_times(g, levels, reps):
     return {l: a[0] / max(a[1], 1) * 1e3 for l, a in acc.items()}
 
 
-def tune(g: "rt.RadGPU", reps: int = 3, budgets=BUDGETS, log=None) -> dict:
-    """Per level of the context's tree, the (variant, budget) with the
+def tune(g: "rt.RadGPU", reps: int = 3, budgets=BUDGETS, keeps=KEEPS, log=None) -> dict:
+    """Per level of the context's tree, the (variant, budget, keep) with the
     smallest event-timed level step.  The context must hold a mixed-precision
     tree with 1 or 3 groups and a saved state (``g.save_state()``) to start
     every timed coarse step from.  Returns {level: {variant, chem_budget,
-    us, default_us, candidates}}."""
+    chem_keep, us, default_us, candidates}}."""
     tree = g.tree
     levels = [l for l in range(tree.lmin, tree.lmax + 1) if tree.level(l).noct > 0]
     shape = {l: g.level_tuning(l) for l in levels}
-    default = {l: (shape[l]["variant"], shape[l]["chem_budget"]) for l in levels}
+    default = {l: (shape[l]["variant"], shape[l]["chem_budget"], shape[l]["chem_keep"]) for l in levels}
     t_default = _level_times(g, levels, reps)
     cands = {l: {} for l in levels}
     variants = [rt.VARIANT_LISTED, rt.VARIANT_PERDIR, rt.VARIANT_LATENCY]
-    for v in variants:
-        for b in budgets:
-            for l in levels:  # a variant a level cannot take keeps its default
-                ok = not (v == rt.VARIANT_LISTED and shape[l]["halo_per_dir"])
-                g.set_level_variant(v if ok else default[l][0], level=l)
-                g.set_chem_budget(b, level=l)
-            t = _level_times(g, levels, reps)
-            for l in levels:
-                if v == rt.VARIANT_LISTED and shape[l]["halo_per_dir"]:
-                    continue
-                cands[l][(v, b)] = t[l]
-            if log:
-                log(f"tune: variant {rt.VARIANT_NAMES[v]} budget {b}: " +
-                    ", ".join(f"{l}: {t[l]:.1f} us" for l in levels))
+    combos = [(v, b, k) for v in variants for b in budgets for k in (keeps if b > 0 else (0.0,))]
+    for v, b, k in combos:
+        for l in levels:  # a variant a level cannot take keeps its default
+            ok = not (v == rt.VARIANT_LISTED and shape[l]["halo_per_dir"])
+            g.set_level_variant(v if ok else default[l][0], level=l)
+            g.set_chem_budget(b, level=l)
+            g.set_chem_keep(k, level=l)
+        t = _level_times(g, levels, reps)
+        for l in levels:
+            if v == rt.VARIANT_LISTED and shape[l]["halo_per_dir"]:
+                continue
+            cands[l][(v, b, k)] = t[l]
+        if log:
+            log(f"tune: variant {rt.VARIANT_NAMES[v]} budget {b} keep {k:g}: " +
+                ", ".join(f"{l}: {t[l]:.1f} us" for l in levels))
     out = {}
     for l in levels:
-        (v, b), us = min(cands[l].items(), key=lambda kv: kv[1])
+        (v, b, k), us = min(cands[l].items(), key=lambda kv: kv[1])
         g.set_level_variant(v, level=l)
         g.set_chem_budget(b, level=l)
+        g.set_chem_keep(k, level=l)
         L = tree.level(l)
         out[l] = {"key": level_key(g.params.ngrp, L.ncell, shape[l]["halo_per_dir"]),
-                  "variant": rt.VARIANT_NAMES[v], "chem_budget": b, "us": us, "default_us": t_default[l],
-                  "default": {"variant": rt.VARIANT_NAMES[default[l][0]], "chem_budget": default[l][1]},
-                  "candidates": {f"{rt.VARIANT_NAMES[cv]}/{cb}": round(ct, 2) for (cv, cb), ct in cands[l].items()}}
+                  "variant": rt.VARIANT_NAMES[v], "chem_budget": b, "chem_keep": k, "us": us,
+                  "default_us": t_default[l],
+                  "default": {"variant": rt.VARIANT_NAMES[default[l][0]], "chem_budget": default[l][1],
+                              "chem_keep": default[l][2]},
+                  "candidates": {f"{rt.VARIANT_NAMES[cv]}/{cb}/{ck:g}": round(ct, 2)
+                                 for (cv, cb, ck), ct in cands[l].items()}}
     return out
 
 
@@ -100,7 +108,8 @@ def save(result: dict, workload: str, path: str = TABLE) -> dict:
     shape; `workload` names where they were measured)"""
     table = load(path)
     for l, r in result.items():
-        table[r["key"]] = {"variant": r["variant"], "chem_budget": r["chem_budget"], "us": round(r["us"], 2),
+        table[r["key"]] = {"variant": r["variant"], "chem_budget": r["chem_budget"],
+                           "chem_keep": r.get("chem_keep", 0.0), "us": round(r["us"], 2),
                            "default_us": round(r["default_us"], 2), "workload": workload, "level": int(l)}
     with open(path, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
@@ -126,5 +135,6 @@ def apply(g: "rt.RadGPU", table: dict = None) -> dict:
             continue
         g.set_level_variant(names[e["variant"]], level=l)
         g.set_chem_budget(int(e["chem_budget"]), level=l)
+        g.set_chem_keep(float(e.get("chem_keep", 0.0)), level=l)
         used[l] = e
     return used

@@ -315,6 +315,14 @@ int emma_rt_restore_state(emma_rt_ctx *ctx);
    paths. */
 int emma_rt_set_chem_budget(emma_rt_ctx *ctx, int level, int budget);
 int emma_rt_get_chem_budget(emma_rt_ctx *ctx, int level, int *budget);
+/* With a budget set, a leaf whose chain looks long when the budget runs out
+   can stay in the first pass instead: no sub-step accepted yet, or more than
+   `keep` sub-steps to go at the pace of its accepted ones (its remaining time
+   over their mean length), so that long chains overlap the rest of the level
+   step.  keep 0 (the default) defers every leaf that is still sub-stepping;
+   on the benchmark workloads that is the faster choice (a long chain kept in
+   the first pass holds its whole work-group's registers and LDS). */
+int emma_rt_set_chem_keep(emma_rt_ctx *ctx, int level, double keep);
 /* Halo variant of the mixed-precision level step (1 or 3 groups) for one
    level or every level (-1).  The results are the same; the variants differ
    in how a tile reaches its neighbours outside the tile (the halo):
@@ -329,14 +337,16 @@ int emma_rt_get_chem_budget(emma_rt_ctx *ctx, int level, int *budget);
            otherwise).  The reference's analogue is the CUDA launch shape it
            takes from the run parameters (nthread, nstream, rad_utils_gpu.cu:
            1025-1043).  emma_rt_get_level_tuning reads a level's variant,
-           chemistry budget and whether it has more halo faces per wave than
-           LISTED holds (any pointer may be NULL). */
+           chemistry budget, whether it has more halo faces per wave than
+           LISTED holds, and the chemistry keep factor (any pointer may be
+           NULL). */
 #define EMMA_RT_VARIANT_AUTO (-1)
 #define EMMA_RT_VARIANT_LISTED 0
 #define EMMA_RT_VARIANT_PERDIR 1
 #define EMMA_RT_VARIANT_LATENCY 2
 int emma_rt_set_level_variant(emma_rt_ctx *ctx, int level, int variant);
-int emma_rt_get_level_tuning(emma_rt_ctx *ctx, int level, int *variant, int *chem_budget, int *halo_per_dir);
+int emma_rt_get_level_tuning(emma_rt_ctx *ctx, int level, int *variant, int *chem_budget, int *halo_per_dir,
+                             double *chem_keep);
 
 int emma_rt_get_stats(emma_rt_ctx *ctx, emma_rt_stats *out);
 /* device timing into emma_rt_stats.kernel_ms: EMMA_RT_TIMING_LAUNCH puts

@@ -17,11 +17,12 @@ from test_gpu_parity import MIXED_TOL, field_errors
 pytestmark = pytest.mark.gpu
 
 
-def _run(W, budget, steps, graph=True):
+def _run(W, budget, steps, graph=True, keep=0.0):
     g = rt.RadGPU(W.params, rt.MIXED)
     g.setup(W.params, W.tree, W.fields)
     if budget:
         g.set_chem_budget(budget)
+        g.set_chem_keep(keep)
     sub = 0
     for _ in range(steps):
         if graph:
@@ -87,6 +88,12 @@ def test_budget_full_level(gpu_lib):
     st = g1.stats()
     assert st["status"] == 0
     assert st["deferred_cells"] > 0 and g0.stats()["deferred_cells"] == 0, (st, g0.stats())
+    # long chains kept in the first pass: fewer leaves deferred, same results
+    g2, s2 = _run(W, 2, 4, keep=8.0)
+    assert s2 == s0, (s0, s2)
+    assert 0 < g2.stats()["deferred_cells"] < st["deferred_cells"], (g2.stats(), st)
+    assert g2.level_tuning(7)["chem_keep"] == 8.0
+    _same(g2.get(7), g0.get(7), W.params.cloc())
 
 
 def test_budget_args(gpu_lib):
@@ -95,6 +102,8 @@ def test_budget_args(gpu_lib):
     g.setup(W.params, W.tree, W.fields)
     with pytest.raises(rt.EmmaRTError):
         g.set_chem_budget(2)  # the FP64 path has no straggler pass
+    with pytest.raises(rt.EmmaRTError):
+        g.set_chem_keep(8.0)
     g = rt.RadGPU(W.params, rt.MIXED)
     g.setup(W.params, W.tree, W.fields)
     with pytest.raises(rt.EmmaRTError):
@@ -103,3 +112,5 @@ def test_budget_args(gpu_lib):
         g.set_chem_budget(2, level=9)
     g.set_chem_budget(4, level=3)
     assert g.chem_budget(3) == 4
+    with pytest.raises(rt.EmmaRTError):
+        g.set_chem_keep(-1.0)
