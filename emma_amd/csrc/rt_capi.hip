@@ -6,10 +6,12 @@
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
+#include <sys/stat.h>
 
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "emma_rt.h"
@@ -1550,4 +1552,159 @@ extern "C" int emma_rt_advance_stencil_noalloc(emma_rt_ctx* c, void* stencil, in
   if ((size_t)nread * (size_t)L.v[19] > sizeof(scratch)) return EMMA_RT_ERR_ARG;  // beyond the staging area
   if (c->scratch == nullptr) return EMMA_RT_ERR_STATE;
   return advance_stencil_on(c, c->scratch, real_bytes, stencil, nread, level, dxcur, dtnew, aexp, chemonly);
+}
+
+// ---------------------------------------------------------------- grid output
+// The serial STDIO grid output of dumpalloct_serial (io.c:1233-1363) for the
+// radiation, chemistry and hydro fields the library holds: per field id one
+// file <folder>grid_<name>/<name>.<ndump %05d>.p<rank %05d> holding int n,
+// float tsim, float xmin, xmax, ymin, ymax, zmin, zmax, then n floats, one per
+// leaf cell, levels lcoarse..lmax, octs in level order, cells 0..7.  Field
+// values are rounded to float as assign_grid_field (io.c:32-200) returns them.
+namespace {
+// the field names of readOutputParam_grid (parameters.c:132-177), by id
+const char* const kGridFieldName[] = {
+    "x", "y", "z", "l", "cpu", "gdata_d", "density", "gdata.p", "res", "f0", "f1", "f2", "marked",
+    "field_d", "field_u", "field_v", "field_w", "field_p", "rfield_e0", "rfield_fx0", "rfield_fy0",
+    "rfield_fz0", "rfield_e1", "rfield_fx1", "rfield_fy1", "rfield_fz1", "rfield_snfb", "rfield_e2",
+    "rfield_fx2", "rfield_fy2", "rfield_fz2", "rfield_src", "xion", "field_dX", "field_dXHE", "field_dXXHE",
+    "field_xHE", "field_xxHE", "rfield_temp", "t_first_xion", "t_last_xion", "rho_first_xion"};
+constexpr int kGridFields = 42;
+
+struct HostLevel {
+  std::vector<double> e, fx, fy, fz, src, nhplus, nh, temp, tf, tl, hyd[6], opos;
+};
+}  // namespace
+
+// whether the library holds field `id` for this context (1), or not (0)
+static int grid_field_ok(emma_rt_ctx* c, int id) {
+  const int G = c->p.ngrp;
+  bool hydro = true, xion = c->xion != 0;
+  for (auto& L : c->lev)
+    if (L.noct > 0 && L.hyd[0] == nullptr) hydro = false;
+  if (id >= 0 && id <= 4) return 1;                       // position, level, cpu
+  if (id >= 13 && id <= 17) return hydro;                 // d u v w p
+  if (id >= 18 && id <= 25) return ((id - 18) / 4) < G;  // group 0, 1
+  if (id >= 27 && id <= 30) return G > 2;                 // group 2
+  if (id == 31 || id == 32 || id == 38) return 1;         // source sum, x, T
+  if (id == 33) return hydro;                             // dX
+  if (id == 39 || id == 40) return xion;                  // reionisation times
+  return 0;  // gravity, marked, snfb, helium, rho_first_xion: not held here
+}
+
+extern "C" int emma_rt_dump_grid(emma_rt_ctx* c, const char* folder, int ndump, int rank, double tsim, int nfield,
+                                 const int* field_ids) {
+  if (c == nullptr || folder == nullptr || nfield < 1 || field_ids == nullptr || ndump < 0 || rank < 0)
+    return EMMA_RT_ERR_ARG;
+  if (!c->has_tree) return EMMA_RT_ERR_STATE;
+  bool want[kGridFields] = {false};
+  for (int f = 0; f < nfield; f++) {
+    if (field_ids[f] < 0 || field_ids[f] >= kGridFields || !grid_field_ok(c, field_ids[f])) return EMMA_RT_ERR_ARG;
+    want[field_ids[f]] = true;
+  }
+  if (!opos_ready(c)) return EMMA_RT_ERR_STATE;  // positions and file bounds need the octs' corners
+  const int G = c->p.ngrp, nl = (int)c->lev.size();
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<HostLevel> H(nl);
+  for (int k = 0; k < nl; k++) {
+    DevLevel& L = c->lev[k];
+    if (L.noct == 0) continue;
+    HostLevel& h = H[k];
+    const size_t n = L.ncell, gn = (size_t)G * n;
+    for (auto* v : {&h.e, &h.fx, &h.fy, &h.fz, &h.src}) v->resize(gn);
+    for (auto* v : {&h.nhplus, &h.nh, &h.temp}) v->resize(n);
+    int st = emma_rt_get_level(c, L.level, h.e.data(), h.fx.data(), h.fy.data(), h.fz.data(), h.src.data(),
+                               h.nhplus.data(), nullptr, h.nh.data(), h.temp.data());
+    if (st) return st;
+    h.opos.resize((size_t)L.noct * 3);
+    HIPCHK(hipMemcpy(h.opos.data(), L.opos, h.opos.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (c->xion && L.t_first) {
+      h.tf.resize(n);
+      h.tl.resize(n);
+      HIPCHK(hipMemcpy(h.tf.data(), L.t_first, n * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(h.tl.data(), L.t_last, n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    for (int j = 0; j < 6; j++) {
+      if (L.hyd[j] == nullptr) continue;
+      h.hyd[j].resize(n);
+      HIPCHK(hipMemcpy(h.hyd[j].data(), L.hyd[j], n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+  }
+  // assign_grid_field: the value of field `id` in cell i of level k, as float
+  auto value = [&](int id, int k, size_t i) -> float {
+    const HostLevel& h = H[k];
+    const DevLevel& L = c->lev[k];
+    const size_t n = L.ncell, o = i >> 3, ic = i & 7;
+    const double dxc = pow(0.5, L.level);
+    switch (id) {
+      case 0: return (float)(h.opos[o * 3 + 0] + (double)(ic & 1) * dxc);
+      case 1: return (float)(h.opos[o * 3 + 1] + (double)((ic >> 1) & 1) * dxc);
+      case 2: return (float)(h.opos[o * 3 + 2] + (double)(ic >> 2) * dxc);
+      case 3: return (float)L.level;
+      case 4: return (float)rank;
+      case 13: case 14: case 15: case 16: case 17: return (float)h.hyd[id - 13][i];
+      case 31: {  // the reference sums into a float
+        float r = 0.f;
+        for (int g = 0; g < G; g++) r = (float)((double)r + h.src[g * n + i]);
+        return r;
+      }
+      case 32: return (float)(h.nhplus[i] / h.nh[i]);
+      case 33: return (float)h.hyd[5][i];
+      case 38: return (float)h.temp[i];
+      case 39: return (float)h.tf[i];
+      case 40: return (float)h.tl[i];
+      default: break;
+    }
+    const int g = id < 26 ? (id - 18) / 4 : 2, q = id < 26 ? (id - 18) % 4 : id - 27;
+    const std::vector<double>& a = q == 0 ? h.e : (q == 1 ? h.fx : (q == 2 ? h.fy : h.fz));
+    return (float)a[g * n + i];
+  };
+  // file bounds over the dumped cells' octs (oct corner, oct corner + oct size)
+  float xmin = 2.f, xmax = -1.f, ymin = 2.f, ymax = -1.f, zmin = 2.f, zmax = -1.f;
+  int ncell_out = 0;
+  for (int k = 0; k < nl; k++) {
+    const DevLevel& L = c->lev[k];
+    const double dx = pow(0.5, L.level - 1);
+    for (int o = 0; o < L.noct; o++) {
+      const double* p = &H[k].opos[(size_t)o * 3];
+      for (int ic = 0; ic < 8; ic++) {
+        if (!(L.hchild[(size_t)o * 8 + ic] < 0 || L.level == c->lmax)) continue;
+        ncell_out++;
+        // the reference compares in float against its float bounds
+        if (p[0] < xmin) xmin = (float)p[0];
+        if (p[1] < ymin) ymin = (float)p[1];
+        if (p[2] < zmin) zmin = (float)p[2];
+        if (p[0] + dx > xmax) xmax = (float)(p[0] + dx);
+        if (p[1] + dx > ymax) ymax = (float)(p[1] + dx);
+        if (p[2] + dx > zmax) zmax = (float)(p[2] + dx);
+      }
+    }
+  }
+  std::vector<float> buf;
+  for (int id = 0; id < kGridFields; id++) {
+    if (!want[id]) continue;
+    std::string dir = std::string(folder) + "grid_" + kGridFieldName[id] + "/";
+    (void)mkdir(dir.c_str(), 0755);
+    char name[64];
+    snprintf(name, sizeof(name), "%s.%05d.p%05d", kGridFieldName[id], ndump, rank);
+    FILE* fp = fopen((dir + name).c_str(), "wb");
+    if (fp == nullptr) return EMMA_RT_ERR_ARG;
+    // the header's time: the reference writes sizeof(float) bytes of its
+    // REAL tsim (io.c:1282), i.e. the first four bytes of the double
+    unsigned char tbytes[sizeof(float)];
+    memcpy(tbytes, &tsim, sizeof(tbytes));
+    const float bounds[6] = {xmin, xmax, ymin, ymax, zmin, zmax};
+    bool ok = fwrite(&ncell_out, sizeof(int), 1, fp) == 1 && fwrite(tbytes, sizeof(tbytes), 1, fp) == 1 &&
+              fwrite(bounds, sizeof(float), 6, fp) == 6;
+    for (int k = 0; k < nl && ok; k++) {
+      const DevLevel& L = c->lev[k];
+      buf.clear();
+      for (size_t i = 0; i < (size_t)L.ncell; i++)
+        if (L.hchild[i] < 0 || L.level == c->lmax) buf.push_back(value(id, k, i));
+      if (!buf.empty()) ok = fwrite(buf.data(), sizeof(float), buf.size(), fp) == buf.size();
+    }
+    if (fclose(fp) != 0) ok = false;
+    if (!ok) return EMMA_RT_ERR_ARG;
+  }
+  return EMMA_RT_OK;
 }

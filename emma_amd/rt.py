@@ -69,7 +69,7 @@ EXPORTS = [
     "emma_rt_set_stromgren_source", "emma_rt_get_level_times", "emma_rt_alloc_pinned",
     "emma_rt_save_state", "emma_rt_restore_state", "emma_rt_set_stars", "emma_rt_set_star_sources",
     "emma_rt_set_chem_budget", "emma_rt_get_chem_budget", "emma_rt_set_level_variant", "emma_rt_get_level_tuning",
-    "emma_rt_advance_stencil_noalloc", "emma_rt_set_chem_keep",
+    "emma_rt_advance_stencil_noalloc", "emma_rt_set_chem_keep", "emma_rt_dump_grid",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -160,6 +160,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
                                                   ctypes.c_int]
     L.emma_rt_get_level_tuning.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _ip, _ip, _dp]
     L.emma_rt_set_chem_keep.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+    L.emma_rt_dump_grid.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_int, _ip]
     _lib = L
     return L
 
@@ -321,6 +323,15 @@ class RadGPU:
         """with a chemistry budget: leaves with more than `keep` sub-steps to
         go at their pace so far stay in the first pass (0: defer all)"""
         self._check(self.lib.emma_rt_set_chem_keep(self.h, int(level), float(keep)))
+
+    def dump_grid(self, folder: str, ndump: int, field_ids, rank: int = 0, tsim: float = 0.0):
+        """dumpalloct_serial's per-field files for `field_ids` under `folder`
+        (see emma_amd.rtio for the format and the reader)"""
+        if not folder.endswith("/"):
+            folder += "/"
+        ids = np.ascontiguousarray(np.asarray(field_ids, dtype=np.int32))
+        self._check(self.lib.emma_rt_dump_grid(self.h, folder.encode(), int(ndump), int(rank), float(tsim),
+                                               int(ids.size), ids.ctypes.data_as(_ip)))
 
     def set_mode(self, mode: int):
         """RADSTEP (per-level transport) or COARSERAD (coarse-level transport)"""

@@ -348,6 +348,24 @@ int emma_rt_set_level_variant(emma_rt_ctx *ctx, int level, int variant);
 int emma_rt_get_level_tuning(emma_rt_ctx *ctx, int level, int *variant, int *chem_budget, int *halo_per_dir,
                              double *chem_keep);
 
+/* Grid output: the serial STDIO dump of dumpalloct_serial (io.c:1233-1363)
+   for the fields this library holds, from the device state.  For each
+   requested field id (the ids of assign_grid_field, io.c:32-200, and
+   readOutputParam_grid's names, parameters.c:132-177) one file
+   <folder>grid_<name>/<name>.<ndump %05d>.p<rank %05d> holding int n,
+   the first sizeof(float) bytes of the double tsim (as the reference writes
+   its REAL tsim), float xmin, xmax, ymin, ymax, zmin, zmax over the dumped
+   cells' octs, then n floats: every leaf cell (and every cell of level lmax),
+   levels lcoarse..lmax, octs in the level's order, cells 0..7.  Held ids:
+   0-2 cell corner x, y, z, 3 level, 4 cpu (= rank), 13-17 d, u, v, w, p
+   (with the hydro coupling), 18-25 and 27-30 E, Fx, Fy, Fz of groups 0-2
+   (groups < ngrp), 31 sum of the sources, 32 x_HII, 33 dX (hydro), 38 T,
+   39-40 reionisation times (with emma_rt_enable_xion_times); any other id
+   returns EMMA_RT_ERR_ARG.  Needs the octs' corners (emma_rt_set_oct_pos) on
+   every level.  folder ends with '/'; the grid_<name>/ folders are created. */
+int emma_rt_dump_grid(emma_rt_ctx *ctx, const char *folder, int ndump, int rank, double tsim, int nfield,
+                      const int *field_ids);
+
 int emma_rt_get_stats(emma_rt_ctx *ctx, emma_rt_stats *out);
 /* device timing into emma_rt_stats.kernel_ms: EMMA_RT_TIMING_LAUNCH puts
    HIP events around every level-step kernel (coarse steps then launch

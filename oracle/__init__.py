@@ -49,6 +49,9 @@ REF_SO_STARS = {3: os.path.join(HERE, "_ref", "libemma_ref_g3_stars.so"),
                 9: os.path.join(HERE, "_ref", "libemma_ref_g9_stars.so")}
 REF_SO_STARS_DEC = {9: os.path.join(HERE, "_ref", "libemma_ref_g9_starsdec.so")}
 
+# the serial grid output of the reference (io.c, dumpalloct_serial) driven by io_driver.c
+REF_SO_IO = {1: os.path.join(HERE, "_ref", "libemma_ref_io_g1.so"), 3: os.path.join(HERE, "_ref", "libemma_ref_io_g3.so")}
+
 RADSTEP = 0
 COARSERAD = 1
 
@@ -637,3 +640,44 @@ class Reference:
         self.stencil_sweeps(st, dx, dt, c)
         self.stencil_update(st, dx, dt, c)
         self.stencil_chem(st, dx, dt, aexp, chemonly)
+
+
+def ref_io_available(ngrp: int) -> bool:
+    return os.path.exists(REF_SO_IO.get(ngrp, ""))
+
+
+def ref_dump_grid(ngrp: int, folder: str, ndump: int, field_ids, tree, fields: dict, opos: dict, rank: int = 0,
+                  tsim: float = 0.0, hydro: dict = None, xion: dict = None) -> None:
+    """The reference's own dumpalloct_serial (io.c:1233-1363) on a struct OCT
+    tree built from `tree`, with the levels' Fields, oct corners {level:
+    (noct, 3)}, optional hydro {level: (d, u, v, w, p, dX)} and reionisation
+    times {level: (t_first, t_last)}: checker for emma_rt_dump_grid."""
+    lib = ctypes.CDLL(REF_SO_IO[ngrp])
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.ref_dump_grid.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, ctypes.POINTER(dp), ctypes.c_int, _ip]
+    levels = list(range(tree.lmin, tree.lmax + 1))
+    noct = np.array([tree.level(l).noct for l in levels], dtype=np.int32)
+    child = np.ascontiguousarray(np.concatenate([tree.level(l).child for l in levels]), dtype=np.int32)
+    pos = np.ascontiguousarray(np.concatenate([np.asarray(opos[l], dtype=np.float64).reshape(-1, 3)
+                                               for l in levels]))
+    keep = []  # arrays referenced by the pointer table
+    ptrs = (dp * (16 * len(levels)))()
+    for k, l in enumerate(levels):
+        F = fields[l]
+        cols = [F.e, F.fx, F.fy, F.fz, F.src, F.nhplus, F.nh, F.temp]
+        cols += list(xion[l]) if xion and l in xion else [None, None]
+        cols += list(hydro[l]) if hydro and l in hydro else [None] * 6
+        for j, a in enumerate(cols):
+            if a is None:
+                continue
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(a)
+            ptrs[k * 16 + j] = a.ctypes.data_as(dp)
+    ids = np.ascontiguousarray(np.asarray(field_ids, dtype=np.int32))
+    if not folder.endswith("/"):
+        folder += "/"
+    st = lib.ref_dump_grid(folder.encode(), int(ndump), int(rank), float(tsim), tree.lmin, tree.lmax, ngrp,
+                           _i(noct), _i(child), _d(pos), ptrs, int(ids.size), _i(ids))
+    if st != 0:
+        raise RuntimeError(f"ref_dump_grid: {st}")
