@@ -1592,20 +1592,10 @@ static int grid_field_ok(emma_rt_ctx* c, int id) {
   return 0;  // gravity, marked, snfb, helium, rho_first_xion: not held here
 }
 
-extern "C" int emma_rt_dump_grid(emma_rt_ctx* c, const char* folder, int ndump, int rank, double tsim, int nfield,
-                                 const int* field_ids) {
-  if (c == nullptr || folder == nullptr || nfield < 1 || field_ids == nullptr || ndump < 0 || rank < 0)
-    return EMMA_RT_ERR_ARG;
-  if (!c->has_tree) return EMMA_RT_ERR_STATE;
-  bool want[kGridFields] = {false};
-  for (int f = 0; f < nfield; f++) {
-    if (field_ids[f] < 0 || field_ids[f] >= kGridFields || !grid_field_ok(c, field_ids[f])) return EMMA_RT_ERR_ARG;
-    want[field_ids[f]] = true;
-  }
-  if (!opos_ready(c)) return EMMA_RT_ERR_STATE;  // positions and file bounds need the octs' corners
+// the levels' host copies of what the grid fields read
+static int grid_snapshot(emma_rt_ctx* c, std::vector<HostLevel>& H) {
   const int G = c->p.ngrp, nl = (int)c->lev.size();
   HIPCHK(hipStreamSynchronize(c->stream));
-  std::vector<HostLevel> H(nl);
   for (int k = 0; k < nl; k++) {
     DevLevel& L = c->lev[k];
     if (L.noct == 0) continue;
@@ -1630,12 +1620,17 @@ extern "C" int emma_rt_dump_grid(emma_rt_ctx* c, const char* folder, int ndump, 
       HIPCHK(hipMemcpy(h.hyd[j].data(), L.hyd[j], n * sizeof(double), hipMemcpyDeviceToHost));
     }
   }
-  // assign_grid_field: the value of field `id` in cell i of level k, as float
-  auto value = [&](int id, int k, size_t i) -> float {
-    const HostLevel& h = H[k];
-    const DevLevel& L = c->lev[k];
-    const size_t n = L.ncell, o = i >> 3, ic = i & 7;
-    const double dxc = pow(0.5, L.level);
+  return EMMA_RT_OK;
+}
+
+// assign_grid_field (io.c:32-200): field `id` of cell i of level k, as float
+static float grid_value(emma_rt_ctx* c, const std::vector<HostLevel>& H, int id, int k, size_t i, int rank) {
+  const int G = c->p.ngrp;
+  const HostLevel& h = H[k];
+  const DevLevel& L = c->lev[k];
+  const size_t n = L.ncell, o = i >> 3, ic = i & 7;
+  const double dxc = pow(0.5, L.level);
+  {
     switch (id) {
       case 0: return (float)(h.opos[o * 3 + 0] + (double)(ic & 1) * dxc);
       case 1: return (float)(h.opos[o * 3 + 1] + (double)((ic >> 1) & 1) * dxc);
@@ -1658,7 +1653,55 @@ extern "C" int emma_rt_dump_grid(emma_rt_ctx* c, const char* folder, int ndump, 
     const int g = id < 26 ? (id - 18) / 4 : 2, q = id < 26 ? (id - 18) % 4 : id - 27;
     const std::vector<double>& a = q == 0 ? h.e : (q == 1 ? h.fx : (q == 2 ? h.fy : h.fz));
     return (float)a[g * n + i];
-  };
+  }
+}
+
+// the dumped cells: leaves, and every cell of level lmax (io.c:1322)
+static bool grid_dumped(emma_rt_ctx* c, const DevLevel& L, size_t i) { return L.hchild[i] < 0 || L.level == c->lmax; }
+
+static int check_grid_ids(emma_rt_ctx* c, int nfield, const int* field_ids, bool* want) {
+  for (int f = 0; f < nfield; f++) {
+    if (field_ids[f] < 0 || field_ids[f] >= kGridFields || !grid_field_ok(c, field_ids[f])) return EMMA_RT_ERR_ARG;
+    if (want) want[field_ids[f]] = true;
+  }
+  if (!opos_ready(c)) return EMMA_RT_ERR_STATE;  // positions and file bounds need the octs' corners
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_grid_field(emma_rt_ctx* c, int field_id, int rank, float* out, long long capacity,
+                                  long long* n_out) {
+  if (c == nullptr || n_out == nullptr || capacity < 0 || rank < 0) return EMMA_RT_ERR_ARG;
+  if (!c->has_tree) return EMMA_RT_ERR_STATE;
+  int st = check_grid_ids(c, 1, &field_id, nullptr);
+  if (st) return st;
+  long long n = 0;
+  for (auto& L : c->lev)
+    for (size_t i = 0; i < (size_t)L.ncell; i++) n += grid_dumped(c, L, i);
+  *n_out = n;
+  if (out == nullptr) return EMMA_RT_OK;  // size query
+  if (capacity < n) return EMMA_RT_ERR_ARG;
+  std::vector<HostLevel> H(c->lev.size());
+  st = grid_snapshot(c, H);
+  if (st) return st;
+  long long j = 0;
+  for (int k = 0; k < (int)c->lev.size(); k++)
+    for (size_t i = 0; i < (size_t)c->lev[k].ncell; i++)
+      if (grid_dumped(c, c->lev[k], i)) out[j++] = grid_value(c, H, field_id, k, i, rank);
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_dump_grid(emma_rt_ctx* c, const char* folder, int ndump, int rank, double tsim, int nfield,
+                                 const int* field_ids) {
+  if (c == nullptr || folder == nullptr || nfield < 1 || field_ids == nullptr || ndump < 0 || rank < 0)
+    return EMMA_RT_ERR_ARG;
+  if (!c->has_tree) return EMMA_RT_ERR_STATE;
+  bool want[kGridFields] = {false};
+  int st = check_grid_ids(c, nfield, field_ids, want);
+  if (st) return st;
+  const int nl = (int)c->lev.size();
+  std::vector<HostLevel> H(nl);
+  st = grid_snapshot(c, H);
+  if (st) return st;
   // file bounds over the dumped cells' octs (oct corner, oct corner + oct size)
   float xmin = 2.f, xmax = -1.f, ymin = 2.f, ymax = -1.f, zmin = 2.f, zmax = -1.f;
   int ncell_out = 0;
@@ -1668,7 +1711,7 @@ extern "C" int emma_rt_dump_grid(emma_rt_ctx* c, const char* folder, int ndump, 
     for (int o = 0; o < L.noct; o++) {
       const double* p = &H[k].opos[(size_t)o * 3];
       for (int ic = 0; ic < 8; ic++) {
-        if (!(L.hchild[(size_t)o * 8 + ic] < 0 || L.level == c->lmax)) continue;
+        if (!grid_dumped(c, L, (size_t)o * 8 + ic)) continue;
         ncell_out++;
         // the reference compares in float against its float bounds
         if (p[0] < xmin) xmin = (float)p[0];
@@ -1700,7 +1743,7 @@ extern "C" int emma_rt_dump_grid(emma_rt_ctx* c, const char* folder, int ndump, 
       const DevLevel& L = c->lev[k];
       buf.clear();
       for (size_t i = 0; i < (size_t)L.ncell; i++)
-        if (L.hchild[i] < 0 || L.level == c->lmax) buf.push_back(value(id, k, i));
+        if (grid_dumped(c, L, i)) buf.push_back(grid_value(c, H, id, k, i, rank));
       if (!buf.empty()) ok = fwrite(buf.data(), sizeof(float), buf.size(), fp) == buf.size();
     }
     if (fclose(fp) != 0) ok = false;

@@ -69,7 +69,7 @@ EXPORTS = [
     "emma_rt_set_stromgren_source", "emma_rt_get_level_times", "emma_rt_alloc_pinned",
     "emma_rt_save_state", "emma_rt_restore_state", "emma_rt_set_stars", "emma_rt_set_star_sources",
     "emma_rt_set_chem_budget", "emma_rt_get_chem_budget", "emma_rt_set_level_variant", "emma_rt_get_level_tuning",
-    "emma_rt_advance_stencil_noalloc", "emma_rt_set_chem_keep", "emma_rt_dump_grid",
+    "emma_rt_advance_stencil_noalloc", "emma_rt_set_chem_keep", "emma_rt_dump_grid", "emma_rt_grid_field",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -162,6 +162,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.emma_rt_set_chem_keep.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
     L.emma_rt_dump_grid.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_int, _ip]
+    L.emma_rt_grid_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
+                                     ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)]
     _lib = L
     return L
 
@@ -332,6 +334,16 @@ class RadGPU:
         ids = np.ascontiguousarray(np.asarray(field_ids, dtype=np.int32))
         self._check(self.lib.emma_rt_dump_grid(self.h, folder.encode(), int(ndump), int(rank), float(tsim),
                                                int(ids.size), ids.ctypes.data_as(_ip)))
+
+    def grid_field(self, field_id: int, rank: int = 0) -> np.ndarray:
+        """one grid-output field's values over the dumped cells (float32)"""
+        n = ctypes.c_longlong(0)
+        self._check(self.lib.emma_rt_grid_field(self.h, int(field_id), int(rank), None, 0, ctypes.byref(n)))
+        out = np.empty(n.value, dtype=np.float32)
+        self._check(self.lib.emma_rt_grid_field(self.h, int(field_id), int(rank),
+                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), out.size,
+                                                ctypes.byref(n)))
+        return out
 
     def set_mode(self, mode: int):
         """RADSTEP (per-level transport) or COARSERAD (coarse-level transport)"""

@@ -365,6 +365,12 @@ int emma_rt_get_level_tuning(emma_rt_ctx *ctx, int level, int *variant, int *che
    every level.  folder ends with '/'; the grid_<name>/ folders are created. */
 int emma_rt_dump_grid(emma_rt_ctx *ctx, const char *folder, int ndump, int rank, double tsim, int nfield,
                       const int *field_ids);
+/* The values of one field in that order (the data of its file): *n_out =
+   the number of dumped cells; out == NULL asks for the count only, else out
+   holds capacity floats (EMMA_RT_ERR_ARG when fewer than the count).  For a
+   host writing its own files, e.g. dump_HDF5_grid's per-field dataset "data"
+   (io.c:1495-1611) with its own HDF5 library. */
+int emma_rt_grid_field(emma_rt_ctx *ctx, int field_id, int rank, float *out, long long capacity, long long *n_out);
 
 int emma_rt_get_stats(emma_rt_ctx *ctx, emma_rt_stats *out);
 /* device timing into emma_rt_stats.kernel_ms: EMMA_RT_TIMING_LAUNCH puts

@@ -49,6 +49,8 @@ def test_dump_matches_reference_writer(gpu_lib, tmp_path, precision):
     for fid in IDS:
         a, b = rtio.path(mine, fid, 4, 1), rtio.path(ref, fid, 4, 1)
         assert filecmp.cmp(a, b, shallow=False), (fid, rtio.FIELD_NAMES[fid])
+    for fid in (0, 4, 18, 31, 32, 39):  # the same values through emma_rt_grid_field
+        assert np.array_equal(g.grid_field(fid, rank=1), rtio.read(rtio.path(mine, fid, 4, 1))["data"]), fid
     r = rtio.read(rtio.path(mine, 32, 4, 1))
     assert r["n"] == sum(int(((W.tree.level(l).child < 0) | (l == W.tree.lmax)).sum()) for l in levels)
     assert np.all((r["data"] >= 0) & (r["data"] <= 1))
