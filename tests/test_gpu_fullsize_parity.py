@@ -1,8 +1,10 @@
 """Mixed precision against the FP64 oracle at full size (GPU).
 
 x_HII, T and N of every cell after two sub-cycles of the 128^3 level (C3,
-2.1 M cells) and one coarse step of the levels 7-11 tree (C4, 3.0 M cells)
-of the benchmark workloads, against the oracle on the same drawn state.
+2.1 M cells), one coarse step of the levels 7-11 tree (C4, 3.0 M cells) and
+one coarse step of the 256^3 + 2-level tree (C5, 19.8 M cells, with the
+benchmark's per-level tuning: the chemistry straggler pass on levels 8 and
+9) of the benchmark workloads, against the oracle on the same drawn state.
 
 The bar is 1e-4 relative.  A cell beyond it must be one where the FP64
 algorithm itself is that sensitive: chemrad's sub-step controller accepts or
@@ -14,10 +16,13 @@ from the same state with every input (N, F, src, eint, x n_H) perturbed by
 least half as much in one of those FP64 runs.  (On C4 three cells do; the
 mixed path moves them by the amounts a 1e-7 perturbation does.)
 """
+import multiprocessing as mp
+
 import numpy as np
 import pytest
 
-from emma_amd import rt, synth
+import fullsize_oracle
+from emma_amd import rt, synth, tune
 from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -48,12 +53,19 @@ def _quantities(F):
     return {"x": F.nhplus / F.nh, "temp": F.temp, "e": F.e}
 
 
-def _errors(A, B):
+def _fold(d):
+    """per cell: the worst group"""
+    return d.max(0) if d.ndim == 2 else d
+
+
+def _errors(A, B, base=None):
+    """per cell relative errors of A against B (or against the arrays
+    base(k) returns for k = x, temp, e)"""
     out = {}
-    qa, qb = _quantities(A), _quantities(B)
+    qa = _quantities(A)
+    qb = _quantities(B) if B is not None else {k: base(k) for k in qa}
     for k in qa:
-        d = _rel(qa[k], qb[k])
-        out[k] = d.max(0) if d.ndim == 2 else d
+        out[k] = _fold(_rel(qa[k], qb[k]))
     return out
 
 
@@ -103,3 +115,34 @@ def test_c3_two_subcycles(gpu_lib):
 def test_c4_one_coarse_step(gpu_lib):
     W = synth.config_c4(seed=4, nsub=1)
     _check(W, 1, "C4")
+
+
+def test_c5_one_coarse_step(gpu_lib, tmp_path):
+    """the headline workload: the oracle runs (drawn state, 1e-7 and 1e-6
+    perturbations, about a minute each) in three processes while the GPU
+    runs the tuned mixed path"""
+    jobs = [("c5", 3, 1, eps, ps, str(tmp_path / tag)) for eps, ps, tag in ((0.0, 0, "base"), (1e-7, 11, "p1"),
+                                                                          (1e-6, 12, "p2"))]
+    with mp.get_context("spawn").Pool(3) as pool:
+        pending = pool.map_async(fullsize_oracle.run, jobs)
+        W = synth.config_c5(seed=3, nsub=1)
+        g = rt.RadGPU(W.params, rt.MIXED)
+        g.setup(W.params, W.tree, W.fields)
+        used = tune.apply(g)
+        g.coarse_step(1.0, 1)
+        assert g.stats()["status"] == 0
+        mixed = {l: g.get(l) for l in range(W.tree.lmin, W.tree.lmax + 1)}
+        g.close()
+        print("C5: GPU done, tuning", {l: (e["variant"], e["chem_budget"]) for l, e in used.items()}, flush=True)
+        print("C5: oracle runs", pending.get(timeout=600), "s", flush=True)
+    for l, A in mixed.items():
+        err = _errors(A, None, base=lambda k: np.load(str(tmp_path / f"base_{l}_{k}.npy")))
+        for k, e in err.items():
+            b = np.load(str(tmp_path / f"base_{l}_{k}.npy"))
+            dev = np.maximum.reduce([_fold(_rel(np.load(str(tmp_path / f"{p}_{l}_{k}.npy")), b)) for p in ("p1", "p2")])
+            out = np.nonzero(e > TOL)[0]
+            print(f"C5 level {l} {k}: max err {e.max():.3e}, cells beyond {TOL:g}: {out.size} of {e.size}"
+                  + (f" (their FP64 envelope {dev[out].min():.3e}..{dev[out].max():.3e}, mixed "
+                     f"{e[out].min():.3e}..{e[out].max():.3e})" if out.size else ""), flush=True)
+            bad = out[e[out] > 2.0 * dev[out]]
+            assert bad.size == 0, (l, k, bad[:10].tolist(), e[bad[:10]].tolist(), dev[bad[:10]].tolist())
