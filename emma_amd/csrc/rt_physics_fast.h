@@ -64,7 +64,7 @@ __device__ __forceinline__ Closure<float> closure_fast(float fx, float fy, float
 // the rate fits of chem_utils.c:37-187 in FP32: pow(x, a) = exp2(a log2 x)
 // with log2(lambda) shared, exp(-E/T) = exp2(-E log2(e) / T)
 __device__ __forceinline__ Rates rates_fast(double Td) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   const float T = (float)Td;
   const float rT = frcp32(T);
   const float sT = __builtin_amdgcn_sqrtf(T);
@@ -96,7 +96,10 @@ __device__ __forceinline__ Rates rates_fast(double Td) {
 // The sub-step loop is split into begin (unit conversions, EMIN test), run
 // (the loop, optionally for a limited number of tries, its whole state in
 // ChemFast so that another thread can continue it bit for bit) and end (the
-// conversions back).
+// conversions back).  The chemistry (and its rate fits) contracts
+// multiply-adds only within one expression (fp contract(on), not fast), so
+// the loop rounds the same in every kernel it is inlined into: the straggler
+// pass takes exactly the sub-steps the undivided loop takes.
 template <int G>
 struct ChemFast {
   static constexpr int GA = G > 0 ? G : kMaxGrp;
@@ -110,7 +113,7 @@ struct ChemFast {
 template <int G>
 __device__ __forceinline__ int chem_fast_begin(const ChemConst& K, int ng, const double* e, const double* src,
                                                double nhplus, double eint_code, double nh, ChemFast<G>& s) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   constexpr int GA = ChemFast<G>::GA;
 #pragma unroll
   for (int g = 0; g < GA; g++) {
@@ -141,7 +144,7 @@ __device__ __forceinline__ int chem_fast_begin(const ChemConst& K, int ng, const
 template <int G>
 __device__ __forceinline__ int chem_fast_run(const ChemConst& K, int ng, int chemonly, ChemFast<G>& s, int budget,
                                              double keep = 0.) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   constexpr int GA = ChemFast<G>::GA;
   double et[GA], rp[GA];
   const double notchem = (double)(!chemonly);
@@ -267,7 +270,7 @@ template <int G>
 __device__ __forceinline__ void chem_fast_end(const ChemConst& K, int ng, int chemonly, const ChemFast<G>& s,
                                               double* e, double* fscale, double& nhplus, double& eint_code,
                                               double& temp) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   constexpr int GA = ChemFast<G>::GA;
   const double kT = 1. / (1.5 * kKboltz);
   double ki_e = K.ki_e, ki_f = K.ki_f, ki_eint = K.ki_eint, a3f = K.pa3;
