@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(LIBDIR, "libemma_rt.so")
-SOURCES = ["rt_kernels.hip", "rt_stencil.hip", "rt_fill.hip", "rt_capi.hip"]
+SOURCES = ["rt_kernels.hip", "rt_stencil.hip", "rt_fill.hip", "rt_capi.hip", "rt_sched.cpp"]
 ARCH = os.environ.get("EMMA_RT_ARCH", "gfx950")
 
 
@@ -44,7 +44,7 @@ def flags() -> list:
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(OBJDIR, src.replace(".hip", ".o"))
+    obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
     cmd = [hipcc()] + flags() + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -16,6 +16,7 @@
 
 #include "emma_rt.h"
 #include "rt_kernels.h"
+#include "rt_sched.h"
 #include "rt_stencil.h"
 
 using namespace emma;
@@ -66,6 +67,7 @@ struct DevLevel {
   double* t_first = nullptr;
   double* t_last = nullptr;
   std::vector<int> hchild;  // host copy with the reflux pairs encoded
+  std::vector<int> hnei;    // host copy of nei_oct (the persistent schedule's planner)
   std::vector<int> hrpair;
   // hydro state (struct Wtype) for the WRADHYD coupling, and the octs' lower
   // corners for the test source; allocated on first use
@@ -85,13 +87,43 @@ struct DevLevel {
   int snap_cur = -1;
 };
 
+// one launch of a cached coarse step: a persistent segment (consecutive level
+// steps) or a COARSERAD transfer (putE2coarse / distribE) with its tree view
+struct SchedLaunch {
+  int kind = 0;  // 0 persistent segment, 1 putE2coarse, 2 distribE
+  SchedArgs args;
+  int blocks = 0;
+  bool coupled = false;
+  std::vector<unsigned char> tv;  // TreeView<R>
+};
+
+// a captured coarse step and what it needs while it exists: the buffer
+// parity after it, and for the persistent schedule the device tables of its
+// segments and their launches (replayed eagerly in EMMA_RT_TIMING_LAUNCH)
+struct Cached {
+  hipGraphExec_t ex = nullptr;
+  std::vector<int> endpar;
+  std::vector<void*> bufs;
+  std::vector<SchedLaunch> launches;
+  emma_rt_sched_info info;
+};
+
+// an operation of the recorded coarse step (persistent schedule)
+struct SchedOp {
+  int kind = 0;  // 0 level step (StepDesc<R>), 1 putE2coarse, 2 distribE (TreeView<R>)
+  int k = 0;
+  std::vector<unsigned char> blob;
+};
+
 struct GraphKey {
   int ncoarse;
   double aexp;
   std::vector<int> parity;
   int xion;
   int mode;
+  int sched;
   bool operator<(const GraphKey& o) const {
+    if (sched != o.sched) return sched < o.sched;
     if (mode != o.mode) return mode < o.mode;
     if (ncoarse != o.ncoarse) return ncoarse < o.ncoarse;
     if (aexp != o.aexp) return aexp < o.aexp;
@@ -142,13 +174,24 @@ struct emma_rt_ctx {
   std::vector<int> ev_level;   // level index of each pair, -1 for a whole graph replay
   double level_ms[kMaxLevels];  // device time per level of the last timed call
   int level_launches[kMaxLevels];
-  std::map<GraphKey, std::pair<hipGraphExec_t, std::vector<int>>> graphs;
+  std::map<GraphKey, Cached> graphs;
+  // coarse-step schedule: the captured graph of level-step launches, or one
+  // persistent kernel per run of level steps (EMMA_RT_SCHED_PERSISTENT)
+  int sched = EMMA_RT_SCHED_GRAPH;
+  int ncu = 0;  // compute units of the device
+  emma_rt_sched_info sched_info;
+  // recording of a coarse step (persistent schedule): the helpers below
+  // record instead of launching while rec is set; FillRad's per-cell block
+  // waits in pending_fill for its level's next step
+  std::vector<SchedOp>* rec = nullptr;
+  std::vector<unsigned char> pending_fill[kMaxLevels];
   // stencil path
   void* d_stencil = nullptr;
   int stride = 0;
   int stencil_rbytes = 0;
 };
 
+static void drop_graphs(emma_rt_ctx* c);
 static void free_tree(emma_rt_ctx* c) {
   for (auto& L : c->lev) {
     for (int b = 0; b < 2; b++) {
@@ -181,13 +224,17 @@ static void free_tree(emma_rt_ctx* c) {
   c->d_defer_ctl = nullptr;
   c->defer_cap = 0;
   c->defer_slots = 0;
-  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.first);
-  c->graphs.clear();
+  drop_graphs(c);
   c->has_tree = false;
 }
 
 static void drop_graphs(emma_rt_ctx* c) {
-  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.first);
+  if (c->graphs.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->graphs) {
+    if (kv.second.ex) (void)hipGraphExecDestroy(kv.second.ex);
+    for (void* b : kv.second.bufs) (void)hipFree(b);
+  }
   c->graphs.clear();
 }
 
@@ -299,6 +346,7 @@ extern "C" const char* emma_rt_strerror(int s) {
     case EMMA_RT_ERR_LAYOUT: return "struct layout mismatch";
     case EMMA_RT_ERR_NODEVICE: return "no GPU device";
     case EMMA_RT_ERR_STATE: return "call out of order";
+    case EMMA_RT_ERR_SCHED: return "persistent coarse step: a dependency wait passed its bound";
     default: return "unknown status";
   }
 }
@@ -359,6 +407,9 @@ extern "C" int emma_rt_create(const emma_rt_params* p, emma_rt_ctx** out) {
     emma_rt_destroy(c);
     return st;
   }
+  if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || c->ncu < 1)
+    c->ncu = 1;
+  memset(&c->sched_info, 0, sizeof(c->sched_info));
   *out = c;
   return EMMA_RT_OK;
 }
@@ -432,6 +483,7 @@ extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int
     L.noct = noct[k];
     L.ncell = noct[k] * 8;
     L.hchild.assign(child + off_c[k], child + off_c[k] + L.ncell);
+    L.hnei.assign(nei_oct + off_n[k], nei_oct + off_n[k] + (size_t)L.noct * 6);
   }
   for (int k = 0; k < nl; k++) {
     DevLevel& L = c->lev[k];
@@ -795,7 +847,44 @@ static hipEvent_t next_event(emma_rt_ctx* c) {
   return c->ev[c->ev_used++];
 }
 
+template <typename R>
+static void record_step(emma_rt_ctx* c, int k, double dtnew, double aexp, int chemonly) {
+  StepDesc<R> D;
+  memset(&D, 0, sizeof(D));
+  D.a = level_args<R>(c, k, dtnew, aexp, chemonly, true);
+  // no straggler pass inside the persistent kernel: every leaf finishes its
+  // chemistry in its tile
+  D.a.chem_budget = 0;
+  D.a.defer_cap = 0;
+  D.a.defer_list = nullptr;
+  D.a.defer_ctl = nullptr;
+  D.K = make_chem(c->p, dtnew, aexp);
+  // the kernel holds the listed and per-direction halo variants; the latency
+  // variant (more registers than the kernel's occupancy allows) runs as
+  // per-direction
+  D.variant = c->lev[k].variant == kListed ? kListed : kPerDir;
+  if (!c->pending_fill[k].empty()) {
+    memcpy(&D.f, c->pending_fill[k].data(), sizeof(D.f));
+    D.fill = 1;
+    c->pending_fill[k].clear();
+  }
+  SchedOp op;
+  op.kind = 0;
+  op.k = k;
+  op.blob.resize(sizeof(D));
+  memcpy(op.blob.data(), &D, sizeof(D));
+  c->rec->push_back(std::move(op));
+  c->lev[k].cur = 1 - c->lev[k].cur;
+}
+
 static int timed_launch(emma_rt_ctx* c, int k, double dtnew, double aexp, int chemonly) {
+  if (c->rec) {
+    if (c->rbytes == 8)
+      record_step<double>(c, k, dtnew, aexp, chemonly);
+    else
+      record_step<float>(c, k, dtnew, aexp, chemonly);
+    return EMMA_RT_OK;
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
     e0 = next_event(c);
@@ -903,13 +992,37 @@ static TreeView<R> tree_view(emma_rt_ctx* c) {
 
 // FillRad's COARSERAD block (src_utils.c:717-733): split cells of the coarse
 // level gather the volume-weighted E, F of their leaves
+template <typename R>
+static void record_transfer(emma_rt_ctx* c, int kind) {
+  const TreeView<R> t = tree_view<R>(c);
+  SchedOp op;
+  op.kind = kind;
+  op.blob.resize(sizeof(t));
+  memcpy(op.blob.data(), &t, sizeof(t));
+  c->rec->push_back(std::move(op));
+}
+
 static hipError_t launch_put(emma_rt_ctx* c) {
+  if (c->rec) {
+    if (c->rbytes == 8)
+      record_transfer<double>(c, 1);
+    else
+      record_transfer<float>(c, 1);
+    return hipSuccess;
+  }
   if (c->rbytes == 8) return launch_put_e2coarse<double>(tree_view<double>(c), c->stream);
   return launch_put_e2coarse<float>(tree_view<float>(c), c->stream);
 }
 
 // RadSolver's COARSERAD push of the coarse values to the leaves (rad_utils.c:2009-2013)
 static hipError_t launch_distrib(emma_rt_ctx* c) {
+  if (c->rec) {
+    if (c->rbytes == 8)
+      record_transfer<double>(c, 2);
+    else
+      record_transfer<float>(c, 2);
+    return hipSuccess;
+  }
   if (c->rbytes == 8) return launch_distrib_e<double>(tree_view<double>(c), c->stream);
   return launch_distrib_e<float>(tree_view<float>(c), c->stream);
 }
@@ -997,6 +1110,11 @@ static hipError_t launch_fill_cells(emma_rt_ctx* c, int k, double aexp) {
   f.pa5 = pow(aexp, 5);
   f.pstar = c->p.unit_n * c->p.unit_d * pow(c->p.unit_v, 2);
   f.unit_N = c->p.unit_N;
+  if (c->rec) {  // fused into the level's next step
+    c->pending_fill[k].resize(sizeof(f));
+    memcpy(c->pending_fill[k].data(), &f, sizeof(f));
+    return hipSuccess;
+  }
   return launch_fill_level<R>(f, c->stream);
 }
 
@@ -1021,6 +1139,22 @@ static hipError_t to_hydro(emma_rt_ctx* c, int k) {
   h.p = L.hyd[4];
   h.dX = L.hyd[5];
   h.E = L.hyd[6];
+  if (c->rec) {  // fused into the level's last recorded step
+    for (auto it = c->rec->rbegin(); it != c->rec->rend(); ++it) {
+      if (it->kind != 0 || it->k != k) continue;
+      if (c->rbytes == 8) {
+        StepDesc<double>* D = reinterpret_cast<StepDesc<double>*>(it->blob.data());
+        D->h = h;
+        D->hydro = 1;
+      } else {
+        StepDesc<float>* D = reinterpret_cast<StepDesc<float>*>(it->blob.data());
+        D->h = h;
+        D->hydro = 1;
+      }
+      return hipSuccess;
+    }
+    return hipErrorInvalidValue;
+  }
   return launch_rad_to_hydro(h, c->stream);
 }
 
@@ -1217,6 +1351,162 @@ static int recurse(emma_rt_ctx* c, int k, double aexp) {
   return EMMA_RT_OK;
 }
 
+// ---------------------------------------------------------------- persistent coarse step
+// The coarse step is recorded (the recursion above with the launch helpers
+// recording instead of launching), each run of consecutive level steps is
+// planned (rt_sched.h) and its tables go to the device once; the captured
+// graph then holds one persistent launch per run (plus the COARSERAD
+// transfers between runs).
+constexpr long long kSchedWaitTicks = 200000000LL;  // 2 s of s_memrealtime (100 MHz)
+
+static std::vector<SchedLevelTopo> sched_topo(emma_rt_ctx* c) {
+  std::vector<SchedLevelTopo> t(c->lev.size());
+  for (size_t k = 0; k < c->lev.size(); k++) {
+    t[k].noct = c->lev[k].noct;
+    t[k].nei = c->lev[k].hnei.data();
+    t[k].child = c->lev[k].hchild.data();
+  }
+  return t;
+}
+
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+template <typename R>
+static int build_persistent(emma_rt_ctx* c, const std::vector<SchedOp>& ops, Cached& out) {
+  const int ng = c->p.ngrp;
+  const std::vector<SchedLevelTopo> topo = sched_topo(c);
+  const std::vector<double> cost(c->lev.size(), 1.);
+  memset(&out.info, 0, sizeof(out.info));
+  size_t i = 0;
+  while (i < ops.size()) {
+    if (ops[i].kind != 0) {
+      SchedLaunch l;
+      l.kind = ops[i].kind;
+      l.tv = ops[i].blob;
+      out.launches.push_back(std::move(l));
+      i++;
+      continue;
+    }
+    size_t j = i;
+    std::vector<int> steps;
+    bool coupled = false;
+    while (j < ops.size() && ops[j].kind == 0) {
+      const StepDesc<R>* D = reinterpret_cast<const StepDesc<R>*>(ops[j].blob.data());
+      coupled = coupled || D->fill || D->hydro;
+      steps.push_back(ops[j].k);
+      j++;
+    }
+    const int bpc = persistent_blocks_per_cu<R>(ng, coupled);
+    if (bpc < 1) return EMMA_RT_ERR_HIP;
+    const int blocks = bpc * c->ncu;
+    SchedPlan plan;
+    if (plan_schedule(topo, steps, cost, blocks, plan) != 0) return EMMA_RT_ERR_ARG;
+    const size_t n = (size_t)plan.nitems, m = plan.deps.size();
+    const size_t o_desc = 0, o_step = align256(o_desc + steps.size() * sizeof(StepDesc<R>));
+    const size_t o_tile = align256(o_step + n * sizeof(int)), o_ntile = align256(o_tile + n * sizeof(int));
+    const size_t o_prio = align256(o_ntile + n * sizeof(int));
+    const size_t o_doff = align256(o_prio + n), o_deps = align256(o_doff + (n + 1) * sizeof(int));
+    const size_t o_done = align256(o_deps + std::max<size_t>(m, 1) * sizeof(int));
+    const size_t o_ctl = align256(o_done + n * sizeof(int)), total = o_ctl + 4 * sizeof(int);
+    char* buf = nullptr;
+    HIPCHK(hipMalloc((void**)&buf, total));
+    out.bufs.push_back(buf);
+    std::vector<unsigned char> descs(steps.size() * sizeof(StepDesc<R>));
+    for (size_t q = i; q < j; q++) memcpy(descs.data() + (q - i) * sizeof(StepDesc<R>), ops[q].blob.data(), sizeof(StepDesc<R>));
+    const int ctl0[4] = {0, 0, 1, 0};  // next ticket, work-groups that left, epoch
+    HIPCHK(hipMemcpy(buf + o_desc, descs.data(), descs.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(buf + o_step, plan.item_step.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(buf + o_tile, plan.item_tile.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(buf + o_ntile, plan.item_ntile.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(buf + o_prio, plan.item_prio.data(), n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(buf + o_doff, plan.dep_off.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if (m > 0) HIPCHK(hipMemcpy(buf + o_deps, plan.deps.data(), m * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(buf + o_done, 0, n * sizeof(int)));
+    HIPCHK(hipMemcpy(buf + o_ctl, ctl0, sizeof(ctl0), hipMemcpyHostToDevice));
+    SchedLaunch l;
+    l.kind = 0;
+    l.blocks = blocks;
+    l.coupled = coupled;
+    l.args.steps = buf + o_desc;
+    l.args.item_step = (const int*)(buf + o_step);
+    l.args.item_tile = (const int*)(buf + o_tile);
+    l.args.item_ntile = (const int*)(buf + o_ntile);
+    l.args.item_prio = (const unsigned char*)(buf + o_prio);
+    l.args.dep_off = (const int*)(buf + o_doff);
+    l.args.deps = (const int*)(buf + o_deps);
+    l.args.done = (int*)(buf + o_done);
+    l.args.ctl = (int*)(buf + o_ctl);
+    l.args.nitems = (int)n;
+    l.args.pad = 0;
+    l.args.status = c->d_status;
+    l.args.wait_ticks = kSchedWaitTicks;
+    out.launches.push_back(std::move(l));
+    out.info.items += (long long)n;
+    out.info.deps += (long long)m;
+    out.info.blocks = std::max(out.info.blocks, blocks);
+    out.info.launches++;
+    out.info.est_span += plan.est_span;
+    out.info.est_chain += plan.est_chain;
+    out.info.est_work += plan.est_work;
+    i = j;
+  }
+  return EMMA_RT_OK;
+}
+
+template <typename R>
+static int run_launches(emma_rt_ctx* c, const std::vector<SchedLaunch>& launches, bool timed) {
+  for (const SchedLaunch& l : launches) {
+    if (l.kind == 0) {
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (timed) {
+        e0 = next_event(c);
+        e1 = next_event(c);
+        if (!e0 || !e1) return EMMA_RT_ERR_HIP;
+        c->ev_level.push_back(-1);
+        HIPCHK(hipEventRecord(e0, c->stream));
+      }
+      HIPCHK(launch_persistent<R>(l.args, c->p.ngrp, l.coupled, l.blocks, c->stream));
+      if (timed) HIPCHK(hipEventRecord(e1, c->stream));
+    } else if (l.kind == 1) {
+      HIPCHK(launch_put_e2coarse<R>(*reinterpret_cast<const TreeView<R>*>(l.tv.data()), c->stream));
+    } else {
+      HIPCHK(launch_distrib_e<R>(*reinterpret_cast<const TreeView<R>*>(l.tv.data()), c->stream));
+    }
+  }
+  return EMMA_RT_OK;
+}
+
+static int run_cached(emma_rt_ctx* c, const Cached& e, bool timed) {
+  return c->rbytes == 8 ? run_launches<double>(c, e.launches, timed) : run_launches<float>(c, e.launches, timed);
+}
+
+// record ncoarse coarse steps from the current buffer parity, plan and
+// upload them; the parity is left as it was (endpar: after the steps)
+static int record_persistent(emma_rt_ctx* c, int ncoarse, double aexp, Cached& e) {
+  std::vector<SchedOp> ops;
+  std::vector<int> par;
+  for (auto& L : c->lev) par.push_back(L.cur);
+  c->rec = &ops;
+  int rs = EMMA_RT_OK;
+  for (int i = 0; i < ncoarse && rs == EMMA_RT_OK; i++) rs = recurse(c, 0, aexp);
+  c->rec = nullptr;
+  for (auto& L : c->lev) e.endpar.push_back(L.cur);
+  for (size_t k = 0; k < c->lev.size(); k++) {
+    c->lev[k].cur = par[k];
+    c->pending_fill[k].clear();
+  }
+  if (rs) return rs;
+  rs = c->rbytes == 8 ? build_persistent<double>(c, ops, e) : build_persistent<float>(c, ops, e);
+  return rs;
+}
+
+static void free_cached(Cached& e) {
+  if (e.ex) (void)hipGraphExecDestroy(e.ex);
+  e.ex = nullptr;
+  for (void* b : e.bufs) (void)hipFree(b);
+  e.bufs.clear();
+}
+
 extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
   if (c == nullptr || !c->has_tree) return EMMA_RT_ERR_STATE;
   if (ncoarse < 1) return EMMA_RT_ERR_ARG;
@@ -1226,7 +1516,7 @@ extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
   if (st) return st;
   st = begin_call(c);
   if (st) return st;
-  if (c->timing == 1) {  // eager launches with per-kernel events
+  if (c->timing == 1 && c->sched == EMMA_RT_SCHED_GRAPH) {  // eager launches with per-kernel events
     for (int i = 0; i < ncoarse; i++) {
       st = recurse(c, 0, aexp);
       if (st) return st;
@@ -1238,47 +1528,135 @@ extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
   key.aexp = aexp;
   key.xion = c->xion;
   key.mode = c->mode;
+  key.sched = c->sched;
   for (auto& L : c->lev) key.parity.push_back(L.cur);
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
+    Cached e;
     hipGraph_t g;
-    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rs = EMMA_RT_OK;
-    const int timing = c->timing;
-    c->timing = 0;  // no event records inside the graph
-    for (int i = 0; i < ncoarse && rs == EMMA_RT_OK; i++) rs = recurse(c, 0, aexp);
-    c->timing = timing;
-    hipError_t ce = hipStreamEndCapture(c->stream, &g);
-    std::vector<int> endpar;
-    for (auto& L : c->lev) endpar.push_back(L.cur);
-    // nothing ran during the capture: restore the buffer parity the replay
-    // starts from (the launch below advances it), also on the error paths
-    for (size_t k = 0; k < c->lev.size(); k++) c->lev[k].cur = key.parity[k];
-    if (rs || ce != hipSuccess) {
-      if (ce == hipSuccess) (void)hipGraphDestroy(g);
-      if (rs) return rs;
-      HIPCHK(ce);
+    if (c->sched == EMMA_RT_SCHED_PERSISTENT) {
+      int rs = record_persistent(c, ncoarse, aexp, e);
+      if (rs) {
+        free_cached(e);
+        return rs;
+      }
+      hipError_t be = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+      if (be != hipSuccess) {
+        free_cached(e);
+        HIPCHK(be);
+      }
+      rs = run_cached(c, e, false);
+      hipError_t ce = hipStreamEndCapture(c->stream, &g);
+      if (rs || ce != hipSuccess) {
+        if (ce == hipSuccess) (void)hipGraphDestroy(g);
+        free_cached(e);
+        if (rs) return rs;
+        HIPCHK(ce);
+      }
+    } else {
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rs = EMMA_RT_OK;
+      const int timing = c->timing;
+      c->timing = 0;  // no event records inside the graph
+      for (int i = 0; i < ncoarse && rs == EMMA_RT_OK; i++) rs = recurse(c, 0, aexp);
+      c->timing = timing;
+      hipError_t ce = hipStreamEndCapture(c->stream, &g);
+      for (auto& L : c->lev) e.endpar.push_back(L.cur);
+      // nothing ran during the capture: restore the buffer parity the replay
+      // starts from (the launch below advances it), also on the error paths
+      for (size_t k = 0; k < c->lev.size(); k++) c->lev[k].cur = key.parity[k];
+      if (rs || ce != hipSuccess) {
+        if (ce == hipSuccess) (void)hipGraphDestroy(g);
+        if (rs) return rs;
+        HIPCHK(ce);
+      }
+      memset(&e.info, 0, sizeof(e.info));
     }
-    hipGraphExec_t ex;
-    hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    hipError_t ie = hipGraphInstantiate(&e.ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    HIPCHK(ie);
-    it = c->graphs.emplace(key, std::make_pair(ex, endpar)).first;
+    if (ie != hipSuccess) {
+      e.ex = nullptr;
+      free_cached(e);
+      HIPCHK(ie);
+    }
+    it = c->graphs.emplace(key, std::move(e)).first;
     c->stats.level_steps = 0;
   }
-  if (c->timing == 2) {  // one event pair around the replay
+  c->sched_info = it->second.info;
+  if (c->timing == 1) {  // persistent schedule, eager: one event pair per persistent launch
+    st = run_cached(c, it->second, true);
+    if (st) return st;
+  } else if (c->timing == 2) {  // one event pair around the replay
     hipEvent_t e0 = next_event(c), e1 = next_event(c);
     if (!e0 || !e1) return EMMA_RT_ERR_HIP;
     c->ev_level.push_back(-1);
     HIPCHK(hipEventRecord(e0, c->stream));
-    HIPCHK(hipGraphLaunch(it->second.first, c->stream));
+    HIPCHK(hipGraphLaunch(it->second.ex, c->stream));
     HIPCHK(hipEventRecord(e1, c->stream));
   } else {
-    HIPCHK(hipGraphLaunch(it->second.first, c->stream));
+    HIPCHK(hipGraphLaunch(it->second.ex, c->stream));
   }
-  for (size_t k = 0; k < c->lev.size(); k++) c->lev[k].cur = it->second.second[k];
+  for (size_t k = 0; k < c->lev.size(); k++) c->lev[k].cur = it->second.endpar[k];
   c->stats.level_steps = steps_per_call(c, ncoarse);
   return end_call(c);
+}
+
+extern "C" int emma_rt_set_schedule(emma_rt_ctx* c, int sched) {
+  if (c == nullptr) return EMMA_RT_ERR_STATE;
+  if (sched != EMMA_RT_SCHED_GRAPH && sched != EMMA_RT_SCHED_PERSISTENT) return EMMA_RT_ERR_ARG;
+  if (sched != c->sched) drop_graphs(c);
+  c->sched = sched;
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_get_schedule(emma_rt_ctx* c, int* sched, emma_rt_sched_info* info) {
+  if (c == nullptr) return EMMA_RT_ERR_STATE;
+  if (sched) *sched = c->sched;
+  if (info) *info = c->sched_info;
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_plan_schedule(int nlev, const int* noct, const int* nei_oct, const int* child, int nsteps,
+                                     const int* step_level, const double* tile_cost, const int* chunk, int slots,
+                                     long long cap_items, long long cap_deps, long long* nitems, long long* ndeps,
+                                     int* item_step, int* item_tile, int* item_ntile, unsigned char* item_prio,
+                                     int* dep_off, int* deps, double* est) {
+  if (nlev < 1 || nlev > kMaxLevels || noct == nullptr || nei_oct == nullptr || child == nullptr || nsteps < 0 ||
+      (nsteps > 0 && step_level == nullptr) || nitems == nullptr || ndeps == nullptr)
+    return EMMA_RT_ERR_ARG;
+  std::vector<SchedLevelTopo> topo(nlev);
+  size_t on = 0, oc = 0;
+  for (int k = 0; k < nlev; k++) {
+    if (noct[k] < 0) return EMMA_RT_ERR_ARG;
+    topo[k].noct = noct[k];
+    topo[k].nei = nei_oct + on;
+    topo[k].child = child + oc;
+    on += (size_t)noct[k] * 6;
+    oc += (size_t)noct[k] * 8;
+  }
+  std::vector<int> steps(step_level, step_level + nsteps);
+  std::vector<double> cost(nlev, 1.);
+  if (tile_cost)
+    for (int k = 0; k < nlev; k++) cost[k] = tile_cost[k];
+  std::vector<int> cs;
+  if (chunk) cs.assign(chunk, chunk + nlev);
+  SchedPlan plan;
+  if (plan_schedule(topo, steps, cost, slots, plan, chunk ? &cs : nullptr) != 0) return EMMA_RT_ERR_ARG;
+  *nitems = plan.nitems;
+  *ndeps = (long long)plan.deps.size();
+  if (est) {
+    est[0] = plan.est_span;
+    est[1] = plan.est_chain;
+    est[2] = plan.est_work;
+  }
+  if (cap_items < plan.nitems || cap_deps < (long long)plan.deps.size()) return cap_items == 0 ? EMMA_RT_OK : EMMA_RT_ERR_ARG;
+  if (item_step) memcpy(item_step, plan.item_step.data(), plan.item_step.size() * sizeof(int));
+  if (item_tile) memcpy(item_tile, plan.item_tile.data(), plan.item_tile.size() * sizeof(int));
+  if (item_ntile) memcpy(item_ntile, plan.item_ntile.data(), plan.item_ntile.size() * sizeof(int));
+  if (item_prio) memcpy(item_prio, plan.item_prio.data(), plan.item_prio.size());
+  if (dep_off) memcpy(dep_off, plan.dep_off.data(), plan.dep_off.size() * sizeof(int));
+  if (deps && !plan.deps.empty()) memcpy(deps, plan.deps.data(), plan.deps.size() * sizeof(int));
+  return EMMA_RT_OK;
 }
 
 extern "C" int emma_rt_set_chem_budget(emma_rt_ctx* c, int level, int budget) {

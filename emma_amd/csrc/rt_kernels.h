@@ -232,6 +232,67 @@ struct HydroArgs {
 };
 hipError_t launch_rad_to_hydro(const HydroArgs& h, hipStream_t s);
 
+// ---------------------------------------------------------------- persistent coarse step
+// A coarse step (all its level steps, with FillRad's block before and the
+// hydro injection after each) as ONE launch of persistent work-groups fed by
+// a device work queue.  An item is a run of consecutive tiles (kTileOcts
+// octs each, the cells of one level-step work-group) of one level step: one
+// tile on all but the largest levels.  Tickets are handed out in the
+// order the host's list schedule chose (a topological order of the items'
+// dependencies); a work-group takes the next ticket, waits until the tickets
+// it depends on are done, runs the item and marks its ticket done.  Every
+// wait is on a smaller ticket, held by a work-group that is running, so the
+// queue cannot deadlock whatever the number of resident work-groups.
+
+// LDS of one tile step (all halo variants use the same layout, so a kernel
+// that runs several variants holds it once)
+template <int G>
+struct TileLds {
+  static constexpr int kHaloCap = 64;  // halo slots per wave
+  float4 share[G * 2 * kTileCells + kTileWaves * kHaloCap * G];
+  unsigned char cancel[kTileCells + kTileWaves * kHaloCap];
+  unsigned char item[kTileWaves][kHaloCap];
+  int item_nei[kTileWaves][kHaloCap];
+};
+
+// one level step of the schedule: its arguments, FillRad's block before it
+// and the hydro injection after it (fill / hydro: 0 = skip)
+template <typename R>
+struct StepDesc {
+  LevelArgs<R> a;
+  ChemConst K;
+  FillArgs<R> f;
+  HydroArgs h;
+  int fill, hydro, variant, pad;
+};
+
+constexpr int kErrSched = -10;  // EMMA_RT_ERR_SCHED
+
+struct SchedArgs {
+  const void* steps;        // StepDesc<R>[nsteps]
+  const int* item_step;     // ticket -> level step
+  const int* item_tile;     // ticket -> first tile of that level step
+  const int* item_ntile;    // ticket -> consecutive tiles of the item
+  const unsigned char* item_prio;  // ticket -> wave priority while it runs (0..3)
+  const int* dep_off;       // [nitems + 1]: the tickets ticket t waits for are deps[dep_off[t] .. dep_off[t+1])
+  const int* deps;
+  int* done;                // [nitems]: == epoch once the ticket's item is complete
+  int* ctl;                 // [0] next ticket, [1] work-groups that left, [2] epoch
+  int nitems;
+  int pad;
+  int* status;              // the library's status word (a wait past its bound: EMMA_RT_ERR_SCHED)
+  long long wait_ticks;     // bound of one dependency wait, s_memrealtime ticks (100 MHz)
+};
+
+// the persistent launch: R and ng select the kernel family of the level
+// steps (the tiled mixed kernel for G = 1, 3; the per-cell kernel
+// otherwise); coupled: the steps carry FillRad's block / the hydro injection
+template <typename R>
+hipError_t launch_persistent(const SchedArgs& s, int ng, bool coupled, int nblocks, hipStream_t st);
+// work-groups of the persistent kernel one CU holds at once
+template <typename R>
+int persistent_blocks_per_cu(int ng, bool coupled);
+
 template <typename R>
 hipError_t launch_photon_sum(const R* e, const int* child, int ncell, int ng, double vol, double* out,
                              hipStream_t s);

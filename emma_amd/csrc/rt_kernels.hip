@@ -13,6 +13,7 @@
 
 #include <algorithm>
 
+#include "rt_fill.h"
 #include "rt_kernels.h"
 #include "rt_physics_fast.h"
 
@@ -423,14 +424,12 @@ __device__ __forceinline__ void block_counters(unsigned long long* counters, uns
 // One thread per cell; neighbours straight from global memory.  Fast selects
 // the mixed-precision physics (rt_physics_fast.h); the FP64 kernel
 // (Fast = false) keeps the reference's operation order.
+// The cells of block `block` (256 consecutive cells, 32 octs).
 template <typename R, int G, bool Fast>
-__global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value, ChemConst K_by_value) {
-  const KernArgs<R> args = kernargs<R>();
-  const LevelArgs<R>& a = args.a;
-  const ChemConst& K = args.K;
+__device__ __forceinline__ void cell_step(const LevelArgs<R>& a, const ChemConst& K, int block, int lc) {
   constexpr int GA = G > 0 ? G : kMaxGrp;
   const int ng = G > 0 ? G : a.ng;
-  const int cell = xcd_block(a.nblocks) * 256 + (int)threadIdx.x;
+  const int cell = block * 256 + lc;
   unsigned int nit_acc = 0;
   unsigned int leaf = 0;
   if (cell < a.ncell) {
@@ -539,6 +538,12 @@ __global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value
   if (a.counters != nullptr) block_counters<4>(a.counters, nit_acc, leaf);
 }
 
+template <typename R, int G, bool Fast>
+__global__ __launch_bounds__(256) void level_step_kernel(LevelArgs<R> a_by_value, ChemConst K_by_value) {
+  const KernArgs<R> args = kernargs<R>();
+  cell_step<R, G, Fast>(args.a, args.K, xcd_block(args.a.nblocks), (int)threadIdx.x);
+}
+
 // ------------------------------------------------------------------ tiled level step
 // Mixed precision, G fixed.  A 256-thread block owns 32 consecutive octs
 // (Morton order: a 4x4x2 brick of octs on a complete level), one cell per
@@ -589,22 +594,20 @@ struct HaloFace {
 };
 
 template <int G, int V>
-__device__ __forceinline__ void tiled_step(const LevelArgs<float>& a, const ChemConst& K) {
+__device__ __forceinline__ void tiled_step(const LevelArgs<float>& a, const ChemConst& K, int tile, TileLds<G>& lds,
+                                           int lc) {
   constexpr bool PerDir = V != kListed;
   constexpr bool Late = V == kPerDir;  // chemistry inputs loaded when the chemistry starts
   constexpr bool NT = V == kListed;    // non-temporal next state and chemistry inputs (large levels)
-  constexpr int kHaloCap = 64;                 // halo slots per wave
-  constexpr int kList = PerDir ? 1 : kHaloCap;  // halo list entries per wave
+  constexpr int kHaloCap = TileLds<G>::kHaloCap;  // halo slots per wave
   // shares: [g][A or B][cell] of the current direction, then the halo slots
   // [wave][slot][g]
   constexpr int kShares = G * 2 * kTileCells;
-  __shared__ float4 sShare[kShares + kTileWaves * kHaloCap * G];
+  float4* sShare = lds.share;
   // 1: the flux through the face is cancelled; [cell], then [wave][slot]
-  __shared__ unsigned char sCancel[kTileCells + kTileWaves * kHaloCap];
-  __shared__ unsigned char sItem[kTileWaves][kList];  // halo slot -> lane*4 + direction
-  __shared__ int sItemNei[kTileWaves][kList];         // halo slot -> nei_oct entry of the face
-  const int tile = xcd_block(a.nblocks);
-  const int lc = (int)threadIdx.x;
+  unsigned char* sCancel = lds.cancel;
+  auto& sItem = lds.item;         // halo slot -> lane*4 + direction
+  auto& sItemNei = lds.item_nei;  // halo slot -> nei_oct entry of the face
   const int wv = lc >> 6, lane = lc & 63;
   const int cell = tile * kTileCells + lc;
   const int oct0 = tile * kTileOcts;
@@ -941,14 +944,16 @@ template <int G, bool PerDir>
 __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4))) void level_step_tiled(
     LevelArgs<float> a_by_value, ChemConst K_by_value) {
   const KernArgs<float> args = kernargs<float>();
-  tiled_step<G, PerDir ? kPerDir : kListed>(args.a, args.K);
+  __shared__ TileLds<G> lds;
+  tiled_step<G, PerDir ? kPerDir : kListed>(args.a, args.K, xcd_block(args.a.nblocks), lds, (int)threadIdx.x);
 }
 
 template <int G>
 __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(2))) void level_step_tiled_latency(
     LevelArgs<float> a_by_value, ChemConst K_by_value) {
   const KernArgs<float> args = kernargs<float>();
-  tiled_step<G, kLatency>(args.a, args.K);
+  __shared__ TileLds<G> lds;
+  tiled_step<G, kLatency>(args.a, args.K, xcd_block(args.a.nblocks), lds, (int)threadIdx.x);
 }
 
 // ------------------------------------------------------------------ chemistry stragglers
@@ -1126,6 +1131,146 @@ __global__ void fill_kernel(unsigned int* p, int n, unsigned int v) {
   if (i < n) p[i] = v;
 }
 
+// ------------------------------------------------------------------ persistent coarse step
+// The schedule's read-only tables are read through the constant address
+// space: at a wave-uniform index the loads are scalar loads, and the level
+// step's arguments inside a StepDesc stay in SGPRs like kernel arguments.
+#define EMMA_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const T& cref(const void* base, int i) {
+  return *(const T*)((const EMMA_CONST T*)base + i);
+}
+
+__device__ __forceinline__ const SchedArgs& sched_args() {
+  return *reinterpret_cast<const SchedArgs*>((const char*)__builtin_amdgcn_kernarg_segment_ptr());
+}
+
+// wait until ticket dt is done in this launch (epoch), bounded by wall-clock
+// time: past the bound the wait is abandoned with EMMA_RT_ERR_SCHED, and
+// once that status is raised no other wait starts, so that every work-group
+// ends within about one bound
+__device__ __forceinline__ void wait_done(const SchedArgs& s, int dt, int epoch) {
+  if (__hip_atomic_load(s.done + dt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) return;
+  const long long t0 = (long long)__builtin_amdgcn_s_memrealtime();
+  while (__hip_atomic_load(s.done + dt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+    if (__hip_atomic_load(s.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == kErrSched) return;
+    __builtin_amdgcn_s_sleep(2);
+    if ((long long)__builtin_amdgcn_s_memrealtime() - t0 > s.wait_ticks) {
+      raise_status(s.status, kErrSched, -1, dt);
+      return;
+    }
+  }
+}
+
+template <int G>
+__device__ __forceinline__ TileLds<G>* tile_lds() {
+  __shared__ TileLds<G> lds;
+  return &lds;
+}
+
+// one item: FillRad's block on the tile's cells, the tile's level step, the
+// hydro injection of its cells.  Each thread fills and injects the cell it
+// steps, so the three need no barrier between them; neighbours' cells are
+// read only for their N, F and split flags, which FillRad does not touch.
+// The thread index as a value the optimiser cannot see through: otherwise
+// every address and mask the item derives from it alone is hoisted out of
+// the work-group's ticket loop and held in registers across all items.
+__device__ __forceinline__ int opaque_thread() {
+  int t = (int)threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
+template <typename R, int G, bool Tiled, bool Coupled>
+__device__ __forceinline__ void run_item(const StepDesc<R>& D, int tile) {
+  const int lc = opaque_thread();
+  const int cell = tile * kTileCells + lc;
+  if constexpr (Coupled)
+    if (D.fill && cell < D.f.ncell) fill_cell<R>(D.f, cell);
+  if constexpr (Tiled) {
+    if (D.variant == kListed)
+      tiled_step<G, kListed>(D.a, D.K, tile, *tile_lds<G>(), lc);
+    else
+      tiled_step<G, kPerDir>(D.a, D.K, tile, *tile_lds<G>(), lc);
+  } else {
+    cell_step<R, G, sizeof(R) == 4>(D.a, D.K, tile, lc);
+  }
+  if constexpr (Coupled)
+    if (D.hydro && cell < D.h.ncell) to_hydro_cell(D.h, cell);
+}
+
+// Work-groups loop over tickets: take the next one, wait for the tickets it
+// depends on (lanes of wave 0, one dependency each), acquire at agent scope
+// (their results may sit in another XCD's L2), run the item, release at
+// agent scope, mark the ticket done.  One wave of the work-group does the
+// acquire and the release for all four.  Items on the schedule's critical path
+// run at a raised wave priority so that the bandwidth-bound items sharing
+// their SIMDs do not stretch the chain.  The last work-group to leave
+// re-arms the queue (ticket 0, next epoch) for the next launch.  Coupled:
+// FillRad's block and the hydro injection are compiled in (without them the
+// kernel needs fewer registers).
+//
+// The work-group's own bookkeeping (ticket, done flag, exit count) is done
+// by all of wave 0 under a wave-uniform branch, never by one lane: a
+// lane-divergent block at each end of a loop that holds barriers lets the
+// compiler split the loop by lane (thread 0 looping over tickets, the other
+// lanes over the old ticket), which never ends.
+template <typename R, int G, bool Tiled, bool Coupled>
+__global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(Tiled ? 4 : 1))) void persistent_step_kernel(
+    SchedArgs s_by_value) {
+  const SchedArgs& s = sched_args();
+  __shared__ int sTicket, sEpoch;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int lane = (int)threadIdx.x & 63;
+  if (wave == 0) sEpoch = __hip_atomic_load(s.ctl + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int epoch = 0;
+  for (;;) {
+    if (wave == 0) {  // lane 0 adds one, the other lanes nothing
+      const int old = atomicAdd(s.ctl, lane == 0 ? 1 : 0);
+      sTicket = __builtin_amdgcn_readfirstlane(old);
+    }
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(sTicket);
+    epoch = __builtin_amdgcn_readfirstlane(sEpoch);
+    if (t >= s.nitems) break;
+    const int d0 = cref<int>(s.dep_off, t), d1 = cref<int>(s.dep_off, t + 1);
+    if (d1 > d0) {
+      if (wave == 0) {
+        for (int i = d0 + lane; i < d1; i += 64) wait_done(s, s.deps[i], epoch);
+        // one acquire for the work-group: its waves share this CU's L1,
+        // which the acquire invalidates along with the XCD's L2
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      }
+      __syncthreads();
+    }
+    const int step = cref<int>(s.item_step, t), tile = cref<int>(s.item_tile, t);
+    const int ntile = cref<int>(s.item_ntile, t);
+    const int prio = cref<unsigned char>(s.item_prio, t);
+    if (prio) __builtin_amdgcn_s_setprio(2);
+    for (int k = 0; k < ntile; k++) {
+      if (k > 0) __syncthreads();  // the tile step's LDS is reused
+      run_item<R, G, Tiled, Coupled>(cref<StepDesc<R>>(s.steps, step), tile + k);
+    }
+    if (prio) __builtin_amdgcn_s_setprio(0);
+    // every wave's stores have reached L2 before wave 0 writes the L2 back
+    // (release at agent scope) and publishes the ticket
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (wave == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_store(s.done + t, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  if (wave == 0) {
+    const int left = __builtin_amdgcn_readfirstlane(atomicAdd(s.ctl + 1, lane == 0 ? 1 : 0));
+    if (left == (int)gridDim.x - 1) {
+      s.ctl[0] = 0;
+      s.ctl[1] = 0;
+      s.ctl[2] = epoch + 1;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 template <typename R>
 hipError_t launch_level_step(const LevelArgs<R>& a0, const ChemConst& K, hipStream_t s) {
@@ -1176,6 +1321,38 @@ hipError_t launch_level_step(const LevelArgs<R>& a0, const ChemConst& K, hipStre
   return hipGetLastError();
 }
 
+template <typename R, int G, bool Tiled>
+static void* persistent_kernel_of(bool coupled) {
+  return coupled ? reinterpret_cast<void*>(persistent_step_kernel<R, G, Tiled, true>)
+                 : reinterpret_cast<void*>(persistent_step_kernel<R, G, Tiled, false>);
+}
+template <typename R>
+static void* persistent_kernel(int ng, bool coupled) {
+  if constexpr (sizeof(R) == 4) {
+    if (ng == 1) return persistent_kernel_of<float, 1, true>(coupled);
+    if (ng == 3) return persistent_kernel_of<float, 3, true>(coupled);
+    return persistent_kernel_of<float, 0, false>(coupled);
+  } else {
+    if (ng == 1) return persistent_kernel_of<double, 1, false>(coupled);
+    if (ng == 3) return persistent_kernel_of<double, 3, false>(coupled);
+    return persistent_kernel_of<double, 0, false>(coupled);
+  }
+}
+
+template <typename R>
+int persistent_blocks_per_cu(int ng, bool coupled) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, persistent_kernel<R>(ng, coupled), kTileCells, 0) != hipSuccess)
+    return 0;
+  return n;
+}
+
+template <typename R>
+hipError_t launch_persistent(const SchedArgs& s, int ng, bool coupled, int nblocks, hipStream_t st) {
+  void* args[] = {const_cast<SchedArgs*>(&s)};
+  return hipLaunchKernel(persistent_kernel<R>(ng, coupled), dim3(nblocks), dim3(kTileCells), args, 0, st);
+}
+
 template <typename R>
 hipError_t launch_photon_sum(const R* e, const int* child, int ncell, int ng, double vol, double* out,
                              hipStream_t s) {
@@ -1193,6 +1370,10 @@ hipError_t launch_fill(unsigned int* p, int n, unsigned int v, hipStream_t s) {
 
 template hipError_t launch_level_step<double>(const LevelArgs<double>&, const ChemConst&, hipStream_t);
 template hipError_t launch_level_step<float>(const LevelArgs<float>&, const ChemConst&, hipStream_t);
+template int persistent_blocks_per_cu<double>(int, bool);
+template int persistent_blocks_per_cu<float>(int, bool);
+template hipError_t launch_persistent<double>(const SchedArgs&, int, bool, int, hipStream_t);
+template hipError_t launch_persistent<float>(const SchedArgs&, int, bool, int, hipStream_t);
 template hipError_t launch_photon_sum<double>(const double*, const int*, int, int, double, double*, hipStream_t);
 template hipError_t launch_photon_sum<float>(const float*, const int*, int, int, double, double*, hipStream_t);
 

@@ -1,0 +1,67 @@
+// rt_sched.h -- the host-side planner of the persistent coarse step.
+//
+// A coarse step is a sequence of level steps (the RADSTEP recursion of
+// advanceamr.c:243-1300, or one step per level in COARSERAD mode).  The
+// persistent kernel runs it as items: one tile (32 octs) of one level step.
+// The planner gives every item the earlier items it must wait for and orders
+// the items into the ticket sequence the device work queue hands out.
+//
+// Dependencies of item (step i of level k, tile t), each on the most recent
+// earlier step of the named level (none when that level has not stepped yet
+// in the launch; the launch boundary orders everything before it):
+//  * level k: the tiles holding t's octs and their same-level face
+//    neighbours.  Their previous step wrote the state t reads (t's cells and
+//    the neighbours' N, F), and read the buffer t is about to overwrite.
+//  * level k-1: the tiles holding the parent cells of t's octs and the coarse
+//    leaves injected across t's coarse faces.  t reads their state
+//    (injection), adds refluxes into those leaves' accumulators, which their
+//    step consumed and cleared, and overwrites cells their restriction read.
+//  * level k+1: the tiles holding the children of t's split cells and the
+//    finer octs across t's leaves' faces.  t restricts from the children and
+//    takes the finer octs' refluxes.
+// Every conflicting pair of accesses in the sequential order is ordered by a
+// chain of these edges (the CPU test checks it against the accesses the
+// kernels make).
+//
+// Order: a list schedule over `slots` work-groups with unit-free tile costs
+// per level, ready items taken longest-remaining-chain first; the tickets
+// follow the simulated start times.  Items whose chain reaches the simulated
+// end get a raised wave priority.
+#pragma once
+
+#include <vector>
+
+namespace emma {
+
+struct SchedLevelTopo {
+  int noct = 0;
+  const int* nei = nullptr;    // [noct][6]: >= 0 same-level oct, < 0 coarse leaf -1-nei or a domain face
+  const int* child = nullptr;  // [noct][8]: >= 0 child oct on the next level, < 0 leaf
+};
+
+struct SchedPlan {
+  int nitems = 0;
+  std::vector<int> item_step, item_tile, item_ntile;  // by ticket: level step, first tile, tiles
+  std::vector<unsigned char> item_prio;
+  std::vector<int> dep_off, deps;         // by ticket; deps are tickets
+  double est_span = 0., est_chain = 0., est_work = 0.;  // simulated makespan, longest chain, total cost
+};
+
+constexpr int kSchedTileOcts = 32;
+
+// tiles per item of a level step with ntiles tiles: one on levels of up to
+// 4 x slots tiles, else enough to give the step about 2 x slots items (at
+// most 64), so that the per-item synchronisation stays small against the
+// work on the large levels while the small levels keep one tile per item
+int chunk_tiles(int ntiles, int slots);
+
+// steps[i]: level index (0 = coarsest) of the i-th level step, in the
+// sequential order; tile_cost[k]: relative cost of one tile step of level k
+// (<= 0: 1); slots: work-groups of the persistent grid; chunk[k] (optional,
+// > 0): tiles per item of level k instead of chunk_tiles.  Returns 0, or -1
+// on bad input.
+int plan_schedule(const std::vector<SchedLevelTopo>& lev, const std::vector<int>& steps,
+                  const std::vector<double>& tile_cost, int slots, SchedPlan& out,
+                  const std::vector<int>* chunk = nullptr);
+
+}  // namespace emma

@@ -40,6 +40,7 @@ ERR_NEIGHBOR = -6
 ERR_LAYOUT = -7
 ERR_NODEVICE = -8
 ERR_STATE = -9
+ERR_SCHED = -10
 
 FP64 = 0
 MIXED = 1
@@ -57,6 +58,10 @@ VARIANT_PERDIR = 1
 VARIANT_LATENCY = 2
 VARIANT_NAMES = {VARIANT_LISTED: "listed", VARIANT_PERDIR: "perdir", VARIANT_LATENCY: "latency"}
 
+SCHED_GRAPH = 0
+SCHED_PERSISTENT = 1
+SCHED_NAMES = {SCHED_GRAPH: "graph", SCHED_PERSISTENT: "persistent"}
+
 EXPORTS = [
     "emma_rt_countdevices", "emma_rt_initlocaldevice", "emma_rt_checkdevice", "emma_rt_strerror",
     "emma_rt_create", "emma_rt_set_params", "emma_rt_destroy",
@@ -70,6 +75,7 @@ EXPORTS = [
     "emma_rt_save_state", "emma_rt_restore_state", "emma_rt_set_stars", "emma_rt_set_star_sources",
     "emma_rt_set_chem_budget", "emma_rt_get_chem_budget", "emma_rt_set_level_variant", "emma_rt_get_level_tuning",
     "emma_rt_advance_stencil_noalloc", "emma_rt_set_chem_keep", "emma_rt_dump_grid", "emma_rt_grid_field",
+    "emma_rt_set_schedule", "emma_rt_get_schedule", "emma_rt_plan_schedule",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -89,6 +95,21 @@ class Stats(ctypes.Structure):
         ("bad_cell", ctypes.c_int),
         ("timed_spans", ctypes.c_int),
         ("deferred_cells", ctypes.c_longlong),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SchedInfo(ctypes.Structure):
+    _fields_ = [
+        ("items", ctypes.c_longlong),
+        ("deps", ctypes.c_longlong),
+        ("blocks", ctypes.c_int),
+        ("launches", ctypes.c_int),
+        ("est_span", ctypes.c_double),
+        ("est_chain", ctypes.c_double),
+        ("est_work", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -164,8 +185,53 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
                                     ctypes.c_int, _ip]
     L.emma_rt_grid_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
                                      ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)]
+    L.emma_rt_set_schedule.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.emma_rt_get_schedule.argtypes = [ctypes.c_void_p, _ip, ctypes.POINTER(SchedInfo)]
+    _llp = ctypes.POINTER(ctypes.c_longlong)
+    L.emma_rt_plan_schedule.argtypes = [ctypes.c_int, _ip, _ip, _ip, ctypes.c_int, _ip, _dp, _ip, ctypes.c_int,
+                                        ctypes.c_longlong, ctypes.c_longlong, _llp, _llp, _ip, _ip, _ip,
+                                        ctypes.POINTER(ctypes.c_ubyte), _ip, _ip, _dp]
     _lib = L
     return L
+
+
+def plan_schedule(tree: T.Tree, steps, slots: int, tile_cost=None, chunk=None) -> dict:
+    """The persistent coarse step's planner (host code, no GPU): for the
+    level indices `steps` (0 = coarsest, sequential order) of `tree`, the
+    items by ticket (level step, first tile, tiles, wave priority) and the
+    tickets each waits for."""
+    L = load()
+    lv = [tree.level(l) for l in range(tree.lmin, tree.lmax + 1)]
+    noct = np.array([x.noct for x in lv], dtype=np.int32)
+    nei = np.ascontiguousarray(np.concatenate([x.nei_oct.reshape(-1) for x in lv]).astype(np.int32))
+    child = np.ascontiguousarray(np.concatenate([x.child.reshape(-1) for x in lv]).astype(np.int32))
+    st = np.ascontiguousarray(np.asarray(steps, dtype=np.int32))
+    cost = None if tile_cost is None else np.ascontiguousarray(np.asarray(tile_cost, dtype=np.float64))
+    cs = None if chunk is None else np.ascontiguousarray(np.asarray(chunk, dtype=np.int32))
+    ni, nd = ctypes.c_longlong(0), ctypes.c_longlong(0)
+    ip = lambda a: a.ctypes.data_as(_ip)
+    args0 = (len(lv), ip(noct), ip(nei), ip(child), len(st), ip(st), None if cost is None else _d(cost),
+             None if cs is None else ip(cs), int(slots))
+    r = L.emma_rt_plan_schedule(*args0, 0, 0, ctypes.byref(ni), ctypes.byref(nd), None, None, None, None, None,
+                                None, None)
+    if r != OK:
+        raise EmmaRTError(r, strerror(r))
+    n, m = ni.value, nd.value
+    item_step = np.zeros(n, np.int32)
+    item_tile = np.zeros(n, np.int32)
+    item_ntile = np.zeros(n, np.int32)
+    item_prio = np.zeros(n, np.uint8)
+    dep_off = np.zeros(n + 1, np.int32)
+    deps = np.zeros(max(m, 1), np.int32)
+    est = np.zeros(3)
+    r = L.emma_rt_plan_schedule(*args0, n, max(m, 1), ctypes.byref(ni), ctypes.byref(nd), ip(item_step),
+                                ip(item_tile), ip(item_ntile), item_prio.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)),
+                                ip(dep_off), ip(deps), _d(est))
+    if r != OK:
+        raise EmmaRTError(r, strerror(r))
+    return {"item_step": item_step, "item_tile": item_tile, "item_ntile": item_ntile, "item_prio": item_prio,
+            "dep_off": dep_off,
+            "deps": deps[:m], "est_span": est[0], "est_chain": est[1], "est_work": est[2]}
 
 
 def strerror(status: int) -> str:
@@ -407,6 +473,19 @@ class RadGPU:
     def coarse_step(self, aexp: float = 1.0, ncoarse: int = 1):
         st = self.lib.emma_rt_coarse_step(self.h, aexp, ncoarse)
         self._check(st, self.stats() if st < 0 and st != ERR_HIP else None)
+
+    def set_schedule(self, sched: int):
+        """coarse-step schedule: SCHED_GRAPH (one launch per level step, in a
+        captured graph) or SCHED_PERSISTENT (one persistent launch fed by a
+        device work queue)"""
+        self._check(self.lib.emma_rt_set_schedule(self.h, int(sched)))
+
+    def schedule(self) -> dict:
+        s, info = ctypes.c_int(0), SchedInfo()
+        self._check(self.lib.emma_rt_get_schedule(self.h, ctypes.byref(s), ctypes.byref(info)))
+        d = info.as_dict()
+        d["sched"] = SCHED_NAMES.get(s.value, s.value)
+        return d
 
     def stats(self) -> dict:
         s = Stats()

@@ -53,6 +53,7 @@ extern "C" {
 #define EMMA_RT_ERR_LAYOUT (-7)     /* host struct layout differs from the library's */
 #define EMMA_RT_ERR_NODEVICE (-8)   /* no GPU visible */
 #define EMMA_RT_ERR_STATE (-9)      /* call out of order (no tree / no stencil buffer) */
+#define EMMA_RT_ERR_SCHED (-10)     /* persistent coarse step: a dependency wait passed its time bound */
 
 /* level-stepping mode of the resident path (compile-time COARSERAD flag of
    the reference, prototypes.h:81-83) */
@@ -371,6 +372,57 @@ int emma_rt_dump_grid(emma_rt_ctx *ctx, const char *folder, int ndump, int rank,
    host writing its own files, e.g. dump_HDF5_grid's per-field dataset "data"
    (io.c:1495-1611) with its own HDF5 library. */
 int emma_rt_grid_field(emma_rt_ctx *ctx, int field_id, int rank, float *out, long long capacity, long long *n_out);
+
+/* ---- coarse-step schedule.  No counterpart in the reference, whose
+   Advance_level recursion (advanceamr.c:243-1300, 1309-1537) launches
+   advanceradGPU level by level from the host.  EMMA_RT_SCHED_GRAPH (the
+   default): emma_rt_coarse_step replays a captured HIP graph of one launch per
+   level step.  EMMA_RT_SCHED_PERSISTENT: every run of consecutive level steps
+   (all of a RADSTEP coarse step; in COARSERAD mode the finer levels' steps,
+   then the coarse level's, with the transfers between them) is ONE launch of
+   persistent work-groups fed by a device work queue.  An item is one tile
+   (32 octs) of one level step, with FillRad's block before and the hydro
+   injection after it (several consecutive tiles per item on the largest
+   levels); items wait only for the tiles whose results they read
+   or whose inputs they overwrite, so tiles of different levels and steps run
+   at once (a small level's latency chain beside the large levels' tiles).
+   The queue's order comes from a list schedule the host computes once per
+   tree; results are identical bit for bit to the graph schedule's.  The
+   chemistry straggler pass (emma_rt_set_chem_budget) is not used inside the
+   persistent kernel, and the latency halo variant runs as per-direction.
+   Per-level device times are not available in this schedule: with
+   EMMA_RT_TIMING_LAUNCH each persistent launch is one timed span. */
+#define EMMA_RT_SCHED_GRAPH 0
+#define EMMA_RT_SCHED_PERSISTENT 1
+typedef struct emma_rt_sched_info {
+  long long items;   /* tile items of the last planned coarse-step call */
+  long long deps;    /* dependency edges among them */
+  int blocks;        /* persistent work-groups per launch */
+  int launches;      /* persistent launches per call */
+  double est_span;   /* list-schedule makespan, in tile steps */
+  double est_chain;  /* longest dependency chain, in tile steps */
+  double est_work;   /* total work, in tile steps */
+} emma_rt_sched_info;
+int emma_rt_set_schedule(emma_rt_ctx *ctx, int sched);
+/* the schedule and (sched_info of the last coarse-step call; zero for the
+   graph schedule) */
+int emma_rt_get_schedule(emma_rt_ctx *ctx, int *sched, emma_rt_sched_info *info);
+/* The planner alone (host code, no device needed): the tree as for
+   emma_rt_tree_create (nlev levels, coarsest first), the level index of every
+   level step in sequential order, a relative cost per tile step of each level
+   (NULL: 1), the tiles per item of each level (NULL: the library's rule, one
+   tile up to 4 x slots tiles per level, else about 2 x slots items per level
+   step) and the work-group count.  Writes nitems / ndeps; with cap_items == 0
+   returns after that (sizes only), else fills, by ticket, the level step,
+   first tile and tile count of each item, its wave priority, and the tickets
+   it waits for (deps[dep_off[t] .. dep_off[t+1])), and est[3] = makespan,
+   longest chain, total work (may be NULL).  EMMA_RT_ERR_ARG on bad input or
+   short buffers. */
+int emma_rt_plan_schedule(int nlev, const int *noct, const int *nei_oct, const int *child, int nsteps,
+                          const int *step_level, const double *tile_cost, const int *chunk, int slots,
+                          long long cap_items, long long cap_deps, long long *nitems, long long *ndeps,
+                          int *item_step, int *item_tile, int *item_ntile, unsigned char *item_prio, int *dep_off,
+                          int *deps, double *est);
 
 int emma_rt_get_stats(emma_rt_ctx *ctx, emma_rt_stats *out);
 /* device timing into emma_rt_stats.kernel_ms: EMMA_RT_TIMING_LAUNCH puts
