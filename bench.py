@@ -264,7 +264,10 @@ def run_uniform(W, precision, steps, warmup, tuning=None):
 
 def run_amr(W, precision, steps, warmup, tuning=None, tune_here=False, log=None):
     """A step is one coarse step of the RADSTEP level recursion (level l takes
-    2^(l-lmin) level steps), one replay of its captured HIP graph.  Every
+    2^(l-lmin) level steps), one replay of its captured HIP graph: per-level
+    launches, or one persistent launch fed by a device work queue (the
+    schedule the tuning table holds for the tree, or the tuner picks; the
+    other schedule is timed after the timed region for the report).  Every
     timed coarse step is the second coarse step of the run: on these seeded
     inputs dense neutral cells drain to the EMIN floor, where the reference
     path aborts within a few coarse steps (chem_utils.c:382-385; the oracle
@@ -280,12 +283,14 @@ def run_amr(W, precision, steps, warmup, tuning=None, tune_here=False, log=None)
     g.coarse_step(1.0, 1)  # first coarse step, from the drawn state
     g.save_state()
     restore = g.restore_state
-    tuned = None
+    tuned = sched_tuned = None
     if tune_here and precision == "mixed" and W.params.ngrp in (1, 3):
         from emma_amd import tune
 
         tuned = tune.tune(g, reps=3, log=log)  # leaves the context set to the fastest per level
         used = _configure(g, W, precision, None)
+        sched_tuned = tune.tune_schedule(g, reps=5, log=log)  # and on the faster schedule
+    sched = g.schedule()["sched"]
 
     # capture + instantiate the graphs of both buffer parities of the coarse level
     for _ in range(max(warmup, 2)):
@@ -308,7 +313,28 @@ def run_amr(W, precision, steps, warmup, tuning=None, tune_here=False, log=None)
         restore()
         g.coarse_step(1.0, 1)
         graph_ms += g.stats()["kernel_ms"]
-    # and launched eagerly with events around every kernel (kernel durations)
+    sched_info = g.schedule()
+    schedules = {sched: graph_ms / steps}
+    # the other schedule, timed the same way (not the value)
+    other = rt.SCHED_PERSISTENT if sched == "graph" else rt.SCHED_GRAPH
+    g.enable_timing(rt.TIMING_OFF)
+    g.set_schedule(other)
+    for _ in range(2):
+        restore()
+        g.coarse_step(1.0, 1)
+    g.enable_timing(rt.TIMING_GRAPH)
+    oms = 0.0
+    for _ in range(steps):
+        restore()
+        g.coarse_step(1.0, 1)
+        oms += g.stats()["kernel_ms"]
+    schedules[rt.SCHED_NAMES[other]] = oms / steps
+    if other == rt.SCHED_PERSISTENT:
+        sched_info = g.schedule()
+    # and launched eagerly with events around every kernel (kernel durations;
+    # per-level launches)
+    g.enable_timing(rt.TIMING_OFF)
+    g.set_schedule(rt.SCHED_GRAPH)
     g.enable_timing(rt.TIMING_LAUNCH)
     acc = {}
     nrep = max(1, min(steps, 5))
@@ -324,8 +350,11 @@ def run_amr(W, precision, steps, warmup, tuning=None, tune_here=False, log=None)
     launches = {l: 1 << (l - W.tree.lmin) for l in level_us}
     lb = level_bytes(W, precision)
     per_coarse = sum(b << (l - W.tree.lmin) for l, b in lb.items())
+    sched_info.pop("sched", None)
     return {"wall": wall, "stats": tot, "graph_ms": graph_ms, "level_us": level_us, "launches_per_step": launches,
-            "steps": steps, "alg_bytes_per_step": per_coarse, "tuning": used, "tuned": tuned}
+            "steps": steps, "alg_bytes_per_step": per_coarse, "tuning": used, "tuned": tuned,
+            "schedule": sched, "schedules_ms": schedules, "persistent_plan": sched_info,
+            "sched_tuned": sched_tuned}
 
 
 def measure(name, W, precision, steps, warmup, tuning=None, tune_here=False):
@@ -351,6 +380,10 @@ def measure(name, W, precision, steps, warmup, tuning=None, tune_here=False):
         "achieved": achieved,
         "levels": _level_table(W, precision, r["level_us"], r["launches_per_step"], r["tuning"]),
         "tuned": r.get("tuned"),
+        "sched_tuned": r.get("sched_tuned"),
+        "schedule": r.get("schedule"),
+        "schedules_ms": r.get("schedules_ms"),
+        "persistent_plan": r.get("persistent_plan"),
     }
 
 
@@ -475,10 +508,13 @@ def main():
           f"per-level tuning: {tuning_src}", file=sys.stderr, flush=True)
     _, m = measure(args.workload, W, precision, args.steps, args.warmup, table, bool(args.tune))
     if args.tune and m["tuned"]:
-        tune.save(m["tuned"], WORKLOAD_NAMES[args.workload], args.tune)
-    step = {"c4": "coarse step: 31 level steps (levels 7-11), one HIP graph replay",
-            "c5": "coarse step: 7 level steps (levels 8-10), one HIP graph replay"}.get(args.workload,
-                                                                                    "radiative sub-cycle")
+        tune.save(m["tuned"], WORKLOAD_NAMES[args.workload], args.tune, m["sched_tuned"])
+    sched_txt = {"graph": "one HIP graph replay of per-level launches",
+                 "persistent": "one HIP graph replay of one persistent launch fed by a device work queue"}
+    step = {"c4": "coarse step: 31 level steps (levels 7-11), ",
+            "c5": "coarse step: 7 level steps (levels 8-10), "}.get(args.workload, "radiative sub-cycle")
+    if args.workload in ("c4", "c5"):
+        step += sched_txt[m["schedule"]]
     out = {
         "metric": "M1+chem cell-updates/s (mixed+batched vs FP64 plain, 1 GPU) and % HBM peak",
         "value": m["value"],
@@ -504,6 +540,9 @@ def main():
             "chem_substeps_per_cell_update": m["chem_substeps_per_cell_update"],
             "levels": m["levels"],
             "tuning": tuning_src,
+            "schedule": m["schedule"],
+            "schedules_device_ms_per_step": m["schedules_ms"],
+            "persistent_plan": m["persistent_plan"],
             "parallelism": "1 GPU",
         },
         "roofline": {
@@ -535,8 +574,10 @@ def main():
         print(f"bench: c4 block, {n4} steps", file=sys.stderr, flush=True)
         _, m4 = measure("c4", W4, precision, n4, min(args.warmup, 3), table, bool(args.tune))
         if args.tune and m4["tuned"]:
-            tune.save(m4["tuned"], WORKLOAD_NAMES["c4"], args.tune)
-        out["c4"] = {"workload": WORKLOAD_NAMES["c4"], "step": "coarse step: 31 level steps, one HIP graph replay",
+            tune.save(m4["tuned"], WORKLOAD_NAMES["c4"], args.tune, m4["sched_tuned"])
+        out["c4"] = {"workload": WORKLOAD_NAMES["c4"], "step": "coarse step: 31 level steps, " + sched_txt[m4["schedule"]],
+                     "schedule": m4["schedule"], "schedules_device_ms_per_step": m4["schedules_ms"],
+                     "persistent_plan": m4["persistent_plan"],
                      "value": m4["value"], "unit": "cell-updates/s", "steps": n4, "ms_per_step": m4["ms_per_step"],
                      "device_ms_per_step": m4["device_ms_per_step"], "step_alg_frac": m4["step_alg_frac"],
                      "kernel": m4["kernel"], "level": m4["dominant_level"], "kernel_us": m4["kernel_us"],

@@ -15,9 +15,12 @@ per-level choices are
 which change the speed and not the results.  :func:`tune` times every
 candidate on the workload's own state (event-timed level steps of repeated
 coarse steps from one saved device state) and keeps, per level, the fastest.
+For the whole tree, :func:`tune_schedule` picks the coarse-step schedule:
+the captured graph of per-level launches or the persistent kernel fed by a
+device work queue (``emma_rt_set_schedule``; the same results bit for bit).
 The table (:func:`save` / :func:`load`) is keyed by the level's shape --
-groups, cells, complete or sparse -- and :func:`apply` sets a context's
-levels from it.
+groups, cells, complete or sparse -- and by the tree's shape for the
+schedule, and :func:`apply` sets a context from it.
 """
 from __future__ import annotations
 
@@ -33,6 +36,42 @@ KEEPS = (0.0, 8.0, 32.0)
 
 def level_key(ngrp: int, ncell: int, halo_per_dir: bool) -> str:
     return f"G{ngrp}-{'sparse' if halo_per_dir else 'complete'}-{ncell}"
+
+
+def tree_key(ngrp: int, tree) -> str:
+    return f"tree-G{ngrp}-" + "-".join(str(tree.level(l).ncell) for l in range(tree.lmin, tree.lmax + 1))
+
+
+def _step_ms(g, reps):
+    """mean device time (ms) of one coarse step (one event pair around each
+    replay) over `reps` coarse steps from the saved state, after two untimed
+    ones (the graphs of both buffer parities)"""
+    for _ in range(2):
+        g.restore_state()
+        g.coarse_step(1.0, 1)
+    g.enable_timing(rt.TIMING_GRAPH)
+    ms = 0.0
+    for _ in range(reps):
+        g.restore_state()
+        g.coarse_step(1.0, 1)
+        ms += g.stats()["kernel_ms"]
+    g.enable_timing(rt.TIMING_OFF)
+    return ms / reps
+
+
+def tune_schedule(g: "rt.RadGPU", reps: int = 5, log=None) -> dict:
+    """The faster coarse-step schedule for the context's tree (its levels as
+    they are set), from a saved state; leaves the context on it."""
+    t = {}
+    for sched in (rt.SCHED_GRAPH, rt.SCHED_PERSISTENT):
+        g.set_schedule(sched)
+        t[sched] = _step_ms(g, reps)
+        if log:
+            log(f"tune: schedule {rt.SCHED_NAMES[sched]}: {t[sched]:.3f} ms per coarse step")
+    best = min(t, key=t.get)
+    g.set_schedule(best)
+    return {"key": tree_key(g.params.ngrp, g.tree), "sched": rt.SCHED_NAMES[best],
+            "ms": {rt.SCHED_NAMES[k]: round(v, 4) for k, v in t.items()}}
 
 
 def _level_times(g, levels, reps):
@@ -103,10 +142,13 @@ def load(path: str = TABLE) -> dict:
         return json.load(f)
 
 
-def save(result: dict, workload: str, path: str = TABLE) -> dict:
-    """merge a tune() result into the table at `path` (entries keyed by level
-    shape; `workload` names where they were measured)"""
+def save(result: dict, workload: str, path: str = TABLE, schedule: dict = None) -> dict:
+    """merge a tune() result (and a tune_schedule() result) into the table at
+    `path` (entries keyed by level / tree shape; `workload` names where they
+    were measured)"""
     table = load(path)
+    if schedule is not None:
+        table[schedule["key"]] = {"sched": schedule["sched"], "ms": schedule["ms"], "workload": workload}
     for l, r in result.items():
         table[r["key"]] = {"variant": r["variant"], "chem_budget": r["chem_budget"],
                            "chem_keep": r.get("chem_keep", 0.0), "us": round(r["us"], 2),
@@ -118,11 +160,15 @@ def save(result: dict, workload: str, path: str = TABLE) -> dict:
 
 
 def apply(g: "rt.RadGPU", table: dict = None) -> dict:
-    """set every level whose shape the table holds; returns {level: entry}"""
+    """set every level whose shape the table holds, and the tree's schedule
+    when the table holds the tree; returns {level: entry}"""
     if table is None:
         table = load()
     names = {v: k for k, v in rt.VARIANT_NAMES.items()}
     used = {}
+    e = table.get(tree_key(g.params.ngrp, g.tree))
+    if e is not None:  # the tree's schedule (g.schedule() reports it)
+        g.set_schedule({v: k for k, v in rt.SCHED_NAMES.items()}[e["sched"]])
     if g.precision != rt.MIXED or g.params.ngrp not in (1, 3):
         return used
     for l in range(g.tree.lmin, g.tree.lmax + 1):
@@ -138,3 +184,45 @@ def apply(g: "rt.RadGPU", table: dict = None) -> dict:
         g.set_chem_keep(float(e.get("chem_keep", 0.0)), level=l)
         used[l] = e
     return used
+
+
+def main(argv=None) -> int:
+    """python -m emma_amd.tune [--schedule-only] [--table PATH] [--out PATH] c4 c5 ...:
+    tune the named workloads (from the state after their first coarse step)
+    and merge the choices into the table"""
+    import argparse
+    import sys
+
+    from . import synth
+
+    ap = argparse.ArgumentParser(prog="python -m emma_amd.tune")
+    ap.add_argument("workloads", nargs="+", choices=["c4", "c5"])
+    ap.add_argument("--schedule-only", action="store_true",
+                    help="keep the table's per-level choices, tune the coarse-step schedule only")
+    ap.add_argument("--table", default=TABLE, help="table the per-level choices are read from")
+    ap.add_argument("--out", default=None, help="table to write (default: --table)")
+    ap.add_argument("--reps", type=int, default=5)
+    args = ap.parse_args(argv)
+    out = args.out or args.table
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    names = {"c4": "AMR levels 7-11 (262144/80000/25000/6000/1200 octs), 3 groups, lognormal sigma=1.5",
+             "c5": "256^3 coarse level + 2 refined levels (300000/75000 octs), 3 groups, 4096 sources, c/100, "
+                   "lognormal sigma=2"}
+    for w in args.workloads:
+        W = synth.config_c4(seed=4, nsub=1) if w == "c4" else synth.config_c5(seed=3, nsub=1)  # the bench's seeds
+        g = rt.RadGPU(W.params, rt.MIXED)
+        g.setup(W.params, W.tree, W.fields)
+        apply(g, load(args.table))
+        g.set_schedule(rt.SCHED_GRAPH)
+        g.coarse_step(1.0, 1)
+        g.save_state()
+        levels = {} if args.schedule_only else tune(g, reps=3, log=log)
+        sched = tune_schedule(g, reps=args.reps, log=log)
+        log(f"{w}: schedule {sched['sched']} {sched['ms']}")
+        save(levels, names[w], out, sched)
+        g.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
