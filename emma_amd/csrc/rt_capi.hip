@@ -35,6 +35,9 @@ static hipError_t& last_hip_error() {
   return e;
 }
 
+static_assert(emma::kItemPrio == emma::kSchedPrio && emma::kItemAcquireAgent == emma::kSchedAcquireAgent &&
+                  emma::kItemReleaseAgent == emma::kSchedReleaseAgent,
+              "item flags of the planner and the persistent kernel");
 static_assert(emma::kNeiTransmissive == EMMA_RT_NEI_TRANSMISSIVE && emma::kNeiReflective == EMMA_RT_NEI_REFLECTIVE,
               "domain-face codes of the kernels and the C-ABI");
 static_assert(emma::kListed == EMMA_RT_VARIANT_LISTED && emma::kPerDir == EMMA_RT_VARIANT_PERDIR &&
@@ -179,6 +182,8 @@ struct emma_rt_ctx {
   // persistent kernel per run of level steps (EMMA_RT_SCHED_PERSISTENT)
   int sched = EMMA_RT_SCHED_GRAPH;
   int ncu = 0;  // compute units of the device
+  int nxcc = 0;  // XCDs the work-groups run on (probed on first use), and their queue numbers
+  int xcc_queue[16];
   emma_rt_sched_info sched_info;
   // recording of a coarse step (persistent schedule): the helpers below
   // record instead of launching while rec is set; FillRad's per-cell block
@@ -1359,6 +1364,30 @@ static int recurse(emma_rt_ctx* c, int k, double aexp) {
 // transfers between runs).
 constexpr long long kSchedWaitTicks = 200000000LL;  // 2 s of s_memrealtime (100 MHz)
 
+// The XCDs the device's work-groups run on (the XCC_ID register, read by a
+// probe grid of 8 work-groups per CU): one queue of the persistent kernel
+// each, numbered in order of their ids.
+static int probe_xcds(emma_rt_ctx* c) {
+  if (c->nxcc > 0) return EMMA_RT_OK;
+  const int n = 8 * c->ncu;
+  int* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, (size_t)n * sizeof(int)));
+  std::vector<int> h(n, -1);
+  hipError_t e = launch_xcc_probe(d, n, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(h.data(), d, (size_t)n * sizeof(int), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  HIPCHK(e);
+  bool seen[16] = {false};
+  for (int x : h)
+    if (x >= 0 && x < 16) seen[x] = true;
+  int nq = 0;
+  for (int i = 0; i < 16; i++) c->xcc_queue[i] = seen[i] ? nq++ : -1;
+  if (nq == 0) return EMMA_RT_ERR_HIP;
+  c->nxcc = nq;
+  return EMMA_RT_OK;
+}
+
 static std::vector<SchedLevelTopo> sched_topo(emma_rt_ctx* c) {
   std::vector<SchedLevelTopo> t(c->lev.size());
   for (size_t k = 0; k < c->lev.size(); k++) {
@@ -1399,21 +1428,26 @@ static int build_persistent(emma_rt_ctx* c, const std::vector<SchedOp>& ops, Cac
     const int bpc = persistent_blocks_per_cu<R>(ng, coupled);
     if (bpc < 1) return EMMA_RT_ERR_HIP;
     const int blocks = bpc * c->ncu;
+    int st = probe_xcds(c);
+    if (st) return st;
     SchedPlan plan;
-    if (plan_schedule(topo, steps, cost, blocks, plan) != 0) return EMMA_RT_ERR_ARG;
+    const int nq = c->nxcc;
+    if (plan_schedule(topo, steps, cost, blocks, plan, nullptr, nq) != 0) return EMMA_RT_ERR_ARG;
     const size_t n = (size_t)plan.nitems, m = plan.deps.size();
     const size_t o_desc = 0, o_step = align256(o_desc + steps.size() * sizeof(StepDesc<R>));
     const size_t o_tile = align256(o_step + n * sizeof(int)), o_ntile = align256(o_tile + n * sizeof(int));
     const size_t o_prio = align256(o_ntile + n * sizeof(int));
     const size_t o_doff = align256(o_prio + n), o_deps = align256(o_doff + (n + 1) * sizeof(int));
     const size_t o_done = align256(o_deps + std::max<size_t>(m, 1) * sizeof(int));
-    const size_t o_ctl = align256(o_done + n * sizeof(int)), total = o_ctl + 4 * sizeof(int);
+    const size_t o_qoff = align256(o_done + n * sizeof(int)), o_ctl = align256(o_qoff + (nq + 1) * sizeof(int));
+    const size_t total = o_ctl + (nq + 2) * sizeof(int);
     char* buf = nullptr;
     HIPCHK(hipMalloc((void**)&buf, total));
     out.bufs.push_back(buf);
     std::vector<unsigned char> descs(steps.size() * sizeof(StepDesc<R>));
     for (size_t q = i; q < j; q++) memcpy(descs.data() + (q - i) * sizeof(StepDesc<R>), ops[q].blob.data(), sizeof(StepDesc<R>));
-    const int ctl0[4] = {0, 0, 1, 0};  // next ticket, work-groups that left, epoch
+    std::vector<int> ctl0(nq + 2, 0);  // queues' next tickets, work-groups that left, epoch
+    ctl0[nq + 1] = 1;
     HIPCHK(hipMemcpy(buf + o_desc, descs.data(), descs.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(buf + o_step, plan.item_step.data(), n * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(buf + o_tile, plan.item_tile.data(), n * sizeof(int), hipMemcpyHostToDevice));
@@ -1422,7 +1456,8 @@ static int build_persistent(emma_rt_ctx* c, const std::vector<SchedOp>& ops, Cac
     HIPCHK(hipMemcpy(buf + o_doff, plan.dep_off.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
     if (m > 0) HIPCHK(hipMemcpy(buf + o_deps, plan.deps.data(), m * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(buf + o_done, 0, n * sizeof(int)));
-    HIPCHK(hipMemcpy(buf + o_ctl, ctl0, sizeof(ctl0), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(buf + o_qoff, plan.qoff.data(), (nq + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(buf + o_ctl, ctl0.data(), ctl0.size() * sizeof(int), hipMemcpyHostToDevice));
     SchedLaunch l;
     l.kind = 0;
     l.blocks = blocks;
@@ -1431,13 +1466,15 @@ static int build_persistent(emma_rt_ctx* c, const std::vector<SchedOp>& ops, Cac
     l.args.item_step = (const int*)(buf + o_step);
     l.args.item_tile = (const int*)(buf + o_tile);
     l.args.item_ntile = (const int*)(buf + o_ntile);
-    l.args.item_prio = (const unsigned char*)(buf + o_prio);
+    l.args.item_flags = (const unsigned char*)(buf + o_prio);
+    l.args.qoff = (const int*)(buf + o_qoff);
     l.args.dep_off = (const int*)(buf + o_doff);
     l.args.deps = (const int*)(buf + o_deps);
     l.args.done = (int*)(buf + o_done);
     l.args.ctl = (int*)(buf + o_ctl);
     l.args.nitems = (int)n;
-    l.args.pad = 0;
+    l.args.nq = nq;
+    for (int i = 0; i < 16; i++) l.args.xcc_queue[i] = c->xcc_queue[i];
     l.args.status = c->d_status;
     l.args.wait_ticks = kSchedWaitTicks;
     out.launches.push_back(std::move(l));
@@ -1618,9 +1655,10 @@ extern "C" int emma_rt_get_schedule(emma_rt_ctx* c, int* sched, emma_rt_sched_in
 
 extern "C" int emma_rt_plan_schedule(int nlev, const int* noct, const int* nei_oct, const int* child, int nsteps,
                                      const int* step_level, const double* tile_cost, const int* chunk, int slots,
-                                     long long cap_items, long long cap_deps, long long* nitems, long long* ndeps,
-                                     int* item_step, int* item_tile, int* item_ntile, unsigned char* item_prio,
-                                     int* dep_off, int* deps, double* est) {
+                                     int nqueues, long long cap_items, long long cap_deps, long long* nitems,
+                                     long long* ndeps, int* qoff, int* item_step, int* item_tile, int* item_ntile,
+                                     unsigned char* item_flags, int* item_rank, int* dep_off, int* deps,
+                                     double* est) {
   if (nlev < 1 || nlev > kMaxLevels || noct == nullptr || nei_oct == nullptr || child == nullptr || nsteps < 0 ||
       (nsteps > 0 && step_level == nullptr) || nitems == nullptr || ndeps == nullptr)
     return EMMA_RT_ERR_ARG;
@@ -1641,7 +1679,8 @@ extern "C" int emma_rt_plan_schedule(int nlev, const int* noct, const int* nei_o
   std::vector<int> cs;
   if (chunk) cs.assign(chunk, chunk + nlev);
   SchedPlan plan;
-  if (plan_schedule(topo, steps, cost, slots, plan, chunk ? &cs : nullptr) != 0) return EMMA_RT_ERR_ARG;
+  if (nqueues < 1 || nqueues > 64) return EMMA_RT_ERR_ARG;
+  if (plan_schedule(topo, steps, cost, slots, plan, chunk ? &cs : nullptr, nqueues) != 0) return EMMA_RT_ERR_ARG;
   *nitems = plan.nitems;
   *ndeps = (long long)plan.deps.size();
   if (est) {
@@ -1653,7 +1692,9 @@ extern "C" int emma_rt_plan_schedule(int nlev, const int* noct, const int* nei_o
   if (item_step) memcpy(item_step, plan.item_step.data(), plan.item_step.size() * sizeof(int));
   if (item_tile) memcpy(item_tile, plan.item_tile.data(), plan.item_tile.size() * sizeof(int));
   if (item_ntile) memcpy(item_ntile, plan.item_ntile.data(), plan.item_ntile.size() * sizeof(int));
-  if (item_prio) memcpy(item_prio, plan.item_prio.data(), plan.item_prio.size());
+  if (qoff) memcpy(qoff, plan.qoff.data(), plan.qoff.size() * sizeof(int));
+  if (item_flags) memcpy(item_flags, plan.item_prio.data(), plan.item_prio.size());
+  if (item_rank) memcpy(item_rank, plan.item_rank.data(), plan.item_rank.size() * sizeof(int));
   if (dep_off) memcpy(dep_off, plan.dep_off.data(), plan.dep_off.size() * sizeof(int));
   if (deps && !plan.deps.empty()) memcpy(deps, plan.deps.data(), plan.deps.size() * sizeof(int));
   return EMMA_RT_OK;

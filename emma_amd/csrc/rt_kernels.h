@@ -268,18 +268,26 @@ struct StepDesc {
 
 constexpr int kErrSched = -10;  // EMMA_RT_ERR_SCHED
 
+// item flags (as the planner writes them)
+constexpr unsigned char kItemPrio = 1;          // raised wave priority
+constexpr unsigned char kItemAcquireAgent = 2;  // reads results of another queue's items (every item with
+                                                // dependencies acquires at agent scope, see the kernel)
+constexpr unsigned char kItemReleaseAgent = 4;  // another queue's items read its results
+
 struct SchedArgs {
   const void* steps;        // StepDesc<R>[nsteps]
+  const int* qoff;          // [nq + 1]: queue q's tickets are qoff[q] .. qoff[q+1]-1
   const int* item_step;     // ticket -> level step
   const int* item_tile;     // ticket -> first tile of that level step
   const int* item_ntile;    // ticket -> consecutive tiles of the item
-  const unsigned char* item_prio;  // ticket -> wave priority while it runs (0..3)
+  const unsigned char* item_flags;  // ticket -> kItem* flags
   const int* dep_off;       // [nitems + 1]: the tickets ticket t waits for are deps[dep_off[t] .. dep_off[t+1])
   const int* deps;
   int* done;                // [nitems]: == epoch once the ticket's item is complete
-  int* ctl;                 // [0] next ticket, [1] work-groups that left, [2] epoch
+  int* ctl;                 // [q < nq] queue q's next ticket, [nq] work-groups that left, [nq + 1] epoch
   int nitems;
-  int pad;
+  int nq;                   // queues: one per XCD; a work-group serves the queue of its XCD
+  int xcc_queue[16];        // XCC_ID (low 4 bits) -> queue, -1: an XCD the library did not see
   int* status;              // the library's status word (a wait past its bound: EMMA_RT_ERR_SCHED)
   long long wait_ticks;     // bound of one dependency wait, s_memrealtime ticks (100 MHz)
 };
@@ -289,6 +297,8 @@ struct SchedArgs {
 // otherwise); coupled: the steps carry FillRad's block / the hydro injection
 template <typename R>
 hipError_t launch_persistent(const SchedArgs& s, int ng, bool coupled, int nblocks, hipStream_t st);
+// the XCC_ID (low 4 bits) of the XCD each of `n` work-groups ran on
+hipError_t launch_xcc_probe(int* out, int n, hipStream_t st);
 // work-groups of the persistent kernel one CU holds at once
 template <typename R>
 int persistent_blocks_per_cu(int ng, bool coupled);

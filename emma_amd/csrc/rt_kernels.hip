@@ -1199,11 +1199,13 @@ __device__ __forceinline__ void run_item(const StepDesc<R>& D, int tile) {
     if (D.hydro && cell < D.h.ncell) to_hydro_cell(D.h, cell);
 }
 
-// Work-groups loop over tickets: take the next one, wait for the tickets it
-// depends on (lanes of wave 0, one dependency each), acquire at agent scope
-// (their results may sit in another XCD's L2), run the item, release at
-// agent scope, mark the ticket done.  One wave of the work-group does the
-// acquire and the release for all four.  Items on the schedule's critical path
+// Work-groups loop over tickets of their XCD's queue (the XCC_ID hardware
+// register names the XCD): take the next one, wait for the tickets it
+// depends on (lanes of wave 0, one dependency each), acquire at agent scope,
+// run the item, release, mark the ticket done.  The release writes the L2
+// back (agent scope) only for items with a dependent in another queue: within
+// a queue the producer's stores are in the XCD's shared L2 once they have
+// completed.  One wave of the work-group does them for all four.  Items on the schedule's critical path
 // run at a raised wave priority so that the bandwidth-bound items sharing
 // their SIMDs do not stretch the chain.  The last work-group to leave
 // re-arms the queue (ticket 0, next epoch) for the next launch.  Coupled:
@@ -1222,51 +1224,61 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(Tile
   __shared__ int sTicket, sEpoch;
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int lane = (int)threadIdx.x & 63;
-  if (wave == 0) sEpoch = __hip_atomic_load(s.ctl + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // the queue of this work-group's XCD
+  int xcc;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+  const int q = s.xcc_queue[xcc & 15];
+  // (an XCD the library did not see serves no queue)
+  const int qb = q >= 0 ? cref<int>(s.qoff, q) : 0, qn = q >= 0 ? cref<int>(s.qoff, q + 1) - qb : 0;
+  if (wave == 0) sEpoch = __hip_atomic_load(s.ctl + s.nq + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   int epoch = 0;
   for (;;) {
     if (wave == 0) {  // lane 0 adds one, the other lanes nothing
-      const int old = atomicAdd(s.ctl, lane == 0 ? 1 : 0);
+      const int old = q >= 0 ? atomicAdd(s.ctl + q, lane == 0 ? 1 : 0) : 0;
       sTicket = __builtin_amdgcn_readfirstlane(old);
     }
     __syncthreads();
-    const int t = __builtin_amdgcn_readfirstlane(sTicket);
+    const int tq = __builtin_amdgcn_readfirstlane(sTicket);
     epoch = __builtin_amdgcn_readfirstlane(sEpoch);
-    if (t >= s.nitems) break;
+    if (tq >= qn) break;
+    const int t = qb + tq;
+    const int flags = cref<unsigned char>(s.item_flags, t);
     const int d0 = cref<int>(s.dep_off, t), d1 = cref<int>(s.dep_off, t + 1);
     if (d1 > d0) {
       if (wave == 0) {
         for (int i = d0 + lane; i < d1; i += 64) wait_done(s, s.deps[i], epoch);
-        // one acquire for the work-group: its waves share this CU's L1,
-        // which the acquire invalidates along with the XCD's L2
+        // one acquire at agent scope for the work-group (its waves share
+        // this CU's caches), also for dependencies on this XCD's items: an
+        // L1-only invalidate (buffer_inv sc0) left same-XCD results stale in
+        // measurements, the agent-scope invalidate does not
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       }
       __syncthreads();
     }
     const int step = cref<int>(s.item_step, t), tile = cref<int>(s.item_tile, t);
     const int ntile = cref<int>(s.item_ntile, t);
-    const int prio = cref<unsigned char>(s.item_prio, t);
+    const int prio = flags & kItemPrio;
     if (prio) __builtin_amdgcn_s_setprio(2);
     for (int k = 0; k < ntile; k++) {
       if (k > 0) __syncthreads();  // the tile step's LDS is reused
       run_item<R, G, Tiled, Coupled>(cref<StepDesc<R>>(s.steps, step), tile + k);
     }
     if (prio) __builtin_amdgcn_s_setprio(0);
-    // every wave's stores have reached L2 before wave 0 writes the L2 back
-    // (release at agent scope) and publishes the ticket
+    // every wave's stores have reached the L2 before wave 0 publishes the
+    // ticket; for readers on other XCDs it writes the L2 back first (release
+    // at agent scope)
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (wave == 0) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      if (flags & kItemReleaseAgent) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       __hip_atomic_store(s.done + t, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
   if (wave == 0) {
-    const int left = __builtin_amdgcn_readfirstlane(atomicAdd(s.ctl + 1, lane == 0 ? 1 : 0));
+    const int left = __builtin_amdgcn_readfirstlane(atomicAdd(s.ctl + s.nq, lane == 0 ? 1 : 0));
     if (left == (int)gridDim.x - 1) {
-      s.ctl[0] = 0;
-      s.ctl[1] = 0;
-      s.ctl[2] = epoch + 1;
+      for (int i = lane; i < s.nq + 1; i += 64) s.ctl[i] = 0;
+      if (lane == 0) s.ctl[s.nq + 1] = epoch + 1;
     }
   }
 }
@@ -1318,6 +1330,17 @@ hipError_t launch_level_step(const LevelArgs<R>& a0, const ChemConst& K, hipStre
     case 3: hipLaunchKernelGGL((level_step_kernel<R, 3, fast>), grid, block, 0, s, a, K); break;
     default: hipLaunchKernelGGL((level_step_kernel<R, 0, fast>), grid, block, 0, s, a, K); break;
   }
+  return hipGetLastError();
+}
+
+__global__ void xcc_probe_kernel(int* out, int n) {
+  int xcc;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+  if (threadIdx.x == 0 && (int)blockIdx.x < n) out[blockIdx.x] = xcc & 15;
+}
+
+hipError_t launch_xcc_probe(int* out, int n, hipStream_t st) {
+  hipLaunchKernelGGL(xcc_probe_kernel, dim3(n), dim3(64), 0, st, out, n);
   return hipGetLastError();
 }
 

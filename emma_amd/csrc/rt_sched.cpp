@@ -33,7 +33,8 @@ int chunk_tiles(int ntiles, int slots) {
 }
 
 int plan_schedule(const std::vector<SchedLevelTopo>& lev, const std::vector<int>& steps,
-                  const std::vector<double>& tile_cost, int slots, SchedPlan& out, const std::vector<int>* chunk) {
+                  const std::vector<double>& tile_cost, int slots, SchedPlan& out, const std::vector<int>* chunk,
+                  int nq) {
   const int nl = (int)lev.size();
   out = SchedPlan();
   if (slots < 1) return -1;
@@ -159,56 +160,102 @@ int plan_schedule(const std::vector<SchedLevelTopo>& lev, const std::vector<int>
     chain[x] = cost[x] + m;
     work += cost[x];
   }
-  // list schedule
+  // queue of each tile: on the coarsest level and on levels with more tiles
+  // than four times one queue's work-groups, nq contiguous ranges of tiles
+  // (one work-group per tile and step would wait for two rounds); on smaller
+  // levels the queue of the tile holding the parent of the tile's first oct
+  // (so that a small level's steps and their coupling to the coarser level
+  // stay in one XCD's L2).  An item takes the queue of its first tile; a tile
+  // has the same queue in every step of its level.
+  const int nqq = std::max(1, nq);
+  const int per_q = std::max(1, slots / nqq);
+  std::vector<std::vector<int>> tq(nl);
+  for (int k = 0; k < nl; k++) {
+    tq[k].assign(nt[k], 0);
+    if (k == 0 || nt[k] > 4 * per_q) {
+      for (int t = 0; t < nt[k]; t++) tq[k][t] = (int)((long long)t * nqq / nt[k]);
+    } else {
+      std::vector<int> parent(lev[k].noct, 0);  // parent cell's oct, level k-1
+      for (int c = 0; c < lev[k - 1].noct * 8; c++)
+        if (lev[k - 1].child[c] >= 0) parent[lev[k - 1].child[c]] = c >> 3;
+      for (int t = 0; t < nt[k]; t++) tq[k][t] = tq[k - 1][parent[t * kSchedTileOcts] / kSchedTileOcts];
+    }
+  }
+  std::vector<int> q_of(n);
+  for (int x = 0; x < n; x++) q_of[x] = tq[steps[seq_step[x]]][seq_tile[x]];
+  // list schedule, the work-groups split evenly over the queues
   using Ready = std::pair<double, int>;  // (chain, -id): longest chain first, then sequential order
-  std::priority_queue<Ready> ready;
+  std::vector<std::priority_queue<Ready>> ready(nqq);
   using Run = std::pair<double, int>;  // (finish, id), earliest first
   std::priority_queue<Run, std::vector<Run>, std::greater<Run>> running;
   for (int x = 0; x < n; x++)
-    if (indeg[x] == 0) ready.push({chain[x], -x});
+    if (indeg[x] == 0) ready[q_of[x]].push({chain[x], -x});
   std::vector<int> order;
   order.reserve(n);
   std::vector<double> start(n, 0.);
   double now = 0., span = 0.;
-  int free_slots = slots;
+  std::vector<int> free_slots(nqq, std::max(1, slots / nqq));
   while ((int)order.size() < n || !running.empty()) {
-    while (free_slots > 0 && !ready.empty()) {
-      const int x = -ready.top().second;
-      ready.pop();
-      start[x] = now;
-      order.push_back(x);
-      running.push({now + cost[x], x});
-      free_slots--;
+    for (int q = 0; q < nqq; q++) {
+      while (free_slots[q] > 0 && !ready[q].empty()) {
+        const int x = -ready[q].top().second;
+        ready[q].pop();
+        start[x] = now;
+        order.push_back(x);
+        running.push({now + cost[x], x});
+        free_slots[q]--;
+      }
     }
     if (running.empty()) return -1;  // unreachable items (cannot happen: deps point backwards)
     const Run r = running.top();
     running.pop();
     now = r.first;
     span = std::max(span, now);
-    free_slots++;
+    free_slots[q_of[r.second]]++;
     for (int e = soff[r.second]; e < soff[r.second + 1]; e++)
-      if (--indeg[succ[e]] == 0) ready.push({chain[succ[e]], -succ[e]});
+      if (--indeg[succ[e]] == 0) ready[q_of[succ[e]]].push({chain[succ[e]], -succ[e]});
   }
   double longest = 0.;
   for (int x = 0; x < n; x++) longest = std::max(longest, chain[x]);
-  // tickets
-  std::vector<int> ticket(n);
-  for (int i = 0; i < n; i++) ticket[order[i]] = i;
+  // tickets: queue by queue, each in the simulated start order (so each
+  // queue's order agrees with one topological order of all items)
+  std::vector<int> rank(n), ticket(n), tord;
+  tord.reserve(n);
+  for (int i = 0; i < n; i++) rank[order[i]] = i;
+  out.qoff.assign(nqq + 1, 0);
+  for (int q = 0; q < nqq; q++) {
+    for (int i = 0; i < n; i++)
+      if (q_of[order[i]] == q) tord.push_back(order[i]);
+    out.qoff[q + 1] = (int)tord.size();
+  }
+  for (int i = 0; i < n; i++) ticket[tord[i]] = i;
   out.nitems = n;
+  out.nq = nqq;
   out.item_step.resize(n);
   out.item_tile.resize(n);
   out.item_ntile.resize(n);
   out.item_prio.resize(n);
+  out.item_rank.resize(n);
   out.dep_off.assign(n + 1, 0);
   out.deps.reserve(dep.size());
+  std::vector<unsigned char> cross_out(n, 0);
+  for (int x = 0; x < n; x++)
+    for (int e = doff[x]; e < doff[x + 1]; e++)
+      if (q_of[dep[e]] != q_of[x]) cross_out[dep[e]] = 1;
   for (int i = 0; i < n; i++) {
-    const int x = order[i];
+    const int x = tord[i];
     out.item_step[i] = seq_step[x];
     out.item_tile[i] = seq_tile[x];
     out.item_ntile[i] = seq_ntile[x];
-    out.item_prio[i] = (start[x] + chain[x] >= 0.8 * span) ? 1 : 0;
-    for (int e = doff[x]; e < doff[x + 1]; e++) out.deps.push_back(ticket[dep[e]]);
+    out.item_rank[i] = rank[x];
+    bool cross_in = false;
+    for (int e = doff[x]; e < doff[x + 1]; e++) {
+      out.deps.push_back(ticket[dep[e]]);
+      cross_in = cross_in || q_of[dep[e]] != q_of[x];
+    }
     out.dep_off[i + 1] = (int)out.deps.size();
+    out.item_prio[i] = (unsigned char)(((start[x] + chain[x] >= 0.8 * span) ? kSchedPrio : 0) |
+                                       (cross_in ? kSchedAcquireAgent : 0) | (cross_out[x] ? kSchedReleaseAgent : 0));
   }
   out.est_span = span;
   out.est_chain = longest;

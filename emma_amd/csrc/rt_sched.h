@@ -23,10 +23,13 @@
 // chain of these edges (the CPU test checks it against the accesses the
 // kernels make).
 //
-// Order: a list schedule over `slots` work-groups with unit-free tile costs
-// per level, ready items taken longest-remaining-chain first; the tickets
-// follow the simulated start times.  Items whose chain reaches the simulated
-// end get a raised wave priority.
+// Order: a list schedule over `slots` work-groups (split evenly over the
+// queues) with unit-free tile costs per level, ready items taken
+// longest-remaining-chain first; each queue's tickets follow the simulated
+// start times, so every queue's order agrees with one topological order of
+// all items.  Items whose chain reaches the simulated end get a raised wave
+// priority; items with a dependency or a dependent in another queue are
+// flagged for agent-scope acquire / release.
 #pragma once
 
 #include <vector>
@@ -39,11 +42,20 @@ struct SchedLevelTopo {
   const int* child = nullptr;  // [noct][8]: >= 0 child oct on the next level, < 0 leaf
 };
 
+// item flags: raised wave priority; the item reads results of another
+// queue's items (acquire at agent scope); another queue's items read its
+// results (release at agent scope)
+constexpr unsigned char kSchedPrio = 1;
+constexpr unsigned char kSchedAcquireAgent = 2;
+constexpr unsigned char kSchedReleaseAgent = 4;
+
 struct SchedPlan {
-  int nitems = 0;
+  int nitems = 0, nq = 1;
+  std::vector<int> qoff;                              // queue q's tickets: qoff[q] .. qoff[q+1]-1
   std::vector<int> item_step, item_tile, item_ntile;  // by ticket: level step, first tile, tiles
-  std::vector<unsigned char> item_prio;
-  std::vector<int> dep_off, deps;         // by ticket; deps are tickets
+  std::vector<unsigned char> item_prio;               // by ticket: kSched* flags
+  std::vector<int> item_rank;                         // by ticket: place in the simulated start order
+  std::vector<int> dep_off, deps;                     // by ticket; deps are tickets
   double est_span = 0., est_chain = 0., est_work = 0.;  // simulated makespan, longest chain, total cost
 };
 
@@ -58,10 +70,13 @@ int chunk_tiles(int ntiles, int slots);
 // steps[i]: level index (0 = coarsest) of the i-th level step, in the
 // sequential order; tile_cost[k]: relative cost of one tile step of level k
 // (<= 0: 1); slots: work-groups of the persistent grid; chunk[k] (optional,
-// > 0): tiles per item of level k instead of chunk_tiles.  Returns 0, or -1
-// on bad input.
+// > 0): tiles per item of level k instead of chunk_tiles; nq: queues (one
+// per XCD, each served by its XCD's work-groups, so that dependencies within
+// a queue stay within one L2): a level's tiles split into nq contiguous
+// ranges, except that a level with no more tiles than four times one
+// queue's work-groups follows its parents' queues.  Returns 0, or -1 on bad input.
 int plan_schedule(const std::vector<SchedLevelTopo>& lev, const std::vector<int>& steps,
                   const std::vector<double>& tile_cost, int slots, SchedPlan& out,
-                  const std::vector<int>* chunk = nullptr);
+                  const std::vector<int>* chunk = nullptr, int nq = 1);
 
 }  // namespace emma

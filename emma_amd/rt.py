@@ -189,17 +189,17 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.emma_rt_get_schedule.argtypes = [ctypes.c_void_p, _ip, ctypes.POINTER(SchedInfo)]
     _llp = ctypes.POINTER(ctypes.c_longlong)
     L.emma_rt_plan_schedule.argtypes = [ctypes.c_int, _ip, _ip, _ip, ctypes.c_int, _ip, _dp, _ip, ctypes.c_int,
-                                        ctypes.c_longlong, ctypes.c_longlong, _llp, _llp, _ip, _ip, _ip,
-                                        ctypes.POINTER(ctypes.c_ubyte), _ip, _ip, _dp]
+                                        ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong, _llp, _llp, _ip, _ip, _ip,
+                                        _ip, ctypes.POINTER(ctypes.c_ubyte), _ip, _ip, _ip, _dp]
     _lib = L
     return L
 
 
-def plan_schedule(tree: T.Tree, steps, slots: int, tile_cost=None, chunk=None) -> dict:
+def plan_schedule(tree: T.Tree, steps, slots: int, tile_cost=None, chunk=None, nqueues: int = 1) -> dict:
     """The persistent coarse step's planner (host code, no GPU): for the
     level indices `steps` (0 = coarsest, sequential order) of `tree`, the
-    items by ticket (level step, first tile, tiles, wave priority) and the
-    tickets each waits for."""
+    queues' ticket ranges and the items by ticket (level step, first tile,
+    tiles, flags, simulated start rank) with the tickets each waits for."""
     L = load()
     lv = [tree.level(l) for l in range(tree.lmin, tree.lmax + 1)]
     noct = np.array([x.noct for x in lv], dtype=np.int32)
@@ -211,26 +211,28 @@ def plan_schedule(tree: T.Tree, steps, slots: int, tile_cost=None, chunk=None) -
     ni, nd = ctypes.c_longlong(0), ctypes.c_longlong(0)
     ip = lambda a: a.ctypes.data_as(_ip)
     args0 = (len(lv), ip(noct), ip(nei), ip(child), len(st), ip(st), None if cost is None else _d(cost),
-             None if cs is None else ip(cs), int(slots))
+             None if cs is None else ip(cs), int(slots), int(nqueues))
     r = L.emma_rt_plan_schedule(*args0, 0, 0, ctypes.byref(ni), ctypes.byref(nd), None, None, None, None, None,
-                                None, None)
+                                None, None, None, None)
     if r != OK:
         raise EmmaRTError(r, strerror(r))
     n, m = ni.value, nd.value
     item_step = np.zeros(n, np.int32)
     item_tile = np.zeros(n, np.int32)
     item_ntile = np.zeros(n, np.int32)
-    item_prio = np.zeros(n, np.uint8)
+    item_flags = np.zeros(n, np.uint8)
+    item_rank = np.zeros(n, np.int32)
+    qoff = np.zeros(nqueues + 1, np.int32)
     dep_off = np.zeros(n + 1, np.int32)
     deps = np.zeros(max(m, 1), np.int32)
     est = np.zeros(3)
-    r = L.emma_rt_plan_schedule(*args0, n, max(m, 1), ctypes.byref(ni), ctypes.byref(nd), ip(item_step),
-                                ip(item_tile), ip(item_ntile), item_prio.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)),
-                                ip(dep_off), ip(deps), _d(est))
+    r = L.emma_rt_plan_schedule(*args0, n, max(m, 1), ctypes.byref(ni), ctypes.byref(nd), ip(qoff), ip(item_step),
+                                ip(item_tile), ip(item_ntile), item_flags.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)),
+                                ip(item_rank), ip(dep_off), ip(deps), _d(est))
     if r != OK:
         raise EmmaRTError(r, strerror(r))
-    return {"item_step": item_step, "item_tile": item_tile, "item_ntile": item_ntile, "item_prio": item_prio,
-            "dep_off": dep_off,
+    return {"qoff": qoff, "item_step": item_step, "item_tile": item_tile, "item_ntile": item_ntile,
+            "item_flags": item_flags, "item_prio": item_flags & 1, "item_rank": item_rank, "dep_off": dep_off,
             "deps": deps[:m], "est_span": est[0], "est_chain": est[1], "est_work": est[2]}
 
 

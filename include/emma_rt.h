@@ -412,17 +412,21 @@ int emma_rt_get_schedule(emma_rt_ctx *ctx, int *sched, emma_rt_sched_info *info)
    level step in sequential order, a relative cost per tile step of each level
    (NULL: 1), the tiles per item of each level (NULL: the library's rule, one
    tile up to 4 x slots tiles per level, else about 2 x slots items per level
-   step) and the work-group count.  Writes nitems / ndeps; with cap_items == 0
-   returns after that (sizes only), else fills, by ticket, the level step,
-   first tile and tile count of each item, its wave priority, and the tickets
-   it waits for (deps[dep_off[t] .. dep_off[t+1])), and est[3] = makespan,
-   longest chain, total work (may be NULL).  EMMA_RT_ERR_ARG on bad input or
-   short buffers. */
+   step), the work-group count and the number of queues (the library uses one
+   per XCD, 8; each level step's items split into that many contiguous
+   ranges).  Writes nitems / ndeps; with cap_items == 0 returns after that
+   (sizes only), else fills qoff[nqueues + 1] (queue q's tickets are qoff[q]
+   .. qoff[q+1]-1) and, by ticket, the level step, first tile and tile count
+   of each item, its flags (1 raised priority, 2 a dependency in another
+   queue, 4 a dependent in another queue), its place in the simulated start
+   order, and the tickets it waits for (deps[dep_off[t] .. dep_off[t+1])),
+   and est[3] = makespan, longest chain, total work (may be NULL).
+   EMMA_RT_ERR_ARG on bad input or short buffers. */
 int emma_rt_plan_schedule(int nlev, const int *noct, const int *nei_oct, const int *child, int nsteps,
-                          const int *step_level, const double *tile_cost, const int *chunk, int slots,
-                          long long cap_items, long long cap_deps, long long *nitems, long long *ndeps,
-                          int *item_step, int *item_tile, int *item_ntile, unsigned char *item_prio, int *dep_off,
-                          int *deps, double *est);
+                          const int *step_level, const double *tile_cost, const int *chunk, int slots, int nqueues,
+                          long long cap_items, long long cap_deps, long long *nitems, long long *ndeps, int *qoff,
+                          int *item_step, int *item_tile, int *item_ntile, unsigned char *item_flags,
+                          int *item_rank, int *dep_off, int *deps, double *est);
 
 int emma_rt_get_stats(emma_rt_ctx *ctx, emma_rt_stats *out);
 /* device timing into emma_rt_stats.kernel_ms: EMMA_RT_TIMING_LAUNCH puts

@@ -101,13 +101,15 @@ def accesses(levels, steps, coupled=True):
     return reads, writes, nt
 
 
-def check_plan(tree, steps, slots, coupled=True, chunk=None, plan=None):
-    """the plan's tickets form a topological order and every conflicting pair
-    of accesses (by the sequential order of the level steps) is ordered by a
-    chain of its dependencies; items may cover several consecutive tiles"""
+def check_plan(tree, steps, slots, coupled=True, chunk=None, plan=None, nqueues=1):
+    """every queue's tickets follow one topological order of all items, the
+    agent-scope flags mark exactly the dependencies across queues, and every
+    conflicting pair of accesses (by the sequential order of the level steps)
+    is ordered by a chain of dependencies; items may cover several
+    consecutive tiles"""
     levels = levels_of(tree)
     if plan is None:
-        plan = rt.plan_schedule(tree, steps, slots, chunk=chunk)
+        plan = rt.plan_schedule(tree, steps, slots, chunk=chunk, nqueues=nqueues)
     reads_t, writes_t, nt = accesses(levels, steps, coupled)
     tbase = np.concatenate([[0], np.cumsum([nt[k] for k in steps])])
     item_step, item_tile, item_ntile = plan["item_step"], plan["item_tile"], plan["item_ntile"]
@@ -127,17 +129,30 @@ def check_plan(tree, steps, slots, coupled=True, chunk=None, plan=None):
         writes[seq[tk]] = set().union(*(writes_t[j] for j in ids))
     assert (covered == 1).all(), "every tile of every level step in exactly one item"
     dep_off, deps = plan["dep_off"], plan["deps"]
-    # topological ticket order, ancestors as bitsets over sequential ids
+    qoff, rank, flags = plan["qoff"], plan["item_rank"], plan["item_flags"]
+    queue = np.searchsorted(qoff, np.arange(n), side="right") - 1  # ticket -> queue
+    assert qoff[0] == 0 and qoff[-1] == n and (np.diff(qoff) >= 0).all()
+    assert sorted(rank.tolist()) == list(range(n))
+    for q in range(len(qoff) - 1):
+        assert (np.diff(rank[qoff[q]:qoff[q + 1]]) > 0).all(), "a queue out of the start order"
+    cross_out = np.zeros(n, dtype=bool)
+    # ancestors as bitsets over sequential ids (items visited in rank order)
     anc = [0] * n
-    for tk in range(n):
+    for tk in np.argsort(rank):
         x = int(seq[tk])
         a = 0
+        cross_in = False
         for d in deps[dep_off[tk]:dep_off[tk + 1]]:
-            assert d < tk, "a ticket waits for a later ticket"
+            assert rank[d] < rank[tk], "a ticket waits for a later one of the start order"
             y = int(seq[d])
             assert y < x, "a dependency on a later item of the sequential order"
             a |= anc[y] | (1 << y)
+            if queue[d] != queue[tk]:
+                cross_in = True
+                cross_out[d] = True
+        assert bool(flags[tk] & 2) == cross_in, "acquire flag"
         anc[x] = a
+    assert ((flags & 4) != 0).tolist() == cross_out.tolist(), "release flag"
     # every conflicting pair ordered: for each resource, the last write before
     # an access, and for a write the reads since that last write
     last_w, reads_since = {}, {}
@@ -164,11 +179,11 @@ def check_plan(tree, steps, slots, coupled=True, chunk=None, plan=None):
 
 
 @pytest.mark.parametrize("ncoarse", [1, 2])
-@pytest.mark.parametrize("slots", [1, 7, 1024])
-def test_radstep_plan_orders_every_conflict(ncoarse, slots):
+@pytest.mark.parametrize("slots,nqueues", [(1, 1), (7, 1), (1024, 1), (8, 8), (1024, 8)])
+def test_radstep_plan_orders_every_conflict(ncoarse, slots, nqueues):
     W = synth.amr_workload(4, [300, 500, 400], "SB99_equiv_mono_TH", seed=11, sigma=1.5, grid=16)
     nl = W.tree.lmax - W.tree.lmin + 1
-    plan = check_plan(W.tree, radstep_steps(nl, ncoarse), slots)
+    plan = check_plan(W.tree, radstep_steps(nl, ncoarse), slots, nqueues=nqueues)
     assert plan["est_span"] >= plan["est_chain"] - 1e-9
     assert plan["est_span"] >= plan["est_work"] / slots - 1e-9
 
@@ -178,15 +193,16 @@ def test_coarserad_segments():
     run, the coarse level's step another (the transfers sit between them)"""
     W = synth.amr_workload(3, [60, 100, 80], "Starburst99_1grpT_3grpS", seed=23, sigma=1.5, grid=16)
     nl = W.tree.lmax - W.tree.lmin + 1
-    check_plan(W.tree, list(range(nl - 1, 0, -1)), 64)
-    check_plan(W.tree, [0], 64)
+    for nq in (1, 8):
+        check_plan(W.tree, list(range(nl - 1, 0, -1)), 64, nqueues=nq)
+        check_plan(W.tree, [0], 64, nqueues=nq)
 
 
 @pytest.mark.parametrize("reflect", [(), (0, 2, 4)])
 def test_domain_faces(reflect):
     W = synth.amr_workload(3, [60, 100, 80], "SB99_equiv_mono_TH", seed=20, sigma=1.5, grid=16)
     tb = T.with_boundaries(W.tree, trans=range(6), reflect=reflect)
-    check_plan(tb, radstep_steps(tb.lmax - tb.lmin + 1, 2), 32)
+    check_plan(tb, radstep_steps(tb.lmax - tb.lmin + 1, 2), 32, nqueues=8)
 
 
 def test_uniform_level_chain():
@@ -230,7 +246,7 @@ def test_chunked_items(chunk):
     levels only; forced here on every level)"""
     W = synth.amr_workload(4, [300, 500, 400], "SB99_equiv_mono_TH", seed=11, sigma=1.5, grid=16)
     nl = W.tree.lmax - W.tree.lmin + 1
-    plan = check_plan(W.tree, radstep_steps(nl, 2), 16, chunk=chunk)
+    plan = check_plan(W.tree, radstep_steps(nl, 2), 16, chunk=chunk, nqueues=8)
     assert plan["item_ntile"].max() == max(chunk)
 
 
@@ -249,7 +265,7 @@ def test_checker_catches_missing_edges(drop):
     W = synth.amr_workload(3, [60, 100, 80], "SB99_equiv_mono_TH", seed=11, sigma=1.5, grid=16)
     nl = W.tree.lmax - W.tree.lmin + 1
     steps = radstep_steps(nl, 2)
-    plan = rt.plan_schedule(W.tree, steps, 8)
+    plan = rt.plan_schedule(W.tree, steps, 8, nqueues=8)
     lev_of_item = np.asarray(steps)[plan["item_step"]]
     off, deps = plan["dep_off"], plan["deps"]
     keep_off, keep = [0], []
@@ -260,6 +276,15 @@ def test_checker_catches_missing_edges(drop):
             if kind != drop:
                 keep.append(d)
         keep_off.append(len(keep))
-    bad = dict(plan, dep_off=np.asarray(keep_off), deps=np.asarray(keep, dtype=np.int32))
-    with pytest.raises(AssertionError):
+    # the agent-scope flags made to agree with the remaining edges, so that
+    # what fails is the ordering of the accesses
+    queue = np.searchsorted(plan["qoff"], np.arange(len(keep_off) - 1), side="right") - 1
+    flags = plan["item_flags"] & 1
+    for t in range(len(keep_off) - 1):
+        for d in keep[keep_off[t]:keep_off[t + 1]]:
+            if queue[d] != queue[t]:
+                flags[t] |= 2
+                flags[d] |= 4
+    bad = dict(plan, dep_off=np.asarray(keep_off), deps=np.asarray(keep, dtype=np.int32), item_flags=flags)
+    with pytest.raises(AssertionError, match="RAW|WAR|WAW"):
         check_plan(W.tree, steps, 8, plan=bad)
