@@ -28,7 +28,7 @@ int ntiles_of(int noct) { return (noct + kSchedTileOcts - 1) / kSchedTileOcts; }
 }  // namespace
 
 int chunk_tiles(int ntiles, int slots) {
-  const int c = ntiles / (2 * std::max(slots, 1));
+  const int c = ntiles / (8 * std::max(slots, 1));
   return std::max(1, std::min(c, 64));
 }
 

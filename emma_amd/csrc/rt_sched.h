@@ -61,10 +61,11 @@ struct SchedPlan {
 
 constexpr int kSchedTileOcts = 32;
 
-// tiles per item of a level step with ntiles tiles: one on levels of up to
-// 4 x slots tiles, else enough to give the step about 2 x slots items (at
-// most 64), so that the per-item synchronisation stays small against the
-// work on the large levels while the small levels keep one tile per item
+// tiles per item of a level step with ntiles tiles: one on levels of fewer
+// than 16 x slots tiles, else enough to give the step about 8 x slots items
+// (at most 64), so that the per-item synchronisation stays small against
+// the work on the largest levels (measured on the 256^3 level: 2, 4 and 8
+// tiles per item alike, 32 slower, one tile per item slower still)
 int chunk_tiles(int ntiles, int slots);
 
 // steps[i]: level index (0 = coarsest) of the i-th level step, in the

@@ -411,7 +411,7 @@ int emma_rt_get_schedule(emma_rt_ctx *ctx, int *sched, emma_rt_sched_info *info)
    emma_rt_tree_create (nlev levels, coarsest first), the level index of every
    level step in sequential order, a relative cost per tile step of each level
    (NULL: 1), the tiles per item of each level (NULL: the library's rule, one
-   tile up to 4 x slots tiles per level, else about 2 x slots items per level
+   tile below 16 x slots tiles per level, else about 8 x slots items per level
    step), the work-group count and the number of queues (the library uses one
    per XCD, 8; each level step's items split into that many contiguous
    ranges).  Writes nitems / ndeps; with cap_items == 0 returns after that

@@ -251,9 +251,9 @@ def test_chunked_items(chunk):
 
 
 def test_chunk_rule():
-    """one tile per item until a level has 4 x slots tiles"""
+    """one tile per item until a level has 16 x slots tiles"""
     tree = T.uniform_tree(5)  # 16^3 = 4096 octs = 128 tiles
-    for slots, expect in ((64, 1), (32, 2), (16, 4), (1, 64)):
+    for slots, expect in ((16, 1), (8, 2), (4, 4), (1, 16)):
         plan = rt.plan_schedule(tree, [0], slots)
         assert plan["item_ntile"].max() == expect, (slots, plan["item_ntile"].max())
 
