@@ -262,7 +262,7 @@ def run_uniform(W, precision, steps, warmup, tuning=None):
             "tuning": used}
 
 
-def run_amr(W, precision, steps, warmup, tuning=None, tune_here=False, log=None):
+def run_amr(W, precision, steps, warmup, tuning=None, tune_here=False, log=None, schedule=None):
     """A step is one coarse step of the RADSTEP level recursion (level l takes
     2^(l-lmin) level steps), one replay of its captured HIP graph: per-level
     launches, or one persistent launch fed by a device work queue (the
@@ -290,6 +290,8 @@ def run_amr(W, precision, steps, warmup, tuning=None, tune_here=False, log=None)
         tuned = tune.tune(g, reps=3, log=log)  # leaves the context set to the fastest per level
         used = _configure(g, W, precision, None)
         sched_tuned = tune.tune_schedule(g, reps=5, log=log)  # and on the faster schedule
+    if schedule is not None:
+        g.set_schedule({v: k for k, v in rt.SCHED_NAMES.items()}[schedule])
     sched = g.schedule()["sched"]
 
     # capture + instantiate the graphs of both buffer parities of the coarse level
@@ -357,12 +359,12 @@ def run_amr(W, precision, steps, warmup, tuning=None, tune_here=False, log=None)
             "sched_tuned": sched_tuned}
 
 
-def measure(name, W, precision, steps, warmup, tuning=None, tune_here=False):
+def measure(name, W, precision, steps, warmup, tuning=None, tune_here=False, schedule=None):
     """One workload's timed region and device times, as the JSON fields of a
     bench line (value, roofline of the dominant kernel, per-level table)."""
     if name in ("c4", "c5"):
         r = run_amr(W, precision, steps, warmup, tuning, tune_here,
-                    log=lambda m: print("bench: " + m, file=sys.stderr, flush=True))
+                    log=lambda m: print("bench: " + m, file=sys.stderr, flush=True), schedule=schedule)
     else:
         r = run_uniform(W, precision, steps, warmup, tuning)
     cu = r["stats"]["cell_updates"]
@@ -482,6 +484,8 @@ def main():
     ap.add_argument("--stencil-path", action="store_true")
     ap.add_argument("--no-tuning", action="store_true",
                     help="the library's default per-level choices instead of the tuning table")
+    ap.add_argument("--schedule", default=None, choices=["graph", "persistent"],
+                    help="coarse-step schedule of the timed region (c4, c5; default: the tuning table's)")
     ap.add_argument("--tune", default=None, metavar="PATH",
                     help="run the autotuner on the workload first (outside the timed region), use its choices "
                          "and merge them into the table at PATH")
@@ -506,7 +510,7 @@ def main():
                   "library defaults" if not table else "table emma_amd/tuning.json")
     print(f"bench: workload {args.workload} ({precision}), {args.steps} steps after {args.warmup} warm-up, "
           f"per-level tuning: {tuning_src}", file=sys.stderr, flush=True)
-    _, m = measure(args.workload, W, precision, args.steps, args.warmup, table, bool(args.tune))
+    _, m = measure(args.workload, W, precision, args.steps, args.warmup, table, bool(args.tune), args.schedule)
     if args.tune and m["tuned"]:
         tune.save(m["tuned"], WORKLOAD_NAMES[args.workload], args.tune, m["sched_tuned"])
     sched_txt = {"graph": "one HIP graph replay of per-level launches",
