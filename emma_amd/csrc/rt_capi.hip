@@ -53,6 +53,8 @@ struct DevLevel {
   int halo_per_dir = 0;  // some wave has more than 64 halo faces (no kListed variant)
   int variant = kListed;  // halo variant of the tiled kernel
   int chem_budget = 0;   // chemistry tries before a leaf goes to the straggler pass (0: none)
+  int morton_m = 0;      // > 0: a complete periodic cube of 2^morton_m octs a side in Morton order
+  int raster = 0;        // the tiled kernel walks the cube's tiles in raster order (morton_m > 0)
   float chem_keep = 0.f;  // long chains stay (see emma_rt_set_chem_keep)
   void* e[2] = {nullptr, nullptr};
   void* fx[2] = {nullptr, nullptr};
@@ -460,6 +462,33 @@ static int default_variant(const DevLevel& L) {
   return tiles <= kLatencyTiles ? kLatency : kPerDir;
 }
 
+// log2 of the side when the level is a complete periodic cube of n^3 octs (n
+// a power of two, 8 <= n <= 2^13) whose octs are in Morton order (x in the
+// lowest bit) with the same-level neighbours nei_oct lists, else 0
+static int morton_cube(int noct, const int* nei) {
+  int m = 3;
+  while (m <= 13 && (1LL << (3 * m)) < noct) m++;
+  if (m > 13 || (1LL << (3 * m)) != noct) return 0;
+  const unsigned n = 1u << m;
+  auto spread = [](unsigned v) {  // bit i -> bit 3i
+    unsigned long long r = 0;
+    for (int i = 0; i < 13; i++) r |= (unsigned long long)((v >> i) & 1u) << (3 * i);
+    return r;
+  };
+  auto key = [&](unsigned x, unsigned y, unsigned z) { return (long long)(spread(x) | spread(y) << 1 | spread(z) << 2); };
+  for (int o = 0; o < noct; o++) {
+    unsigned c[3] = {0, 0, 0};
+    for (int i = 0; i < 3 * m; i++) c[i % 3] |= (unsigned)((o >> i) & 1) << (i / 3);
+    for (int d = 0; d < 3; d++)
+      for (int s = 0; s < 2; s++) {
+        unsigned q[3] = {c[0], c[1], c[2]};
+        q[d] = (q[d] + (s ? 1u : n - 1u)) & (n - 1u);
+        if (nei[o * 6 + 2 * d + s] != key(q[0], q[1], q[2])) return 0;
+      }
+  }
+  return m;
+}
+
 extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int* noct, const int* nei_oct,
                                    const int* child, int precision) {
   if (c == nullptr || noct == nullptr || nei_oct == nullptr || child == nullptr) return EMMA_RT_ERR_ARG;
@@ -554,6 +583,8 @@ extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int
     }
     L.halo_per_dir = maxh > 64 ? 1 : 0;
     L.variant = default_variant(L);
+    L.morton_m = (k == 0) ? morton_cube(L.noct, nei) : 0;
+    L.raster = L.morton_m > 0;
   }
   // device allocation and upload
   for (int k = 0; k < nl; k++) {
@@ -763,6 +794,7 @@ static LevelArgs<R> level_args(emma_rt_ctx* c, int k, double dtnew, double aexp,
     A.update_split = 0;
   }
   A.variant = L.variant;
+  A.raster_m = (L.raster && L.morton_m > 0) ? L.morton_m : 0;
   if (c->d_defer != nullptr && L.chem_budget > 0 && c->defer_seq < c->defer_slots) {
     A.chem_budget = L.chem_budget;
     A.chem_keep = L.chem_keep;
@@ -1721,6 +1753,31 @@ extern "C" int emma_rt_set_level_variant(emma_rt_ctx* c, int level, int variant)
   for (auto& L : c->lev)
     if (level == -1 || L.level == level) L.variant = variant == EMMA_RT_VARIANT_AUTO ? default_variant(L) : variant;
   drop_graphs(c);
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_set_level_order(emma_rt_ctx* c, int level, int order) {
+  if (c == nullptr || !c->has_tree) return EMMA_RT_ERR_STATE;
+  if (order != EMMA_RT_ORDER_MORTON && order != EMMA_RT_ORDER_RASTER) return EMMA_RT_ERR_ARG;
+  for (int k = 0; k < (int)c->lev.size(); k++) {
+    if (level >= 0 && c->lmin + k != level) continue;
+    DevLevel& L = c->lev[k];
+    if (order == EMMA_RT_ORDER_RASTER && L.morton_m == 0) {
+      if (level >= 0) return EMMA_RT_ERR_ARG;
+      continue;
+    }
+    L.raster = order == EMMA_RT_ORDER_RASTER;
+  }
+  if (level >= 0 && level_of(c, level) == nullptr) return EMMA_RT_ERR_ARG;
+  drop_graphs(c);
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_get_level_order(emma_rt_ctx* c, int level, int* order, int* cube_side_log2) {
+  DevLevel* L = level_of(c, level);
+  if (L == nullptr) return EMMA_RT_ERR_ARG;
+  if (order) *order = L->raster ? EMMA_RT_ORDER_RASTER : EMMA_RT_ORDER_MORTON;
+  if (cube_side_log2) *cube_side_log2 = L->morton_m;
   return EMMA_RT_OK;
 }
 

@@ -22,6 +22,10 @@ struct LevelArgs {
   // mixed path, 1 or 3 groups: the halo variant of the tiled kernel
   // (kListed, kPerDir or kLatency)
   int variant;
+  // > 0: the level is a complete periodic cube of 2^raster_m octs a side in
+  // Morton order, and the tiled kernel walks each XCD's octant of tiles in
+  // raster order (z fastest) instead of Morton order (see raster_tile)
+  int raster_m;
   int cnslot;  // accumulators in cracc
   // chemistry stragglers (tiled mixed kernel): a leaf whose sub-step loop is
   // still running after chem_budget tries (> 0; <= 0: no limit) stores its

@@ -28,6 +28,35 @@ __device__ __forceinline__ int xcd_block(int nb) {
   return (b & 7) * per + (b >> 3);
 }
 
+// The tile a block takes on a complete Morton-ordered cube of n = 2^m octs a
+// side.  A tile is 32 Morton-consecutive octs, a 4 x 4 x 2 brick, so the
+// tile index interleaves the tile coordinates (z, x, y from the lowest bit:
+// tz0 tx0 ty0 tz1 ...; tz has one bit more).  Blocks are dealt round-robin
+// to the 8 XCDs; XCD k takes octant k (the top bit of tx, ty, tz) and walks
+// it z fastest, then y, then x: the neighbours across the largest faces
+// (4 x 4 octs, +-z) are the next and previous tiles, the +-y ones n/4
+// tiles away, within the ~128 tiles an XCD holds at once, so their halo
+// cells are still in its L2.  In Morton order most of them were not.
+__device__ __forceinline__ int raster_tile(int m) {
+  const int b = blockIdx.x;
+  const int k = b & 7, p = b >> 3;
+  const int nz = 1 << (m - 2), ny = 1 << (m - 3);  // octant extent in tiles along z, y
+  const int tz = (p & (nz - 1)) | ((k >> 2) << (m - 2));
+  const int ty = ((p >> (m - 2)) & (ny - 1)) | (((k >> 1) & 1) << (m - 3));
+  const int tx = (p >> (2 * m - 5)) | ((k & 1) << (m - 3));
+  int t = 0;
+#pragma unroll
+  for (int i = 0; i < 14; i++) {
+    if (i < m - 2) {
+      t |= ((tz >> i) & 1) << (3 * i);
+      t |= ((tx >> i) & 1) << (3 * i + 1);
+      t |= ((ty >> i) & 1) << (3 * i + 2);
+    }
+  }
+  t |= ((tz >> (m - 2)) & 1) << (3 * (m - 2));
+  return t;
+}
+
 __device__ __forceinline__ void raise_status(int* status, int code, int level, int cell) {
   if (atomicCAS(status, 0, code) == 0) {
     status[1] = level;
@@ -945,7 +974,9 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
     LevelArgs<float> a_by_value, ChemConst K_by_value) {
   const KernArgs<float> args = kernargs<float>();
   __shared__ TileLds<G> lds;
-  tiled_step<G, PerDir ? kPerDir : kListed>(args.a, args.K, xcd_block(args.a.nblocks), lds, (int)threadIdx.x);
+  const int m = args.a.raster_m;
+  tiled_step<G, PerDir ? kPerDir : kListed>(args.a, args.K, m > 0 ? raster_tile(m) : xcd_block(args.a.nblocks), lds,
+                                            (int)threadIdx.x);
 }
 
 template <int G>

@@ -58,6 +58,9 @@ VARIANT_PERDIR = 1
 VARIANT_LATENCY = 2
 VARIANT_NAMES = {VARIANT_LISTED: "listed", VARIANT_PERDIR: "perdir", VARIANT_LATENCY: "latency"}
 
+ORDER_MORTON = 0
+ORDER_RASTER = 1
+
 SCHED_GRAPH = 0
 SCHED_PERSISTENT = 1
 SCHED_NAMES = {SCHED_GRAPH: "graph", SCHED_PERSISTENT: "persistent"}
@@ -75,7 +78,8 @@ EXPORTS = [
     "emma_rt_save_state", "emma_rt_restore_state", "emma_rt_set_stars", "emma_rt_set_star_sources",
     "emma_rt_set_chem_budget", "emma_rt_get_chem_budget", "emma_rt_set_level_variant", "emma_rt_get_level_tuning",
     "emma_rt_advance_stencil_noalloc", "emma_rt_set_chem_keep", "emma_rt_dump_grid", "emma_rt_grid_field",
-    "emma_rt_set_schedule", "emma_rt_get_schedule", "emma_rt_plan_schedule",
+    "emma_rt_set_schedule", "emma_rt_get_schedule", "emma_rt_plan_schedule", "emma_rt_set_level_order",
+    "emma_rt_get_level_order",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -186,6 +190,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.emma_rt_grid_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
                                      ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)]
     L.emma_rt_set_schedule.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.emma_rt_set_level_order.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.emma_rt_get_level_order.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _ip]
     L.emma_rt_get_schedule.argtypes = [ctypes.c_void_p, _ip, ctypes.POINTER(SchedInfo)]
     _llp = ctypes.POINTER(ctypes.c_longlong)
     L.emma_rt_plan_schedule.argtypes = [ctypes.c_int, _ip, _ip, _ip, ctypes.c_int, _ip, _dp, _ip, ctypes.c_int,
@@ -475,6 +481,15 @@ class RadGPU:
     def coarse_step(self, aexp: float = 1.0, ncoarse: int = 1):
         st = self.lib.emma_rt_coarse_step(self.h, aexp, ncoarse)
         self._check(st, self.stats() if st < 0 and st != ERR_HIP else None)
+
+    def set_level_order(self, order: int, level: int = -1):
+        """tile order of the tiled kernel on complete Morton cubes (ORDER_*)"""
+        self._check(self.lib.emma_rt_set_level_order(self.h, int(level), int(order)))
+
+    def level_order(self, level: int) -> dict:
+        o, m = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.lib.emma_rt_get_level_order(self.h, int(level), ctypes.byref(o), ctypes.byref(m)))
+        return {"order": o.value, "cube_side_log2": m.value}
 
     def set_schedule(self, sched: int):
         """coarse-step schedule: SCHED_GRAPH (one launch per level step, in a

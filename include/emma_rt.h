@@ -392,6 +392,20 @@ int emma_rt_grid_field(emma_rt_ctx *ctx, int field_id, int rank, float *out, lon
    persistent kernel, and the latency halo variant runs as per-direction.
    Per-level device times are not available in this schedule: with
    EMMA_RT_TIMING_LAUNCH each persistent launch is one timed span. */
+/* Tile order of the tiled mixed kernel on a complete periodic cube of 2^m
+   octs a side stored in Morton order (the library checks the level's
+   nei_oct against that layout at emma_rt_tree_create): EMMA_RT_ORDER_RASTER
+   (the default there) walks each XCD's octant of 4 x 4 x 2-oct tiles z
+   fastest, then y, then x, so that the neighbours across a tile's larger
+   faces are processed next to it and their halo cells are still in that
+   XCD's L2; EMMA_RT_ORDER_MORTON keeps the tiles' Morton order.  Speed only,
+   results unchanged.  level < 0: every level that can take it.
+   EMMA_RT_ERR_ARG for RASTER on a level that is not such a cube. */
+#define EMMA_RT_ORDER_MORTON 0
+#define EMMA_RT_ORDER_RASTER 1
+int emma_rt_set_level_order(emma_rt_ctx *ctx, int level, int order);
+int emma_rt_get_level_order(emma_rt_ctx *ctx, int level, int *order, int *cube_side_log2);
+
 #define EMMA_RT_SCHED_GRAPH 0
 #define EMMA_RT_SCHED_PERSISTENT 1
 typedef struct emma_rt_sched_info {

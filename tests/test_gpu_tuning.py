@@ -7,9 +7,11 @@ with contraction allowed in the FP32 transport the compiler fuses each
 kernel's arithmetic its own way, so they agree to FP32 rounding, and every
 variant meets the mixed-precision bar against the oracle.
 """
+import numpy as np
 import pytest
 
 from emma_amd import rt, synth, tune
+from emma_amd.synth import Fields
 from test_gpu_parity import MIXED_TOL, field_errors
 
 pytestmark = pytest.mark.gpu
@@ -112,3 +114,32 @@ def test_tuner(gpu_lib, tmp_path):
     for l, r in res.items():
         t = g2.level_tuning(l)
         assert t["variant"] == names[r["variant"]] and t["chem_budget"] == r["chem_budget"]
+
+
+@pytest.mark.parametrize("amr", [False, True])
+def test_raster_tile_order(gpu_lib, amr):
+    """A complete Morton-ordered cube takes the raster tile order by default;
+    the order changes which block runs which tile, not the results"""
+    if amr:
+        W = synth.amr_workload(4, [300, 500], "Starburst99_1grpT_3grpS", seed=19, sigma=1.5, nsrc=3, grid=16)
+    else:
+        W = synth.uniform_workload(5, "Starburst99_1grpT_3grpS", seed=19, nsrc=6)
+    res = {}
+    for order in (rt.ORDER_RASTER, rt.ORDER_MORTON):
+        g = rt.RadGPU(W.params, rt.MIXED)
+        g.setup(W.params, W.tree, W.fields)
+        info = g.level_order(W.tree.lmin)
+        assert info["cube_side_log2"] == W.tree.lmin - 1 and info["order"] == rt.ORDER_RASTER, info
+        if W.tree.lmax > W.tree.lmin:
+            assert g.level_order(W.tree.lmax)["cube_side_log2"] == 0
+            with pytest.raises(rt.EmmaRTError):
+                g.set_level_order(rt.ORDER_RASTER, W.tree.lmax)
+        g.set_level_order(order, W.tree.lmin)
+        for _ in range(2):
+            g.coarse_step(1.0, 1)
+        assert g.stats()["status"] == 0
+        res[order] = {l: g.get(l) for l in range(W.tree.lmin, W.tree.lmax + 1)}
+        g.close()
+    for l in res[rt.ORDER_RASTER]:
+        for k in Fields.NAMES:
+            assert np.array_equal(getattr(res[rt.ORDER_RASTER][l], k), getattr(res[rt.ORDER_MORTON][l], k)), (l, k)
