@@ -1199,10 +1199,6 @@ __device__ __forceinline__ TileLds<G>* tile_lds() {
   return &lds;
 }
 
-// one item: FillRad's block on the tile's cells, the tile's level step, the
-// hydro injection of its cells.  Each thread fills and injects the cell it
-// steps, so the three need no barrier between them; neighbours' cells are
-// read only for their N, F and split flags, which FillRad does not touch.
 // The thread index as a value the optimiser cannot see through: otherwise
 // every address and mask the item derives from it alone is hoisted out of
 // the work-group's ticket loop and held in registers across all items.
@@ -1212,6 +1208,11 @@ __device__ __forceinline__ int opaque_thread() {
   return t;
 }
 
+// one tile of an item: FillRad's block on the tile's cells, the tile's
+// level step, the hydro injection of its cells.  Each thread fills and
+// injects the cell it steps, so the three need no barrier between them;
+// neighbours' cells are read only for their N, F and split flags, which
+// FillRad does not touch.
 template <typename R, int G, bool Tiled, bool Coupled>
 __device__ __forceinline__ void run_item(const StepDesc<R>& D, int tile) {
   const int lc = opaque_thread();
@@ -1234,14 +1235,15 @@ __device__ __forceinline__ void run_item(const StepDesc<R>& D, int tile) {
 // register names the XCD): take the next one, wait for the tickets it
 // depends on (lanes of wave 0, one dependency each), acquire at agent scope,
 // run the item, release, mark the ticket done.  The release writes the L2
-// back (agent scope) only for items with a dependent in another queue: within
-// a queue the producer's stores are in the XCD's shared L2 once they have
-// completed.  One wave of the work-group does them for all four.  Items on the schedule's critical path
-// run at a raised wave priority so that the bandwidth-bound items sharing
-// their SIMDs do not stretch the chain.  The last work-group to leave
-// re-arms the queue (ticket 0, next epoch) for the next launch.  Coupled:
-// FillRad's block and the hydro injection are compiled in (without them the
-// kernel needs fewer registers).
+// back (agent scope) only for items with a dependent in another queue:
+// within a queue the producer's stores are in the XCD's shared L2 once they
+// have completed.  One wave of the work-group does the acquire and the
+// release for all four.  Items on the schedule's critical path run at a
+// raised wave priority so that the bandwidth-bound items sharing their SIMDs
+// do not stretch the chain.  The last work-group to leave re-arms the queues
+// (tickets 0, next epoch) for the next launch.  Coupled: FillRad's block and
+// the hydro injection are compiled in (without them the kernel needs fewer
+// registers).
 //
 // The work-group's own bookkeeping (ticket, done flag, exit count) is done
 // by all of wave 0 under a wave-uniform branch, never by one lane: a
