@@ -45,7 +45,7 @@ def test_level_golden(gpu_lib, G):
 
 
 @pytest.mark.parametrize("G", [1, 3])
-@pytest.mark.parametrize("mode", ["radsolver", "graph"])
+@pytest.mark.parametrize("mode", ["radsolver", "graph", "persistent"])
 def test_amr_golden(gpu_lib, G, mode):
     z = gio.load(f"amr_g{G}.npz")
     tr = gio.amr_tree(z)
@@ -53,9 +53,11 @@ def test_amr_golden(gpu_lib, G, mode):
     g = rt.RadGPU(p, rt.FP64)
     g.enable_xion_times(True)
     g.setup(p, tr, {l: gio.fields(z, f"in{l}_") for l in range(tr.lmin, tr.lmax + 1)})
+    if mode == "persistent":
+        g.set_schedule(rt.SCHED_PERSISTENT)
     for tphy in z["tphy"]:
         g.set_tphy(float(tphy))
-        if mode == "graph":
+        if mode in ("graph", "persistent"):
             g.coarse_step(1.0, 1)
         else:
             from oracle import coarse_step

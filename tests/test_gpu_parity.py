@@ -36,8 +36,12 @@ def field_errors(A: Fields, B: Fields, c: float):
 
 
 def run_pair(oracle, W, precision, steps, mode="radsolver"):
+    """mode: "radsolver" (host-driven level steps), "graph" or "persistent"
+    (emma_rt_coarse_step under that schedule)"""
     g = rt.RadGPU(W.params, precision)
     g.setup(W.params, W.tree, W.fields)
+    if mode == "persistent":
+        g.set_schedule(rt.SCHED_PERSISTENT)
     oracle.setup(W.params, W.tree, W.fields)
     for _ in range(steps):
         if mode == "radsolver":
@@ -62,7 +66,7 @@ def test_uniform_fp64(gpu_lib, oracle, table, G):
 
 
 @pytest.mark.parametrize("table,G", [("SB99_equiv_mono_TH", 1), ("Starburst99_1grpT_3grpS", 3)])
-@pytest.mark.parametrize("mode", ["radsolver", "graph"])
+@pytest.mark.parametrize("mode", ["radsolver", "graph", "persistent"])
 def test_amr_fp64(gpu_lib, oracle, table, G, mode):
     W = synth.amr_workload(3, [60, 100, 80], table, seed=11, sigma=1.5, nsrc=2, grid=16)
     g = run_pair(oracle, W, rt.FP64, steps=2, mode=mode)
@@ -84,11 +88,12 @@ def test_mixed_bound(gpu_lib, oracle):
 
 
 @pytest.mark.parametrize("table,G", [("SB99_equiv_mono_TH", 1), ("Starburst99_1grpT_3grpS", 3)])
-def test_amr_mixed_bound(gpu_lib, oracle, table, G):
+@pytest.mark.parametrize("mode", ["graph", "persistent"])
+def test_amr_mixed_bound(gpu_lib, oracle, table, G, mode):
     """mixed precision on a 4-level tree: partial LDS tiles, coarse faces,
     FP32 reflux accumulators"""
     W = synth.amr_workload(3, [60, 100, 80], table, seed=11, sigma=1.5, nsrc=2, grid=16)
-    g = run_pair(oracle, W, rt.MIXED, steps=2, mode="graph")
+    g = run_pair(oracle, W, rt.MIXED, steps=2, mode=mode)
     for l in range(W.tree.lmin, W.tree.lmax + 1):
         err = field_errors(g.get(l), oracle.get(l), W.params.cloc())
         print("amr mixed G=%d level %d" % (G, l), err)
@@ -169,7 +174,7 @@ def test_level_size_limit(gpu_lib, table, G):
 
 
 @pytest.mark.parametrize("table,G", [("SB99_equiv_mono_TH", 1), ("Starburst99_1grpT_3grpS", 3)])
-@pytest.mark.parametrize("mode", ["host", "graph"])
+@pytest.mark.parametrize("mode", ["host", "graph", "persistent"])
 @pytest.mark.parametrize("precision", [rt.FP64, rt.MIXED])
 @pytest.mark.parametrize("faces", ["periodic", "trans_ref"])
 def test_coarserad(gpu_lib, oracle, table, G, mode, precision, faces):
@@ -187,9 +192,11 @@ def test_coarserad(gpu_lib, oracle, table, G, mode, precision, faces):
         W = synth.Workload(W.name, W.params, tb, W.fields, W.nsub)
     g = rt.RadGPU(W.params, precision)
     g.setup(W.params, W.tree, W.fields, mode=rt.COARSERAD)
+    if mode == "persistent":
+        g.set_schedule(rt.SCHED_PERSISTENT)
     oracle.setup(W.params, W.tree, W.fields, mode=COARSERAD)
     for _ in range(3):
-        if mode == "graph":
+        if mode in ("graph", "persistent"):
             g.coarse_step(1.0, 1)
         else:
             coarse_step_coarserad(g, W.tree, W.params)
@@ -207,6 +214,8 @@ def coupled_pair(oracle, W, precision, star, tphy, mode, steps=2, seed=7):
     hyd = {l: synth.hydro_like(F, rng) for l, F in W.fields.items()}
     g = rt.RadGPU(W.params, precision)
     g.setup(W.params, W.tree, W.fields, tphy)
+    if mode == "persistent":
+        g.set_schedule(rt.SCHED_PERSISTENT)
     oracle.setup(W.params, W.tree, W.fields, tphy)
     for s in (g, oracle):
         s.set_coupling(True)
@@ -226,7 +235,7 @@ def coupled_pair(oracle, W, precision, star, tphy, mode, steps=2, seed=7):
 
 
 @pytest.mark.parametrize("table,G", [("SB99_equiv_mono_TH", 1), ("Starburst99_1grpT_3grpS", 3)])
-@pytest.mark.parametrize("mode", ["radsolver", "graph"])
+@pytest.mark.parametrize("mode", ["radsolver", "graph", "persistent"])
 @pytest.mark.parametrize("precision", [rt.FP64, rt.MIXED])
 def test_hydro_coupling_and_source(gpu_lib, oracle, table, G, mode, precision):
     """FillRad (hydro -> radiation, Stromgren source, E2T) before every level
@@ -448,7 +457,7 @@ def test_refined_block_complete_level_kernel(gpu_lib, oracle, table, G, precisio
 
 
 @pytest.mark.parametrize("precision", [rt.FP64, rt.MIXED])
-@pytest.mark.parametrize("mode", ["radsolver", "graph"])
+@pytest.mark.parametrize("mode", ["radsolver", "graph", "persistent"])
 def test_nine_groups(gpu_lib, oracle, precision, mode):
     """Nine photon groups (Starburst99_3grpT_3grpS: three time groups of three
     frequency groups) run the kernels' generic group count; the oracle agrees
@@ -470,7 +479,8 @@ def test_nine_groups(gpu_lib, oracle, precision, mode):
 @pytest.mark.parametrize("table,G", [("SB99_equiv_mono_TH", 1), ("Starburst99_1grpT_3grpS", 3),
                                      ("Starburst99_3grpT_3grpS", 9)])
 @pytest.mark.parametrize("precision", [rt.FP64, rt.MIXED])
-def test_amr_bit_reproducible(gpu_lib, table, G, precision):
+@pytest.mark.parametrize("sched", [rt.SCHED_GRAPH, rt.SCHED_PERSISTENT], ids=["graph", "persistent"])
+def test_amr_bit_reproducible(gpu_lib, table, G, precision, sched):
     """Refluxing gives every (coarse leaf, face) pair its own accumulators
     with one writer per launch, summed in pair order by the coarse leaf: two
     runs of a 4-level tree (per-level transport, two coarse steps: host-driven
@@ -485,6 +495,7 @@ def test_amr_bit_reproducible(gpu_lib, table, G, precision):
         from oracle import coarse_step
 
         coarse_step(g, W.tree, W.params)
+        g.set_schedule(sched)
         g.coarse_step(1.0, 1)
         g.coarse_step(1.0, 1)
         runs.append({l: g.get(l) for l in range(W.tree.lmin, W.tree.lmax + 1)})
@@ -498,7 +509,8 @@ def test_amr_bit_reproducible(gpu_lib, table, G, precision):
                                              ("Starburst99_3grpT_3grpS", 9, (0, 2, 4))])
 @pytest.mark.parametrize("precision", [rt.FP64, rt.MIXED])
 @pytest.mark.parametrize("amr", [False, True])
-def test_domain_boundaries(gpu_lib, oracle, table, G, reflect, precision, amr):
+@pytest.mark.parametrize("mode", ["graph", "persistent"])
+def test_domain_boundaries(gpu_lib, oracle, table, G, reflect, precision, amr, mode):
     """Non-periodic domain faces (EMMA_RT_NEI_TRANSMISSIVE / _REFLECTIVE, the
     reference's TRANS*/REF* gather; the oracle agrees with the reference's
     TRANS builds bit for bit, test_oracle_ref.py): a complete 16^3 level (the
@@ -513,11 +525,11 @@ def test_domain_boundaries(gpu_lib, oracle, table, G, reflect, precision, amr):
         W = synth.uniform_workload(4, table, seed=29, nsrc=6)
     tb = T.with_boundaries(W.tree, trans=range(6), reflect=reflect)
     W = synth.Workload(W.name, W.params, tb, W.fields, W.nsub)
-    g = run_pair(oracle, W, precision, steps=2, mode="graph")
+    g = run_pair(oracle, W, precision, steps=2, mode=mode)
     c = W.params.cloc()
     for l in range(W.tree.lmin, W.tree.lmax + 1):
         err = field_errors(g.get(l), oracle.get(l), c)
-        print("boundaries G=%d prec=%d amr=%d level %d" % (G, precision, amr, l), err)
+        print("boundaries G=%d prec=%d amr=%d %s level %d" % (G, precision, amr, mode, l), err)
         if precision == rt.FP64:
             assert max(err.values()) < FP64_TOL, (l, err)
         else:
