@@ -146,6 +146,41 @@ def test_coupled_same_as_graph(gpu_lib, table, G, precision):
 
 
 @pytest.mark.parametrize("precision", [rt.FP64, rt.MIXED])
+@pytest.mark.parametrize("ncoarse", [1, 2])
+def test_coarserad_coupled_same_as_graph(gpu_lib, precision, ncoarse):
+    """COARSERAD with the hydro coupling and the test source: in the
+    persistent schedule the coarse level's hydro injection runs inside its
+    items, before distribE (it reads the coarse level's x, eint; distribE
+    writes the finer levels' N, F); several coarse steps per call give two
+    persistent launches each"""
+    W = small("Starburst99_1grpT_3grpS", 31, nsrc=0, faces="trans_ref")
+    rng = np.random.default_rng(9)
+    hyd = {l: synth.hydro_like(F, rng) for l, F in W.fields.items()}
+    runs = {}
+    for sched in (rt.SCHED_GRAPH, rt.SCHED_PERSISTENT):
+        g = rt.RadGPU(W.params, precision)
+        g.setup(W.params, W.tree, W.fields, tphy=1e-3, mode=rt.COARSERAD)
+        no_latency_variant(g, W.tree)
+        g.set_coupling(True)
+        g.set_stromgren_source(synth.Star(x=0.02, y=0.02, z=0.02))
+        for l, H in hyd.items():
+            g.set_hydro(l, H)
+            g.set_oct_pos(l, synth.oct_corners(W.tree.level(l)))
+        g.set_schedule(sched)
+        for _ in range(2):
+            g.coarse_step(1.0, ncoarse)
+        assert g.stats()["status"] == 0
+        if sched == rt.SCHED_PERSISTENT:
+            assert g.schedule()["launches"] == 2 * ncoarse
+        runs[sched] = ({l: g.get(l) for l in W.fields}, {l: g.get_hydro(l) for l in W.fields})
+        g.close()
+    assert_same(runs[rt.SCHED_GRAPH][0], runs[rt.SCHED_PERSISTENT][0])
+    for l in W.fields:
+        for a, b in zip(runs[rt.SCHED_GRAPH][1][l], runs[rt.SCHED_PERSISTENT][1][l]):
+            assert np.array_equal(a, b), l
+
+
+@pytest.mark.parametrize("precision", [rt.FP64, rt.MIXED])
 def test_stars_and_xion_times_same_as_graph(gpu_lib, precision):
     from test_stars import star_setup, workload
 
