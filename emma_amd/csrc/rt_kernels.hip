@@ -729,16 +729,27 @@ __device__ __forceinline__ void tiled_step(const LevelArgs<float>& a, const Chem
       const int nc = a.ncoarse;
       const int j = h.coarse ? -1 - hno : js;
       const int st = h.coarse ? nc : n;
-      const float* pe = h.coarse ? a.ce : a.e;
-      const float* px = h.coarse ? a.cfx : a.fx;
-      const float* py = h.coarse ? a.cfy : a.fy;
-      const float* pz = h.coarse ? a.cfz : a.fz;
+      // The lane's array is this level's plus a per-lane distance (0, or the
+      // coarse array's address minus this level's), both pointers read
+      // unconditionally by scalar loads.  A per-lane choice between the two
+      // argument fields instead becomes a per-lane vector load of the pointer
+      // (one more memory round trip before the data loads), or scalar loads
+      // under a branch each.
+      auto lane_base = [&](const float* own, const float* coarse) {
+        const unsigned long long dist = (unsigned long long)coarse - (unsigned long long)own;
+        return uniform_base(own) + (h.coarse ? dist : 0ull);
+      };
+      const EMMA_GLOBAL char* pe = lane_base(a.e, a.ce);
+      const EMMA_GLOBAL char* px = lane_base(a.fx, a.cfx);
+      const EMMA_GLOBAL char* py = lane_base(a.fy, a.cfy);
+      const EMMA_GLOBAL char* pz = lane_base(a.fz, a.cfz);
 #pragma unroll
       for (int g = 0; g < G; g++) {
-        h.v[g][0] = pe[g * st + j];
-        h.v[g][1] = px[g * st + j];
-        h.v[g][2] = py[g * st + j];
-        h.v[g][3] = pz[g * st + j];
+        const size_t o = byte_offset<float>(g * st + j);
+        h.v[g][0] = *(const EMMA_GLOBAL float*)(pe + o);
+        h.v[g][1] = *(const EMMA_GLOBAL float*)(px + o);
+        h.v[g][2] = *(const EMMA_GLOBAL float*)(py + o);
+        h.v[g][3] = *(const EMMA_GLOBAL float*)(pz + o);
       }
     }
   };
