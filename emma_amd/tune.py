@@ -142,11 +142,11 @@ def load(path: str = TABLE) -> dict:
         return json.load(f)
 
 
-def save(result: dict, workload: str, path: str = TABLE, schedule: dict = None) -> dict:
+def save(result: dict, workload: str, path: str = TABLE, schedule: dict = None, base: str = None) -> dict:
     """merge a tune() result (and a tune_schedule() result) into the table at
     `path` (entries keyed by level / tree shape; `workload` names where they
-    were measured)"""
-    table = load(path)
+    were measured); a table not yet at `path` starts as a copy of `base`"""
+    table = load(path) if os.path.exists(path) or base is None else load(base)
     if schedule is not None:
         table[schedule["key"]] = {"sched": schedule["sched"], "ms": schedule["ms"], "workload": workload}
     for l, r in result.items():
@@ -219,7 +219,7 @@ def main(argv=None) -> int:
         levels = {} if args.schedule_only else tune(g, reps=3, log=log)
         sched = tune_schedule(g, reps=args.reps, log=log)
         log(f"{w}: schedule {sched['sched']} {sched['ms']}")
-        save(levels, names[w], out, sched)
+        save(levels, names[w], out, sched, base=args.table)
         g.close()
     return 0
 

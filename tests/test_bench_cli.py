@@ -30,3 +30,25 @@ def test_algorithmic_bytes_per_cell_update():
 def test_workload_names():
     assert set(bench.WORKLOAD_NAMES) == {"c2", "c3", "c4", "c5"}
     assert "128^3" in bench.WORKLOAD_NAMES["c3"]
+
+
+def test_tuning_table_written_elsewhere_keeps_the_source_entries(tmp_path):
+    """`python -m emma_amd.tune --out NEW`: the new table starts from the
+    table the per-level choices were read from, so a schedule-only run does
+    not drop them; an existing output table is merged into"""
+    import json
+
+    from emma_amd import tune
+
+    src = tmp_path / "src.json"
+    src.write_text(json.dumps({"G3-complete-64": {"variant": "listed", "chem_budget": 8, "chem_keep": 0.0,
+                                                   "us": 1.0, "default_us": 2.0, "workload": "w", "level": 5}}))
+    sched = {"key": "tree-G3-64", "sched": "persistent", "ms": {"graph": 0.7, "persistent": 0.69}}
+    out = tmp_path / "out.json"
+    t = tune.save({}, "w2", str(out), sched, base=str(src))
+    assert set(t) == {"G3-complete-64", "tree-G3-64"}
+    assert json.loads(out.read_text()) == t
+    sched2 = {"key": "tree-G3-128", "sched": "graph", "ms": {"graph": 1.0, "persistent": 1.1}}
+    t2 = tune.save({}, "w3", str(out), sched2, base=str(src))
+    assert set(t2) == {"G3-complete-64", "tree-G3-64", "tree-G3-128"}
+    assert tune.save({}, "w4", str(tmp_path / "fresh.json"), sched) == {"tree-G3-64": t["tree-G3-64"] | {"workload": "w4"}}
