@@ -128,8 +128,8 @@ constexpr int kTileWaves = kTileCells / 64;
 //           which holds on complete levels (16 in x and y, 32 in z).
 //  kPerDir  each lane evaluates its own halo face of direction d at the
 //           start of pass d (up to 192 per wave: sparse AMR levels); the
-//           chemistry inputs are loaded when the chemistry starts (fewer
-//           live registers: four work-groups per CU).
+//           leaves' chemistry inputs are loaded at the start of the last
+//           pass (fewer live registers: four work-groups per CU).
 //  kLatency as kPerDir, but the halo faces of all three directions and the
 //           chemistry inputs are loaded up front: one memory round trip
 //           instead of four, with the registers that takes (two waves per

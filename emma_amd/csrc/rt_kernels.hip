@@ -626,7 +626,7 @@ template <int G, int V>
 __device__ __forceinline__ void tiled_step(const LevelArgs<float>& a, const ChemConst& K, int tile, TileLds<G>& lds,
                                            int lc) {
   constexpr bool PerDir = V != kListed;
-  constexpr bool Late = V == kPerDir;  // chemistry inputs loaded when the chemistry starts
+  constexpr bool Late = V == kPerDir;  // chemistry inputs loaded in the last direction pass
   constexpr bool NT = V == kListed;    // non-temporal next state and chemistry inputs (large levels)
   constexpr int kHaloCap = TileLds<G>::kHaloCap;  // halo slots per wave
   // shares: [g][A or B][cell] of the current direction, then the halo slots
@@ -665,7 +665,8 @@ __device__ __forceinline__ void tiled_step(const LevelArgs<float>& a, const Chem
     }
     // the chemistry inputs: early on complete levels (their latency hides
     // behind the transport); on sparse levels, whose per-direction halo
-    // loads need the registers, when the chemistry starts (finish_leaf)
+    // loads need the registers, at the start of the last direction pass
+    // (load_chem below)
     if constexpr (!Late) {
 #pragma unroll
       for (int g = 0; g < G; g++) Src[g] = gld_in<NT>(a.src + g * n, cell);
@@ -675,6 +676,17 @@ __device__ __forceinline__ void tiled_step(const LevelArgs<float>& a, const Chem
     }
   }
   sCancel[lc] = (unsigned char)(ch >= 0 && !a.keep_split_flux);
+  // sparse levels: the chemistry inputs of a leaf (the only cells whose step
+  // runs the chemistry), issued while the last direction pass computes
+  auto load_chem = [&]() {
+    if (valid && ch < 0) {
+#pragma unroll
+      for (int g = 0; g < G; g++) Src[g] = gld(a.src + g * n, cell);
+      pre[0] = gld(a.nhplus, cell);
+      pre[1] = gld(a.eint, cell);
+      pre[2] = gld(a.nh, cell);
+    }
+  };
 
   const bool transport = a.transport != 0;
   const bool moves = valid && transport && !(ch >= 0 && !a.update_split);
@@ -817,7 +829,10 @@ __device__ __forceinline__ void tiled_step(const LevelArgs<float>& a, const Chem
   if (transport) {
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      if constexpr (V == kPerDir) load_face(hf[0], d);  // latency behind this cell's shares
+      if constexpr (V == kPerDir) {
+        load_face(hf[0], d);  // latency behind this cell's shares
+        if (d == 2) load_chem();
+      }
       if (valid) {
 #pragma unroll
         for (int g = 0; g < G; g++) {
@@ -964,6 +979,8 @@ __device__ __forceinline__ void tiled_step(const LevelArgs<float>& a, const Chem
     }
   }
 
+  if constexpr (Late)
+    if (!transport) load_chem();
   unsigned int nit_acc = 0, leaf = 0;
   if (valid) {
     if (a.t_first != nullptr) xion_times(a, cell);
@@ -974,7 +991,7 @@ __device__ __forceinline__ void tiled_step(const LevelArgs<float>& a, const Chem
     } else {
       leaf = 1;
       nit_acc = (unsigned)finish_leaf<float, G, true, kRefluxTiled, NT>(a, K, cell, ch, G, Ue, Ux, Uy, Uz, De, Dx, Dy, Dz,
-                                                                    Late ? nullptr : Src, Late ? nullptr : pre);
+                                                                    Src, pre);
     }
   }
   if (a.counters != nullptr) block_counters<kTileWaves>(a.counters, nit_acc, leaf);
