@@ -239,13 +239,19 @@ def run_uniform(W, precision, steps, warmup, tuning=None):
     # buffer parity, so the graph of the other parity is captured as well)
     start = g.get(lvl)
     g.coarse_step(1.0, steps)
+    st = g.stats()  # counted on the device (cell updates, chemistry sub-steps)
+    _check_counted(st, W.leaf_cell_updates_per_coarse_step() * steps)
+    # timed without the kernels' diagnostic counters (same results bit for
+    # bit; their graphs are captured here, untimed)
+    g.enable_counters(False)
+    g.set(lvl, 0, start)
+    g.coarse_step(1.0, steps)
     if steps % 2:
         g.coarse_step(1.0, steps)
     g.set(lvl, 0, start)
     t0 = time.perf_counter()
     g.coarse_step(1.0, steps)  # single level: `steps` sub-cycles, one launch each
     wall = time.perf_counter() - t0
-    st = g.stats()
     # the same replay again, between one pair of HIP events
     g.set(lvl, 0, start)
     g.enable_timing(rt.TIMING_GRAPH)
@@ -260,6 +266,14 @@ def run_uniform(W, precision, steps, warmup, tuning=None):
     return {"wall": wall, "stats": st, "graph_ms": graph_ms, "level_us": {lvl: ms / max(n, 1) * 1e3},
             "launches_per_step": {lvl: 1}, "steps": steps, "alg_bytes_per_step": level_bytes(W, precision)[lvl],
             "tuning": used}
+
+
+def _check_counted(stats, expected):
+    """the device's count of leaf cell updates is the host's (every leaf of
+    every level step advanced)"""
+    if stats["status"] != 0 or stats["cell_updates"] != expected:
+        raise RuntimeError(f"device counted {stats['cell_updates']} leaf cell updates (status {stats['status']}), "
+                           f"expected {expected}")
 
 
 def run_amr(W, precision, steps, warmup, tuning=None, tune_here=False, log=None, schedule=None):
@@ -294,20 +308,27 @@ def run_amr(W, precision, steps, warmup, tuning=None, tune_here=False, log=None,
         g.set_schedule({v: k for k, v in rt.SCHED_NAMES.items()}[schedule])
     sched = g.schedule()["sched"]
 
+    # one coarse step counted on the device (cell updates, chemistry
+    # sub-steps), untimed; the timed ones run without the kernels' diagnostic
+    # counters (same results bit for bit)
+    restore()
+    g.coarse_step(1.0, 1)
+    counted = g.stats()
+    _check_counted(counted, W.leaf_cell_updates_per_coarse_step())
+    g.enable_counters(False)
     # capture + instantiate the graphs of both buffer parities of the coarse level
     for _ in range(max(warmup, 2)):
         restore()
         g.coarse_step(1.0, 1)
     wall = 0.0
-    tot = {"cell_updates": 0, "chem_substeps": 0, "level_steps": 0}
     for _ in range(steps):
         restore()
         t0 = time.perf_counter()
         g.coarse_step(1.0, 1)  # timed: one graph replay
         wall += time.perf_counter() - t0
-        st = g.stats()
-        for k in tot:
-            tot[k] += st[k]
+        if g.stats()["status"] != 0:
+            raise RuntimeError(f"coarse step status {g.stats()['status']}")
+    tot = {k: counted[k] * steps for k in ("cell_updates", "chem_substeps", "level_steps")}
     # the same replays again, each between one pair of HIP events
     g.enable_timing(rt.TIMING_GRAPH)
     graph_ms = 0.0

@@ -127,8 +127,10 @@ struct GraphKey {
   int xion;
   int mode;
   int sched;
+  int counters;
   bool operator<(const GraphKey& o) const {
     if (sched != o.sched) return sched < o.sched;
+    if (counters != o.counters) return counters < o.counters;
     if (mode != o.mode) return mode < o.mode;
     if (ncoarse != o.ncoarse) return ncoarse < o.ncoarse;
     if (aexp != o.aexp) return aexp < o.aexp;
@@ -164,6 +166,7 @@ struct emma_rt_ctx {
   int defer_seq = 0;
   emma_rt_stats stats;
   int timing = 0;
+  int counters = 1;  // the kernels' device counters (sub-steps, leaves, deferred cells)
   int xion = 0;
   double tphy = 0.;
   // FillRad / final-update coupling (WRADHYD) and the Stromgren test source
@@ -888,7 +891,7 @@ template <typename R>
 static void record_step(emma_rt_ctx* c, int k, double dtnew, double aexp, int chemonly) {
   StepDesc<R> D;
   memset(&D, 0, sizeof(D));
-  D.a = level_args<R>(c, k, dtnew, aexp, chemonly, true);
+  D.a = level_args<R>(c, k, dtnew, aexp, chemonly, c->counters != 0);
   // no straggler pass inside the persistent kernel: every leaf finishes its
   // chemistry in its tile
   D.a.chem_budget = 0;
@@ -930,7 +933,7 @@ static int timed_launch(emma_rt_ctx* c, int k, double dtnew, double aexp, int ch
     c->ev_level.push_back(k);
     HIPCHK(hipEventRecord(e0, c->stream));
   }
-  HIPCHK(launch_level(c, k, dtnew, aexp, chemonly, true));
+  HIPCHK(launch_level(c, k, dtnew, aexp, chemonly, c->counters != 0));
   if (c->timing) HIPCHK(hipEventRecord(e1, c->stream));
   c->stats.level_steps++;
   return EMMA_RT_OK;
@@ -985,9 +988,10 @@ static int end_call(emma_rt_ctx* c) {
   c->stats.status = status[0];
   c->stats.bad_level = status[1];
   c->stats.bad_cell = status[2];
-  c->stats.chem_substeps = (long long)cnt[0];
-  c->stats.cell_updates = (long long)cnt[1];
-  c->stats.deferred_cells = (long long)cnt[2];
+  const bool counted = c->counters != 0;
+  c->stats.chem_substeps = counted ? (long long)cnt[0] : -1;
+  c->stats.cell_updates = counted ? (long long)cnt[1] : -1;
+  c->stats.deferred_cells = counted ? (long long)cnt[2] : -1;
   double ms = 0.;
   for (int k = 0; k < kMaxLevels; k++) {
     c->level_ms[k] = 0.;
@@ -1598,6 +1602,7 @@ extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
   key.xion = c->xion;
   key.mode = c->mode;
   key.sched = c->sched;
+  key.counters = c->counters;
   for (auto& L : c->lev) key.parity.push_back(L.cur);
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
@@ -1812,6 +1817,12 @@ extern "C" int emma_rt_get_chem_budget(emma_rt_ctx* c, int level, int* budget) {
 extern "C" int emma_rt_enable_timing(emma_rt_ctx* c, int enable) {
   if (c == nullptr || enable < 0 || enable > 2) return EMMA_RT_ERR_ARG;
   c->timing = enable;
+  return EMMA_RT_OK;
+}
+
+extern "C" int emma_rt_enable_counters(emma_rt_ctx* c, int enable) {
+  if (c == nullptr || enable < 0 || enable > 1) return EMMA_RT_ERR_ARG;
+  c->counters = enable;
   return EMMA_RT_OK;
 }
 

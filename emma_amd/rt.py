@@ -72,7 +72,7 @@ EXPORTS = [
     "emma_rt_advance_stencil", "emma_rt_create_pinned_stencil", "emma_rt_destroy_pinned_stencil",
     "emma_rt_tree_create", "emma_rt_set_level", "emma_rt_get_level", "emma_rt_enable_xion_times",
     "emma_rt_get_xion_times", "emma_rt_set_tphy", "emma_rt_radsolver", "emma_rt_coarse_step",
-    "emma_rt_get_stats", "emma_rt_enable_timing", "emma_rt_set_mode", "emma_rt_fillrad",
+    "emma_rt_get_stats", "emma_rt_enable_timing", "emma_rt_enable_counters", "emma_rt_set_mode", "emma_rt_fillrad",
     "emma_rt_set_hydro", "emma_rt_get_hydro", "emma_rt_set_coupling", "emma_rt_set_oct_pos",
     "emma_rt_set_stromgren_source", "emma_rt_get_level_times", "emma_rt_alloc_pinned",
     "emma_rt_save_state", "emma_rt_restore_state", "emma_rt_set_stars", "emma_rt_set_star_sources",
@@ -164,6 +164,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.emma_rt_coarse_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
     L.emma_rt_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
     L.emma_rt_enable_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.emma_rt_enable_counters.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.emma_rt_set_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.emma_rt_fillrad.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
     L.emma_rt_set_hydro.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_dp] * 6
@@ -294,6 +295,7 @@ class RadGPU:
         self._check(self.lib.emma_rt_create(ctypes.byref(self._cp), ctypes.byref(h)))
         self.h = h
         self.tree = None
+        self.counters = True  # the library's default (emma_rt_enable_counters)
 
     def _check(self, st, stats=None):
         if st != OK:
@@ -374,6 +376,13 @@ class RadGPU:
         their graph between one pair of events); True/False map to
         TIMING_LAUNCH/TIMING_OFF"""
         self._check(self.lib.emma_rt_enable_timing(self.h, int(mode)))
+
+    def enable_counters(self, on: bool = True):
+        """the kernels' device counters (stats' cell_updates, chem_substeps,
+        deferred_cells; on by default).  Off, the level steps skip their
+        per-tile reduction and those fields read -1; same results bit for bit"""
+        self._check(self.lib.emma_rt_enable_counters(self.h, int(bool(on))))
+        self.counters = bool(on)
 
     def set_chem_budget(self, budget: int, level: int = -1):
         """chemistry tries before a leaf goes to the straggler pass (0: off),

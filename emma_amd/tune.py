@@ -42,6 +42,20 @@ def tree_key(ngrp: int, tree) -> str:
     return f"tree-G{ngrp}-" + "-".join(str(tree.level(l).ncell) for l in range(tree.lmin, tree.lmax + 1))
 
 
+def _uncounted(f):
+    """run f(g, ...) with the kernels' diagnostic counters off, as the bench
+    times its coarse steps, and restore the context's setting"""
+    def run(g, *args):
+        was = getattr(g, "counters", True)
+        g.enable_counters(False)
+        try:
+            return f(g, *args)
+        finally:
+            g.enable_counters(was)
+    return run
+
+
+@_uncounted
 def _step_ms(g, reps):
     """mean device time (ms) of one coarse step (one event pair around each
     replay) over `reps` coarse steps from the saved state, after two untimed
@@ -74,6 +88,7 @@ def tune_schedule(g: "rt.RadGPU", reps: int = 5, log=None) -> dict:
             "ms": {rt.SCHED_NAMES[k]: round(v, 4) for k, v in t.items()}}
 
 
+@_uncounted
 def _level_times(g, levels, reps):
     """mean device time (us) of each level's step over `reps` coarse steps
     from the saved state"""

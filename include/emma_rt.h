@@ -91,7 +91,9 @@ typedef struct emma_rt_ctx emma_rt_ctx;
 
 /* Counters of the last radsolver / coarse-step call. */
 typedef struct emma_rt_stats {
-  long long cell_updates;   /* leaf cells advanced (one per leaf per level step) */
+  long long cell_updates;   /* leaf cells advanced (one per leaf per level step); -1 with the
+                               counters off (emma_rt_enable_counters), as chem_substeps and
+                               deferred_cells */
   long long chem_substeps;  /* accepted chemistry sub-steps, summed over cells */
   long long level_steps;    /* level-step kernel launches */
   double kernel_ms;         /* device time (HIP events), see emma_rt_enable_timing */
@@ -453,6 +455,13 @@ int emma_rt_get_stats(emma_rt_ctx *ctx, emma_rt_stats *out);
 #define EMMA_RT_TIMING_LAUNCH 1
 #define EMMA_RT_TIMING_GRAPH 2
 int emma_rt_enable_timing(emma_rt_ctx *ctx, int enable);
+/* the kernels' device counters behind emma_rt_stats.cell_updates,
+   chem_substeps and deferred_cells (on by default: enable = 1).  Off (0),
+   the level steps skip their per-tile reduction (a barrier and a few atomics
+   per 256 cells) and those three fields read -1; the results are the same
+   bit for bit.  No counterpart in the reference, whose RadSolver counts
+   nothing on the device. */
+int emma_rt_enable_counters(emma_rt_ctx *ctx, int enable);
 /* per-level device time of the last call's event-timed level-step launches
    (EMMA_RT_TIMING_LAUNCH, or single level steps): ms[k] and launches[k] for
    level lmin + k, k < nlev (either pointer may be NULL).  The reference times

@@ -284,3 +284,26 @@ def test_full_size_same_as_graph(gpu_lib, config):
               flush=True)
         g.close()
     assert_same(res[rt.SCHED_GRAPH], res[rt.SCHED_PERSISTENT])
+
+
+@pytest.mark.parametrize("sched", [rt.SCHED_GRAPH, rt.SCHED_PERSISTENT], ids=["graph", "persistent"])
+def test_counters_off_same_results(gpu_lib, sched):
+    """with the kernels' diagnostic counters off the coarse steps give the
+    same state bit for bit and the counter fields read -1; on again, the
+    counts are back (the host's leaf count, and sub-steps)"""
+    W = small("Starburst99_1grpT_3grpS", seed=5)
+
+    def counters(on):
+        def prepare(g):
+            g.enable_counters(on)
+        return prepare
+
+    res = [run(W, rt.MIXED, sched, prepare=counters(on)) for on in (True, False, True)]
+    for (out, st, _), on in zip(res, (True, False, True)):
+        assert st["status"] == 0, st
+        if on:
+            assert st["cell_updates"] == W.leaf_cell_updates_per_coarse_step() and st["chem_substeps"] > 0, st
+        else:
+            assert st["cell_updates"] == -1 and st["chem_substeps"] == -1 and st["deferred_cells"] == -1, st
+    assert_same(res[0][0], res[1][0])
+    assert_same(res[0][0], res[2][0])
