@@ -153,7 +153,10 @@ struct emma_rt_ctx {
   std::vector<DevLevel> lev;  // index level - lmin
   int* d_status = nullptr;
   unsigned long long* d_counters = nullptr;
-  unsigned long long h_counters[kCounterSlots * kCounterStride];  // read back by end_call
+  // page-locked copies of the status word and the counters, read back by
+  // end_call with asynchronous copies behind the call's work and one wait
+  int* h_status = nullptr;
+  unsigned long long* h_counters = nullptr;
   void* scratch = nullptr;  // this device's no-alloc staging area (emma_rt_advance_stencil_noalloc)
   double* d_photons = nullptr;
   // straggler list of the tiled mixed kernel (capacity: the largest level's
@@ -402,8 +405,11 @@ extern "C" int emma_rt_create(const emma_rt_params* p, emma_rt_ctx** out) {
   if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
       hipMalloc((void**)&c->d_status, 4 * sizeof(int)) != hipSuccess ||
       hipMalloc((void**)&c->d_counters, kCounterSlots * kCounterStride * sizeof(unsigned long long)) != hipSuccess ||
-      hipMalloc((void**)&c->d_photons, sizeof(double)) != hipSuccess) {
-    delete c;
+      hipMalloc((void**)&c->d_photons, sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_status, 4 * sizeof(int)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_counters, kCounterSlots * kCounterStride * sizeof(unsigned long long)) !=
+          hipSuccess) {
+    emma_rt_destroy(c);
     return EMMA_RT_ERR_HIP;
   }
   if (launch_fill((unsigned int*)c->d_status, 4, 0u, c->stream) != hipSuccess ||
@@ -443,7 +449,9 @@ extern "C" int emma_rt_destroy(emma_rt_ctx* c) {
   (void)hipFree(c->d_status);
   (void)hipFree(c->d_counters);
   (void)hipFree(c->d_photons);
-  (void)hipStreamDestroy(c->stream);
+  if (c->h_status) (void)hipHostFree(c->h_status);
+  if (c->h_counters) (void)hipHostFree(c->h_counters);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return EMMA_RT_OK;
 }
@@ -966,7 +974,7 @@ static int begin_call(emma_rt_ctx* c) {
   HIPCHK(launch_fill((unsigned int*)c->d_status, 4, 0u, c->stream));
   c->defer_seq = 0;
   if (c->defer_slots > 0) HIPCHK(launch_fill((unsigned int*)c->d_defer_ctl, c->defer_slots, 0u, c->stream));
-  HIPCHK(launch_fill((unsigned int*)c->d_counters, 2 * kCounterSlots * kCounterStride, 0u, c->stream));
+  if (c->counters) HIPCHK(launch_fill((unsigned int*)c->d_counters, 2 * kCounterSlots * kCounterStride, 0u, c->stream));
   memset(&c->stats, 0, sizeof(c->stats));
   c->ev_used = 0;
   c->ev_level.clear();
@@ -974,21 +982,26 @@ static int begin_call(emma_rt_ctx* c) {
 }
 
 static int end_call(emma_rt_ctx* c) {
+  // the status word (and the counters) copied behind the call's work, one wait
+  const bool counted = c->counters != 0;
+  HIPCHK(hipMemcpyAsync(c->h_status, c->d_status, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (counted)
+    HIPCHK(hipMemcpyAsync(c->h_counters, c->d_counters, kCounterSlots * kCounterStride * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  int status[4];
+  const int* status = c->h_status;
   const unsigned long long* slots = c->h_counters;
-  HIPCHK(hipMemcpy(status, c->d_status, sizeof(status), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(c->h_counters, c->d_counters, sizeof(c->h_counters), hipMemcpyDeviceToHost));
   unsigned long long cnt[3] = {0ull, 0ull, 0ull};
-  for (int i = 0; i < kCounterSlots; i++) {
-    cnt[0] += slots[(size_t)i * kCounterStride + 0];
-    cnt[1] += slots[(size_t)i * kCounterStride + 1];
-    cnt[2] += slots[(size_t)i * kCounterStride + 2];
+  if (counted) {
+    for (int i = 0; i < kCounterSlots; i++) {
+      cnt[0] += slots[(size_t)i * kCounterStride + 0];
+      cnt[1] += slots[(size_t)i * kCounterStride + 1];
+      cnt[2] += slots[(size_t)i * kCounterStride + 2];
+    }
   }
   c->stats.status = status[0];
   c->stats.bad_level = status[1];
   c->stats.bad_cell = status[2];
-  const bool counted = c->counters != 0;
   c->stats.chem_substeps = counted ? (long long)cnt[0] : -1;
   c->stats.cell_updates = counted ? (long long)cnt[1] : -1;
   c->stats.deferred_cells = counted ? (long long)cnt[2] : -1;
@@ -2013,7 +2026,7 @@ static int advance_stencil_on(emma_rt_ctx* c, void* dbuf, int real_bytes, void* 
   const ChemConst K = make_chem(c->p, dtnew, aexp);
   const double cl = cloc_of(c->p, aexp);
   HIPCHK(launch_stencil(dbuf, L, c->p.ngrp, real_bytes, nread, cl, dtnew / dxcur, chemonly, coarserad, transport, K,
-                        c->d_status, c->d_counters, c->stream));
+                        c->d_status, c->counters ? c->d_counters : nullptr, c->stream));
   // without transport the kernel writes no fluxes: rfieldnew alone comes back
   const size_t nback = transport ? back : rtype;
   for (int ic = 0; ic < 8; ic++) {
