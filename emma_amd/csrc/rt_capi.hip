@@ -595,7 +595,7 @@ extern "C" int emma_rt_tree_create(emma_rt_ctx* c, int lmin, int lmax, const int
     L.halo_per_dir = maxh > 64 ? 1 : 0;
     L.variant = default_variant(L);
     L.morton_m = (k == 0) ? morton_cube(L.noct, nei) : 0;
-    L.raster = L.morton_m > 0;
+    L.raster = 0;  // Morton order unless the caller (or the tuner) picks raster
   }
   // device allocation and upload
   for (int k = 0; k < nl; k++) {

@@ -10,7 +10,9 @@ per-level choices are
 * the chemistry sub-step budget after which a leaf goes to the straggler
   pass (``emma_rt_set_chem_budget``; 0 = no pass), and the factor beyond
   which a leaf's chain counts as long and stays in the first pass
-  (``emma_rt_set_chem_keep``),
+  (``emma_rt_set_chem_keep``), and
+* on a complete Morton-ordered cube, the tile order of the tiled kernel
+  (``emma_rt_set_level_order``: Morton or raster),
 
 which change the speed and not the results.  :func:`tune` times every
 candidate on the workload's own state (event-timed level steps of repeated
@@ -32,6 +34,9 @@ from . import rt
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 BUDGETS = (0, 2, 3, 4, 6, 8)
 KEEPS = (0.0, 8.0, 32.0)
+
+
+ORDER_NAMES = {rt.ORDER_MORTON: "morton", rt.ORDER_RASTER: "raster"}
 
 
 def level_key(ngrp: int, ncell: int, halo_per_dir: bool) -> str:
@@ -147,6 +152,23 @@ def tune(g: "rt.RadGPU", reps: int = 3, budgets=BUDGETS, keeps=KEEPS, log=None) 
                               "chem_keep": default[l][2]},
                   "candidates": {f"{rt.VARIANT_NAMES[cv]}/{cb}/{ck:g}": round(ct, 2)
                                  for (cv, cb, ck), ct in cands[l].items()}}
+    # the tile order of a complete Morton cube, with the level's choices
+    # above: Morton order keeps more of the halo in L2 on some cubes, the
+    # raster order on others
+    for l in levels:
+        if g.level_order(l)["cube_side_log2"] == 0:
+            continue
+        t_o = {}
+        for o in (rt.ORDER_MORTON, rt.ORDER_RASTER):
+            g.set_level_order(o, level=l)
+            t_o[o] = _level_times(g, [l], reps)[l]
+        best = min(t_o, key=t_o.get)
+        g.set_level_order(best, level=l)
+        out[l]["order"] = ORDER_NAMES[best]
+        out[l]["order_us"] = {ORDER_NAMES[o]: round(t, 2) for o, t in t_o.items()}
+        out[l]["us"] = t_o[best]
+        if log:
+            log(f"tune: level {l} tile order " + ", ".join(f"{ORDER_NAMES[o]} {t:.1f} us" for o, t in t_o.items()))
     return out
 
 
@@ -168,6 +190,8 @@ def save(result: dict, workload: str, path: str = TABLE, schedule: dict = None, 
         table[r["key"]] = {"variant": r["variant"], "chem_budget": r["chem_budget"],
                            "chem_keep": r.get("chem_keep", 0.0), "us": round(r["us"], 2),
                            "default_us": round(r["default_us"], 2), "workload": workload, "level": int(l)}
+        if "order" in r:
+            table[r["key"]]["order"] = r["order"]
     with open(path, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
         f.write("\n")
@@ -197,6 +221,8 @@ def apply(g: "rt.RadGPU", table: dict = None) -> dict:
         g.set_level_variant(names[e["variant"]], level=l)
         g.set_chem_budget(int(e["chem_budget"]), level=l)
         g.set_chem_keep(float(e.get("chem_keep", 0.0)), level=l)
+        if "order" in e and g.level_order(l)["cube_side_log2"] > 0:
+            g.set_level_order({v: k for k, v in ORDER_NAMES.items()}[e["order"]], level=l)
         used[l] = e
     return used
 

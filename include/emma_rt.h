@@ -396,12 +396,13 @@ int emma_rt_grid_field(emma_rt_ctx *ctx, int field_id, int rank, float *out, lon
    EMMA_RT_TIMING_LAUNCH each persistent launch is one timed span. */
 /* Tile order of the tiled mixed kernel on a complete periodic cube of 2^m
    octs a side stored in Morton order (the library checks the level's
-   nei_oct against that layout at emma_rt_tree_create): EMMA_RT_ORDER_RASTER
-   (the default there) walks each XCD's octant of 4 x 4 x 2-oct tiles z
-   fastest, then y, then x, so that the neighbours across a tile's larger
-   faces are processed next to it and their halo cells are still in that
-   XCD's L2; EMMA_RT_ORDER_MORTON keeps the tiles' Morton order.  Speed only,
-   results unchanged.  level < 0: every level that can take it.
+   nei_oct against that layout at emma_rt_tree_create): EMMA_RT_ORDER_MORTON
+   (the default) keeps the tiles' Morton order; EMMA_RT_ORDER_RASTER walks
+   each XCD's octant of 4 x 4 x 2-oct tiles z fastest, then y, then x, so
+   that the neighbours across a tile's larger faces are processed next to it
+   (the autotuner picks per cube; Morton order read less from HBM on the
+   128^3 and 256^3 cubes measured).  Speed only, results unchanged.
+   level < 0: every level that can take it.
    EMMA_RT_ERR_ARG for RASTER on a level that is not such a cube. */
 #define EMMA_RT_ORDER_MORTON 0
 #define EMMA_RT_ORDER_RASTER 1

@@ -111,15 +111,47 @@ def test_tuner(gpu_lib, tmp_path):
     used = tune.apply(g2, tune.load(path))
     assert set(used) == set(res)
     names = {v: k for k, v in rt.VARIANT_NAMES.items()}
+    orders = {v: k for k, v in tune.ORDER_NAMES.items()}
     for l, r in res.items():
         t = g2.level_tuning(l)
         assert t["variant"] == names[r["variant"]] and t["chem_budget"] == r["chem_budget"]
+        cube = g2.level_order(l)["cube_side_log2"] > 0
+        assert ("order" in r) == cube, (l, r)
+        if cube:  # the tile order of a complete cube is tuned too
+            assert set(r["order_us"]) == {"morton", "raster"}, r
+            assert abs(r["us"] - min(r["order_us"].values())) < 0.01, r
+            assert g2.level_order(l)["order"] == orders[r["order"]], (l, r)
+
+
+def test_tuner_tile_order(gpu_lib, tmp_path):
+    """on a complete Morton-ordered cube the tuner also picks the tile order,
+    and the saved table sets it"""
+    W = synth.uniform_workload(5, "Starburst99_1grpT_3grpS", seed=19, nsrc=6)
+    g = rt.RadGPU(W.params, rt.MIXED)
+    g.setup(W.params, W.tree, W.fields)
+    g.coarse_step(1.0, 1)
+    g.save_state()
+    res = tune.tune(g, reps=1, budgets=(0,))
+    r = res[W.tree.lmin]
+    assert r["order"] in ("morton", "raster") and set(r["order_us"]) == {"morton", "raster"}, r
+    path = str(tmp_path / "t.json")
+    tune.save(res, "cube", path)
+    for order in ("morton", "raster"):  # either entry is applied as written
+        table = tune.load(path)
+        table[r["key"]]["order"] = order
+        g2 = rt.RadGPU(W.params, rt.MIXED)
+        g2.setup(W.params, W.tree, W.fields)
+        tune.apply(g2, table)
+        assert g2.level_order(W.tree.lmin)["order"] == {v: k for k, v in tune.ORDER_NAMES.items()}[order]
+        g2.close()
+    g.close()
 
 
 @pytest.mark.parametrize("amr", [False, True])
 def test_raster_tile_order(gpu_lib, amr):
-    """A complete Morton-ordered cube takes the raster tile order by default;
-    the order changes which block runs which tile, not the results"""
+    """A complete Morton-ordered cube takes the Morton tile order by default
+    and can take the raster order; the order changes which block runs which
+    tile, not the results"""
     if amr:
         W = synth.amr_workload(4, [300, 500], "Starburst99_1grpT_3grpS", seed=19, sigma=1.5, nsrc=3, grid=16)
     else:
@@ -129,7 +161,7 @@ def test_raster_tile_order(gpu_lib, amr):
         g = rt.RadGPU(W.params, rt.MIXED)
         g.setup(W.params, W.tree, W.fields)
         info = g.level_order(W.tree.lmin)
-        assert info["cube_side_log2"] == W.tree.lmin - 1 and info["order"] == rt.ORDER_RASTER, info
+        assert info["cube_side_log2"] == W.tree.lmin - 1 and info["order"] == rt.ORDER_MORTON, info
         if W.tree.lmax > W.tree.lmin:
             assert g.level_order(W.tree.lmax)["cube_side_log2"] == 0
             with pytest.raises(rt.EmmaRTError):
