@@ -953,6 +953,8 @@ static int ensure_defer_slots(emma_rt_ctx* c, long long n) {
   if (c->d_defer == nullptr || n <= c->defer_slots) return EMMA_RT_OK;
   if (n > (1LL << 24)) return EMMA_RT_ERR_ARG;
   HIPCHK(hipStreamSynchronize(c->stream));
+  // captured coarse steps hold the old list-length words
+  drop_graphs(c);
   (void)hipFree(c->d_defer_ctl);
   c->d_defer_ctl = nullptr;
   c->defer_slots = 0;

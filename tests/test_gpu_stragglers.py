@@ -114,3 +114,29 @@ def test_budget_args(gpu_lib):
     assert g.chem_budget(3) == 4
     with pytest.raises(rt.EmmaRTError):
         g.set_chem_keep(-1.0)
+
+
+def test_budget_after_list_words_grow(gpu_lib):
+    """A call with more level steps than there are straggler list words
+    reallocates them; a coarse-step shape captured before (its graph held the
+    old words) is captured again, and the pass still gives the run without it
+    (one coarse step, then 40 in one call: 280 level steps against 256 words,
+    then one again)"""
+    W = synth.amr_workload(3, [60, 100, 80], "Starburst99_1grpT_3grpS", seed=11, sigma=1.5, nsrc=2, grid=16)
+    runs = []
+    for budget in (0, 2):
+        g = rt.RadGPU(W.params, rt.MIXED)
+        g.setup(W.params, W.tree, W.fields)
+        if budget:
+            g.set_chem_budget(budget)
+        sub = []
+        for n in (1, 40, 1):
+            g.coarse_step(1.0, n)
+            st = g.stats()
+            assert st["status"] == 0, (budget, n, st)
+            sub.append(st["chem_substeps"])
+        runs.append((g, sub))
+    (g0, s0), (g1, s1) = runs
+    assert s1 == s0, (s0, s1)
+    for l in range(W.tree.lmin, W.tree.lmax + 1):
+        _same(g1.get(l), g0.get(l), W.params.cloc())
