@@ -972,11 +972,23 @@ static long long steps_per_call(emma_rt_ctx* c, int ncoarse) {
   return steps;
 }
 
-static int begin_call(emma_rt_ctx* c) {
+// the device words a call starts from: status, straggler list lengths,
+// counters (fill kernels, so that a coarse step can capture them at the
+// head of its graph)
+static int clear_call_words(emma_rt_ctx* c) {
   HIPCHK(launch_fill((unsigned int*)c->d_status, 4, 0u, c->stream));
-  c->defer_seq = 0;
   if (c->defer_slots > 0) HIPCHK(launch_fill((unsigned int*)c->d_defer_ctl, c->defer_slots, 0u, c->stream));
   if (c->counters) HIPCHK(launch_fill((unsigned int*)c->d_counters, 2 * kCounterSlots * kCounterStride, 0u, c->stream));
+  return EMMA_RT_OK;
+}
+
+// clear_device = false: the caller runs clear_call_words inside its graph
+static int begin_call(emma_rt_ctx* c, bool clear_device = true) {
+  if (clear_device) {
+    int st = clear_call_words(c);
+    if (st) return st;
+  }
+  c->defer_seq = 0;
   memset(&c->stats, 0, sizeof(c->stats));
   c->ev_used = 0;
   c->ev_level.clear();
@@ -1602,9 +1614,12 @@ extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
   if (st) return st;
   st = ensure_defer_slots(c, steps_per_call(c, ncoarse));
   if (st) return st;
-  st = begin_call(c);
+  // replays of the captured coarse step clear the call's device words at
+  // the head of the graph; eager launches (TIMING_LAUNCH) clear them here
+  const bool eager = c->timing == 1;
+  st = begin_call(c, eager);
   if (st) return st;
-  if (c->timing == 1 && c->sched == EMMA_RT_SCHED_GRAPH) {  // eager launches with per-kernel events
+  if (eager && c->sched == EMMA_RT_SCHED_GRAPH) {  // eager launches with per-kernel events
     for (int i = 0; i < ncoarse; i++) {
       st = recurse(c, 0, aexp);
       if (st) return st;
@@ -1634,7 +1649,8 @@ extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
         free_cached(e);
         HIPCHK(be);
       }
-      rs = run_cached(c, e, false);
+      rs = clear_call_words(c);
+      if (rs == EMMA_RT_OK) rs = run_cached(c, e, false);
       hipError_t ce = hipStreamEndCapture(c->stream, &g);
       if (rs || ce != hipSuccess) {
         if (ce == hipSuccess) (void)hipGraphDestroy(g);
@@ -1644,7 +1660,7 @@ extern "C" int emma_rt_coarse_step(emma_rt_ctx* c, double aexp, int ncoarse) {
       }
     } else {
       HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      int rs = EMMA_RT_OK;
+      int rs = clear_call_words(c);
       const int timing = c->timing;
       c->timing = 0;  // no event records inside the graph
       for (int i = 0; i < ncoarse && rs == EMMA_RT_OK; i++) rs = recurse(c, 0, aexp);

@@ -259,6 +259,36 @@ def test_persistent_status_emin(gpu_lib):
     assert codes[0] == codes[1] == rt.ERR_EMIN, codes
 
 
+@pytest.mark.parametrize("sched", [rt.SCHED_GRAPH, rt.SCHED_PERSISTENT], ids=["graph", "persistent"])
+def test_replays_start_from_cleared_words(gpu_lib, sched):
+    """a replayed coarse step clears the status word, the straggler list
+    lengths and the counters at the head of its graph: replays of the same
+    graphs alternate between a state that raises EMIN and one that does not,
+    and each reports its own outcome and counts"""
+    W = small("Starburst99_1grpT_3grpS", 7)
+    bad = {l: F.copy() for l, F in W.fields.items()}
+    for F in bad.values():
+        F.e[:] = 1e-30
+        F.fx[:] = F.fy[:] = F.fz[:] = 0.0
+    g = rt.RadGPU(W.params, rt.MIXED)
+    g.setup(W.params, W.tree, W.fields)
+    g.set_schedule(sched)
+    g.set_chem_budget(2)
+    for rep in range(3):
+        for fields, ok in ((bad, False), (W.fields, True)):
+            for l, F in fields.items():
+                g.set(l, 0, F)
+            if ok:
+                g.coarse_step(1.0, 1)
+                st = g.stats()
+                assert st["status"] == 0 and st["cell_updates"] == W.leaf_cell_updates_per_coarse_step(), (rep, st)
+            else:
+                with pytest.raises(rt.EmmaRTError) as ei:
+                    g.coarse_step(1.0, 1)
+                assert ei.value.status == rt.ERR_EMIN, (rep, ei.value.status)
+    g.close()
+
+
 @pytest.mark.parametrize("config", ["c4", "c5"])
 def test_full_size_same_as_graph(gpu_lib, config):
     """the benchmark trees (C4: levels 7-11, 31 level steps; C5: 256^3 + two
