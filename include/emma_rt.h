@@ -451,7 +451,7 @@ int emma_rt_get_stats(emma_rt_ctx *ctx, emma_rt_stats *out);
    eagerly, not from the captured graph); EMMA_RT_TIMING_GRAPH does the same
    for single level steps, and coarse steps replay their graph between one
    pair of events (the replay's device time, launch gaps inside the graph
-   included) */
+   and the clearing of the call's status words at its head included) */
 #define EMMA_RT_TIMING_OFF 0
 #define EMMA_RT_TIMING_LAUNCH 1
 #define EMMA_RT_TIMING_GRAPH 2
