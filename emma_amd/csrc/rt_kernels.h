@@ -138,6 +138,11 @@ constexpr int kTileWaves = kTileCells / 64;
 constexpr int kListed = 0;
 constexpr int kPerDir = 1;
 constexpr int kLatency = 2;
+// kListed on a level with no coarser level (the tree's coarsest): no coarse
+// faces and no injected leaves, so the code for them is compiled out.  The
+// kernels pick it from the level's arguments (ncoarse == 0); not a caller's
+// choice.
+constexpr int kListedTop = 3;
 // grid of the chemistry straggler pass (it loops over the list)
 constexpr int kStragglerThreads = 256;
 constexpr int kStragglerBlocks = 4096;

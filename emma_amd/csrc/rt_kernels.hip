@@ -625,9 +625,11 @@ struct HaloFace {
 template <int G, int V>
 __device__ __forceinline__ void tiled_step(const LevelArgs<float>& a, const ChemConst& K, int tile, TileLds<G>& lds,
                                            int lc) {
-  constexpr bool PerDir = V != kListed;
+  constexpr bool Listed = V == kListed || V == kListedTop;
+  constexpr bool PerDir = !Listed;
   constexpr bool Late = V == kPerDir;  // chemistry inputs loaded in the last direction pass
-  constexpr bool NT = V == kListed;    // non-temporal next state and chemistry inputs (large levels)
+  constexpr bool NT = Listed;          // non-temporal next state and chemistry inputs (large levels)
+  constexpr bool Top = V == kListedTop;  // no coarse faces on the level
   constexpr int kHaloCap = TileLds<G>::kHaloCap;  // halo slots per wave
   // shares: [g][A or B][cell] of the current direction, then the halo slots
   // [wave][slot][g]
@@ -729,7 +731,7 @@ __device__ __forceinline__ void tiled_step(const LevelArgs<float>& a, const Chem
     const int js = hbound ? (hcell & ~7) + trans_source(2 * h.d + h.side, hic)
                           : ((h.lane && !h.coarse) ? hno * 8 + (hic ^ (1 << h.d)) : 0);
     h.child = gld(a.child, js);
-    if (__ballot(h.coarse) == 0) {
+    if (Top || __ballot(h.coarse) == 0) {
 #pragma unroll
       for (int g = 0; g < G; g++) {
         h.v[g][0] = gld(a.e + g * n, js);
@@ -862,7 +864,7 @@ __device__ __forceinline__ void tiled_step(const LevelArgs<float>& a, const Chem
         // (unsigned: a domain-face entry is near INT_MIN; jl and j are then unused)
         const int jl = (int)(((unsigned)noD[d] - (unsigned)oct0) * 8u + (unsigned)(ic ^ (1 << d)));
         const int slot = PerDir ? lane : hbase[d] + lanes_below_mask(hball[d]);
-        const bool coarse_face = !inT && noD[d] < 0 && noD[d] > kNeiReflective;
+        const bool coarse_face = !Top && !inT && noD[d] < 0 && noD[d] > kNeiReflective;
         int iout, sout, cout;  // neighbour share, stride over groups, cancel flag: leaving face
         if (inT) {
           iout = bit * kTileCells + jl;
@@ -1003,8 +1005,13 @@ __global__ __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(4)))
   const KernArgs<float> args = kernargs<float>();
   __shared__ TileLds<G> lds;
   const int m = args.a.raster_m;
-  tiled_step<G, PerDir ? kPerDir : kListed>(args.a, args.K, m > 0 ? raster_tile(m) : xcd_block(args.a.nblocks), lds,
-                                            (int)threadIdx.x);
+  const int tile = m > 0 ? raster_tile(m) : xcd_block(args.a.nblocks);
+  if constexpr (PerDir)
+    tiled_step<G, kPerDir>(args.a, args.K, tile, lds, (int)threadIdx.x);
+  else if (args.a.ncoarse == 0)  // wave-uniform: the level's arguments
+    tiled_step<G, kListedTop>(args.a, args.K, tile, lds, (int)threadIdx.x);
+  else
+    tiled_step<G, kListed>(args.a, args.K, tile, lds, (int)threadIdx.x);
 }
 
 template <int G>
