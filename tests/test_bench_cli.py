@@ -52,3 +52,34 @@ def test_tuning_table_written_elsewhere_keeps_the_source_entries(tmp_path):
     t2 = tune.save({}, "w3", str(out), sched2, base=str(src))
     assert set(t2) == {"G3-complete-64", "tree-G3-64", "tree-G3-128"}
     assert tune.save({}, "w4", str(tmp_path / "fresh.json"), sched) == {"tree-G3-64": t["tree-G3-64"] | {"workload": "w4"}}
+
+
+def test_tuner_times_without_counters_and_restores_them():
+    """the tuner's timing helpers run with the device counters off, as the
+    bench's timed steps do, and give the context back as it was, also when
+    the timing raises"""
+    import pytest
+
+    from emma_amd import tune
+
+    class Ctx:
+        def __init__(self, on):
+            self.counters = on
+            self.seen = []
+
+        def enable_counters(self, on):
+            self.counters = bool(on)
+
+    @tune._uncounted
+    def timed(g, fail):
+        g.seen.append(g.counters)
+        if fail:
+            raise RuntimeError("timing failed")
+        return 1.0
+
+    for on in (True, False):
+        g = Ctx(on)
+        assert timed(g, False) == 1.0 and g.seen == [False] and g.counters is on
+        with pytest.raises(RuntimeError):
+            timed(g, True)
+        assert g.seen == [False, False] and g.counters is on
